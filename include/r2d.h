@@ -1,0 +1,223 @@
+/*
+ * resolve2d on MI355X -- C ABI of the GPU step.
+ *
+ * This is the drop-in boundary for resolve2d's step call.  Each entry point
+ * names the reference interface it replaces (paths relative to the resolve2d
+ * source tree).  Conventions:
+ *   - every call returns an r2d_status; 0 is success, so the reference's
+ *     "bool false on error" of src/wasm_root.zig is `status != 0`;
+ *   - worlds are opaque handles (the reference's wasm layer keeps one global
+ *     `?*Solver`, src/wasm_root.zig:15); many worlds live in one context and
+ *     can be stepped together;
+ *   - bodies are addressed by stable ids exactly like RigidBody.Id
+ *     (src/core/Bodies/RigidBody.zig:15), widened from u16 to u32 so worlds
+ *     may exceed 65,535 bodies; ids are never reused;
+ *   - iteration order (src/core/lib.zig:135, an AutoArrayHashMap in insertion
+ *     order, with swapRemove on removal) is preserved and exposed through
+ *     r2d_body_id_at, as src/wasm_root.zig:55-58 does;
+ *   - one host thread per context; device work is ordered on the context's
+ *     own HIP stream; body state stays resident in device memory and is
+ *     copied only when the host reads or edits it.
+ */
+#ifndef R2D_H
+#define R2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum r2d_status {
+    R2D_OK = 0,
+    R2D_E_INVALID_ARG = 1,   /* null handle, bad enum, zero-sized shape ...            */
+    R2D_E_INVALID_ID = 2,    /* error.InvalidRigidBodyId (Constraints/DistanceJoint.zig:44-47 etc.),
+                                error.NoSuchIdExists (src/core/lib.zig:309)               */
+    R2D_E_OOM = 3,           /* error.OutOfMemory (host or device allocation)          */
+    R2D_E_NONFINITE = 4,     /* a body's AABB is not finite: the reference's @intFromFloat
+                                in SpatialHash.zig:93-96 is undefined there; the world is
+                                left unstepped                                           */
+    R2D_E_DEVICE = 5,        /* HIP runtime failure                                      */
+    R2D_E_NO_DEVICE = 6,     /* no usable GPU                                            */
+    R2D_E_RANGE = 7,         /* index out of range (r2d_body_id_at), too many cells     */
+    R2D_E_CAPACITY = 8       /* request larger than a fixed limit                        */
+} r2d_status;
+
+typedef struct r2d_ctx r2d_ctx;
+typedef struct r2d_world r2d_world;
+
+/* EntityFactory.BodyOptions (src/core/lib.zig:35-46) */
+typedef struct r2d_body_options {
+    float pos_x, pos_y;
+    float vel_x, vel_y;      /* default 0 */
+    float angle;             /* default 0 */
+    float omega;             /* default 0 */
+    float mu;                /* default 0.5 */
+    int32_t mass_kind;       /* 0: mass_value is a density (kg/m^2), 1: a mass (kg) */
+    float mass_value;
+} r2d_body_options;
+
+/* Constraint.Parameters (src/core/Constraints/Constraint.zig:27-31):
+ * defaults power_max = +inf, power_min = -inf, beta = 10 */
+typedef struct r2d_constraint_params {
+    float power_max, power_min, beta;
+} r2d_constraint_params;
+
+/* Everything src/wasm_root.zig's getters expose for one body (:88-251). */
+typedef struct r2d_body_state {
+    uint32_t id;
+    uint32_t type;           /* 0 disc, 1 rectangle (RigidBody.Type) */
+    uint32_t is_static;
+    uint32_t num_normals;    /* 1 disc, 4 rectangle */
+    float pos_x, pos_y;
+    float mom_x, mom_y;
+    float force_x, force_y;
+    float mass;
+    float angle;
+    float ang_momentum;
+    float torque;
+    float inertia;
+    float mu;
+    float aabb_x, aabb_y, aabb_half_width, aabb_half_height;
+    float radius;            /* discs */
+    float width, height;     /* rectangles */
+} r2d_body_state;
+
+/* One contact manifold as src/core/collision.zig:54-71 stores it. */
+typedef struct r2d_manifold_point {
+    uint32_t valid;
+    float acc_pn, acc_pt;
+    float ref_r_x, ref_r_y, inc_r_x, inc_r_y;
+    float pos_x, pos_y;
+    float depth, original_depth;
+    float mass_n, mass_t;
+} r2d_manifold_point;
+
+typedef struct r2d_manifold_state {
+    uint32_t ref_id, inc_id;
+    float normal_x, normal_y, tangent_x, tangent_y;
+    float friction;
+    r2d_manifold_point points[2];
+} r2d_manifold_state;
+
+/* --------------------------------------------------------------- context */
+/* Creates a context on HIP device `device` with its own stream. */
+int r2d_ctx_create(int device, r2d_ctx** out);
+int r2d_ctx_destroy(r2d_ctx* ctx);
+/* Waits for all device work of the context. */
+int r2d_ctx_sync(r2d_ctx* ctx);
+
+/* ----------------------------------------------------------------- world */
+/* Solver.init (src/core/lib.zig:145-157).  Like the reference, the two hash
+ * arguments are stored but the step uses cell width 4.0 and 2*N buckets
+ * (lib.zig:254-255) unless r2d_set_hash_params selects another mode. */
+int r2d_world_create(r2d_ctx* ctx, float spatialhash_cell_width, size_t spatialhash_table_size_mult,
+                     r2d_world** out);
+/* Solver.deinit (src/core/lib.zig:159-179). */
+int r2d_world_destroy(r2d_world* w);
+/* Solver.clear (src/core/lib.zig:181-187): drops bodies, constraints, force
+ * generators, manifolds and excluded pairs; the id counter keeps counting. */
+int r2d_world_clear(r2d_world* w);
+
+/* EntityFactory.makeDiscBody (src/core/lib.zig:73-82). */
+int r2d_make_disc(r2d_world* w, const r2d_body_options* bo, float radius, uint32_t* id_out);
+/* EntityFactory.makeRectangleBody (src/core/lib.zig:84-93). */
+int r2d_make_rect(r2d_world* w, const r2d_body_options* bo, float width, float height, uint32_t* id_out);
+/* `body_unwrap().static = flag` (src/examples/0_1_car_platformer.zig:25, demos/native/src/main.zig). */
+int r2d_set_static(r2d_world* w, uint32_t id, int is_static);
+/* Solver.removeRigidBody (src/core/lib.zig:308-310): swapRemove, the last body
+ * takes the removed body's iteration slot.  R2D_E_INVALID_ID if absent. */
+int r2d_remove_body(r2d_world* w, uint32_t id);
+
+/* EntityFactory.makeDownwardsGravity (src/core/lib.zig:95-98). */
+int r2d_make_gravity(r2d_world* w, float g);
+/* Anchored spring force generator on one body: F += -k (x - anchor) - c v.
+ * Extension: the reference's ForceGen.Type has only downwards_gravity
+ * (src/core/Forces/ForceGen.zig:12-14); this fits its per-body apply(). */
+int r2d_make_spring(r2d_world* w, uint32_t id, float k, float c, float anchor_x, float anchor_y);
+
+/* EntityFactory.makeDistanceJoint (src/core/lib.zig:106-110). */
+int r2d_make_distance_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id1, uint32_t id2,
+                            float target_distance, size_t* handle_out);
+/* EntityFactory.makeOffsetDistanceJoint (src/core/lib.zig:100-104). */
+int r2d_make_offset_distance_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id1, uint32_t id2,
+                                   float r1_x, float r1_y, float r2_x, float r2_y, float target_distance,
+                                   size_t* handle_out);
+/* EntityFactory.makeFixedPositionJoint (src/core/lib.zig:112-116). */
+int r2d_make_fixed_position_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id, float target_x,
+                                  float target_y, size_t* handle_out);
+/* EntityFactory.makeMotorJoint (src/core/lib.zig:118-122). */
+int r2d_make_motor_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id, float target_omega,
+                         size_t* handle_out);
+/* EntityFactory.excludeCollisionPair (src/core/lib.zig:124-129). */
+int r2d_exclude_collision_pair(r2d_world* w, uint32_t id1, uint32_t id2);
+
+/* ------------------------------------------------------------------ step */
+/* Solver.process (src/core/lib.zig:189-251), the hot path: broadphase and
+ * narrowphase once, then sub_steps x (forces, integration, pre-step,
+ * collision_iters x (constraints, manifolds), position update).
+ * R2D_E_INVALID_ID if a constraint names a removed body (the reference
+ * raises the same error from Constraint.solve; here the world is left
+ * unstepped). */
+int r2d_process(r2d_world* w, float dt, size_t sub_steps, size_t collision_iters);
+
+/* Steps n independent worlds of one context together, `steps` times, as one
+ * device launch sequence per step.  Equivalent to calling r2d_process on
+ * every world `steps` times.  The first failing world's status is returned;
+ * other worlds are stepped regardless. */
+int r2d_process_batch(r2d_ctx* ctx, r2d_world* const* worlds, size_t n, float dt, size_t sub_steps,
+                      size_t collision_iters, size_t steps);
+
+/* ----------------------------------------------------------- state I/O */
+/* solverGetNumBodies (src/wasm_root.zig:51-53). */
+int r2d_num_bodies(r2d_world* w, size_t* n_out);
+/* solverGetBodyIdBasedOnIter (src/wasm_root.zig:55-58). */
+int r2d_body_id_at(r2d_world* w, size_t iter_index, uint32_t* id_out);
+/* Getters of src/wasm_root.zig:88-251 for one body. */
+int r2d_get_body(r2d_world* w, uint32_t id, r2d_body_state* out);
+/* Setters of src/wasm_root.zig:156-224 (momentum, force, angular momentum,
+ * torque) plus position and angle, which callers write through props
+ * (demos/native/src/main.zig:114-135).  Shape, mass and id are ignored. */
+int r2d_set_body(r2d_world* w, uint32_t id, const r2d_body_state* in);
+/* All bodies in iteration order; *n_out = count (written even if cap is short). */
+int r2d_get_bodies(r2d_world* w, r2d_body_state* out, size_t cap, size_t* n_out);
+/* The manifolds of the last step in the reference's order (Solver.manifolds,
+ * read by demos/native/src/Renderer.zig:84-167). */
+int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t cap, size_t* n_out);
+
+/* ------------------------------------------------------------ options */
+/* Spatial-hash parameters.  mode 0 (default): the reference's hard-coded
+ * cell 4.0 and 2*N buckets (src/core/lib.zig:254-255), bit-exact pair set.
+ * mode 1: the given cell width and table_mult*N buckets (the roadmap's
+ * "wire solver spatial-hash configuration", README.md:62-63).  Discovery
+ * order, and so results, can differ from mode 0. */
+int r2d_set_hash_params(r2d_world* w, int mode, float cell_width, size_t table_mult);
+/* Solver schedule.  0 (default): the reference's sequential Gauss-Seidel
+ * order, reproduced exactly.  1: graph-coloured batches (same equations,
+ * another order; results agree within tolerance, not bit for bit). */
+int r2d_set_schedule(r2d_world* w, int mode);
+
+/* --------------------------------------------------------- profiling */
+/* When on, every step records HIP events around its phases on the
+ * context's stream (broadphase, narrowphase, scheduling, solve). */
+int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
+/* Accumulated milliseconds and step count per phase: out_ms[4], *steps. */
+int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
+int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
+/* Totals of the last step of the context: entries, raw candidates,
+ * manifolds, solver items (out[4]). */
+int r2d_ctx_last_counts(r2d_ctx* ctx, uint64_t* out);
+
+/* Test hook with no counterpart in the reference: evaluates the device's
+ * scalar arithmetic on n inputs so tests can compare its bits with the CPU
+ * restatement.  op 0 sin, 1 cos (the reference's rotate2 trig,
+ * src/core/nmath.zig:129-136), 2 sqrt, 3 reciprocal, 4 division of input
+ * pairs (in[2i] / in[2i+1]; `in` holds 2n floats). */
+int r2d_debug_eval(r2d_ctx* ctx, int op, const float* in, float* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* R2D_H */

@@ -1,0 +1,930 @@
+// Host runtime of the C ABI in include/r2d.h: contexts, worlds (a host mirror
+// plus resident device state), and the batched step.
+//
+// Scene construction restates EntityFactory (src/core/lib.zig:13-130) and the
+// shape constructors (Bodies/Disc.zig:29-55, Bodies/Rectangle.zig:33-64) in
+// f32 with the reference's operation order, so the initial state has the same
+// bits as the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/r2d.h"
+#include "r2d_device.h"
+#include "r2d_kernels.h"
+
+using namespace r2d;
+
+#define R2D_EXPORT extern "C" __attribute__((visibility("default")))
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return R2D_E_DEVICE;  \
+    } while (0)
+
+namespace {
+
+// body arrays of the packed host/device block, each `cap` long
+enum BodyField {
+    F_PX = 0, F_PY, F_ANG, F_MX, F_MY, F_L, F_FX, F_FY, F_TQ, F_MASS, F_INERTIA, F_MU, F_RAD, F_HW, F_HH,
+    F_AX, F_AY, F_AHW, F_AHH, F_ID, F_FLAGS, F_NFIELDS
+};
+
+struct HJoint {
+    int32_t type;
+    uint32_t id1, id2;
+    float pmax, pmin, beta, r1x, r1y, r2x, r2y, target, tx, ty;
+};
+
+struct HGen {
+    int32_t type;
+    float gx, gy;
+    uint32_t id;
+    float k, c, ax, ay;
+};
+
+}  // namespace
+
+struct r2d_world {
+    r2d_ctx* ctx = nullptr;
+    uint32_t current_body_id = 0;
+    int32_t n = 0;
+    size_t cap = 0;
+    std::vector<uint32_t> block;       // F_NFIELDS * cap words (floats stored by bit pattern)
+    std::vector<float> width, height;  // rectangle shape (host only)
+    std::vector<int32_t> id2slot;
+    std::vector<HJoint> joints;
+    std::vector<HGen> gens;
+    std::vector<uint64_t> excl;        // canonical (min << 32 | max), sorted, unique
+    float cell_arg = 4.0f;
+    size_t mult_arg = 2;
+    int hash_mode = 0;
+    float hash_cell = 4.0f;
+    size_t hash_mult = 2;
+    int sched = 0;
+    // device
+    uint32_t* dblock = nullptr;
+    size_t dcap = 0;
+    Joint* djoints = nullptr;
+    size_t djcap = 0;
+    Gen* dgens = nullptr;
+    size_t dgcap = 0;
+    uint64_t* dexcl = nullptr;
+    size_t dxcap = 0;
+    bool host_valid = true;   // host mirror is current
+    bool dev_valid = false;   // device body block is current
+    bool topo_dirty = true;   // joints / generators / exclusions need upload
+    // manifolds of the last step this world took part in
+    uint64_t epoch = 0;
+    uint32_t batch_index = 0;
+
+    float* f(int field) { return reinterpret_cast<float*>(block.data() + (size_t)field * cap); }
+    uint32_t* u(int field) { return block.data() + (size_t)field * cap; }
+    int32_t slot_of(uint32_t id) const { return id < id2slot.size() ? id2slot[id] : -1; }
+};
+
+struct r2d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // step scratch
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    StepDev S{};
+    uint32_t cap_n = 0, cap_b = 0, cap_w = 0, cap_j = 0;
+    uint32_t cap_e = 0, cap_r = 0, cap_m = 0, cap_i = 0;
+    WorldDev* dworlds = nullptr;
+    size_t dworlds_cap = 0;
+    uint32_t* hcounters = nullptr;     // pinned
+    int32_t* hstatus = nullptr;        // pinned
+    size_t hstatus_cap = 0;
+    // profiling
+    bool profiling = false;
+    KernelTimer timer{};
+    double phase_ms[4] = {0, 0, 0, 0};
+    uint64_t phase_steps = 0;
+    uint64_t epoch = 0;
+    uint64_t last_counts[4] = {0, 0, 0, 0};
+    std::vector<r2d_world*> last_batch;
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ensure_world_device(r2d_world* w)
+{
+    if (w->dcap < w->cap || !w->dblock) {
+        if (w->dblock) HIPCHK(hipFree(w->dblock));
+        w->dblock = nullptr;
+        size_t c = std::max<size_t>(w->cap, 16);
+        if (hipMalloc(&w->dblock, (size_t)F_NFIELDS * c * 4) != hipSuccess) return R2D_E_OOM;
+        w->dcap = c;
+        w->dev_valid = false;
+    }
+    return R2D_OK;
+}
+
+// host mirror <-> device block; the layouts match when dcap == cap
+int upload_bodies(r2d_world* w)
+{
+    int rc = ensure_world_device(w);
+    if (rc) return rc;
+    if (w->dcap != w->cap) {
+        // grow host block to the device capacity so one copy moves everything
+        std::vector<uint32_t> nb((size_t)F_NFIELDS * w->dcap, 0u);
+        for (int fld = 0; fld < F_NFIELDS; ++fld)
+            std::memcpy(nb.data() + (size_t)fld * w->dcap, w->block.data() + (size_t)fld * w->cap,
+                        (size_t)w->n * 4);
+        w->block.swap(nb);
+        w->cap = w->dcap;
+        w->width.resize(w->cap);
+        w->height.resize(w->cap);
+    }
+    HIPCHK(hipMemcpyAsync(w->dblock, w->block.data(), (size_t)F_NFIELDS * w->cap * 4, hipMemcpyHostToDevice,
+                          w->ctx->stream));
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    w->dev_valid = true;
+    return R2D_OK;
+}
+
+int download_bodies(r2d_world* w)
+{
+    if (w->host_valid) return R2D_OK;
+    HIPCHK(hipMemcpyAsync(w->block.data(), w->dblock, (size_t)F_NFIELDS * w->cap * 4, hipMemcpyDeviceToHost,
+                          w->ctx->stream));
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    w->host_valid = true;
+    return R2D_OK;
+}
+
+// host edits need the current state; afterwards the device copy is stale
+int begin_edit(r2d_world* w)
+{
+    int rc = download_bodies(w);
+    if (rc) return rc;
+    w->dev_valid = false;
+    return R2D_OK;
+}
+
+void grow_host(r2d_world* w, size_t need)
+{
+    if (need <= w->cap) return;
+    size_t nc = std::max<size_t>(16, w->cap * 2);
+    while (nc < need) nc *= 2;
+    std::vector<uint32_t> nb((size_t)F_NFIELDS * nc, 0u);
+    for (int fld = 0; fld < F_NFIELDS; ++fld)
+        std::memcpy(nb.data() + (size_t)fld * nc, w->block.data() + (size_t)fld * w->cap, (size_t)w->n * 4);
+    w->block.swap(nb);
+    w->cap = nc;
+    w->width.resize(nc);
+    w->height.resize(nc);
+}
+
+// appendBody (lib.zig:66-71)
+uint32_t append_body(r2d_world* w)
+{
+    grow_host(w, (size_t)w->n + 1);
+    uint32_t id = w->current_body_id;
+    if (id >= w->id2slot.size()) w->id2slot.resize(std::max<size_t>(64, (size_t)id * 2 + 2), -1);
+    w->id2slot[id] = w->n;
+    w->u(F_ID)[w->n] = id;
+    w->n += 1;
+    w->current_body_id += 1;
+    return id;
+}
+
+int upload_topology(r2d_world* w)
+{
+    if (!w->topo_dirty) return R2D_OK;
+    hipStream_t st = w->ctx->stream;
+    // joints: ids resolved to current slots
+    std::vector<Joint> js(w->joints.size());
+    for (size_t i = 0; i < w->joints.size(); ++i) {
+        const HJoint& h = w->joints[i];
+        Joint& d = js[i];
+        std::memset(&d, 0, sizeof d);
+        d.type = h.type;
+        d.s1 = w->slot_of(h.id1);
+        d.s2 = (h.type <= 1) ? w->slot_of(h.id2) : -1;
+        d.pmax = h.pmax; d.pmin = h.pmin; d.beta = h.beta;
+        d.r1x = h.r1x; d.r1y = h.r1y; d.r2x = h.r2x; d.r2y = h.r2y;
+        d.target = h.target; d.tx = h.tx; d.ty = h.ty;
+    }
+    if (js.size() > w->djcap) {
+        if (w->djoints) HIPCHK(hipFree(w->djoints));
+        w->djcap = std::max<size_t>(16, js.size() * 2);
+        if (hipMalloc(&w->djoints, w->djcap * sizeof(Joint)) != hipSuccess) return R2D_E_OOM;
+    }
+    if (!js.empty()) HIPCHK(hipMemcpyAsync(w->djoints, js.data(), js.size() * sizeof(Joint), hipMemcpyHostToDevice, st));
+    std::vector<Gen> gs(w->gens.size());
+    for (size_t i = 0; i < w->gens.size(); ++i) {
+        const HGen& h = w->gens[i];
+        Gen& d = gs[i];
+        d.type = h.type; d.gx = h.gx; d.gy = h.gy;
+        d.slot = h.type == 1 ? w->slot_of(h.id) : -1;
+        d.k = h.k; d.c = h.c; d.ax = h.ax; d.ay = h.ay;
+    }
+    if (gs.size() > w->dgcap) {
+        if (w->dgens) HIPCHK(hipFree(w->dgens));
+        w->dgcap = std::max<size_t>(16, gs.size() * 2);
+        if (hipMalloc(&w->dgens, w->dgcap * sizeof(Gen)) != hipSuccess) return R2D_E_OOM;
+    }
+    if (!gs.empty()) HIPCHK(hipMemcpyAsync(w->dgens, gs.data(), gs.size() * sizeof(Gen), hipMemcpyHostToDevice, st));
+    if (w->excl.size() > w->dxcap) {
+        if (w->dexcl) HIPCHK(hipFree(w->dexcl));
+        w->dxcap = std::max<size_t>(16, w->excl.size() * 2);
+        if (hipMalloc(&w->dexcl, w->dxcap * sizeof(uint64_t)) != hipSuccess) return R2D_E_OOM;
+    }
+    if (!w->excl.empty())
+        HIPCHK(hipMemcpyAsync(w->dexcl, w->excl.data(), w->excl.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    w->topo_dirty = false;
+    return R2D_OK;
+}
+
+uint32_t table_size(const r2d_world* w)
+{
+    size_t mult = w->hash_mode == 0 ? 2 : w->hash_mult;
+    return (uint32_t)(mult * (size_t)w->n);
+}
+
+float hash_cell(const r2d_world* w) { return w->hash_mode == 0 ? 4.0f : w->hash_cell; }
+
+WorldDev make_desc(r2d_world* w)
+{
+    WorldDev d;
+    std::memset(&d, 0, sizeof d);
+    float* b = reinterpret_cast<float*>(w->dblock);
+    size_t c = w->dcap;
+    d.px = b + F_PX * c; d.py = b + F_PY * c; d.ang = b + F_ANG * c;
+    d.mx = b + F_MX * c; d.my = b + F_MY * c; d.L = b + F_L * c;
+    d.fx = b + F_FX * c; d.fy = b + F_FY * c; d.tq = b + F_TQ * c;
+    d.mass = b + F_MASS * c; d.inertia = b + F_INERTIA * c; d.mu = b + F_MU * c;
+    d.rad = b + F_RAD * c; d.hw = b + F_HW * c; d.hh = b + F_HH * c;
+    d.ax = b + F_AX * c; d.ay = b + F_AY * c; d.ahw = b + F_AHW * c; d.ahh = b + F_AHH * c;
+    d.id = w->dblock + F_ID * c; d.flags = w->dblock + F_FLAGS * c;
+    d.joints = w->djoints; d.gens = w->dgens; d.excl = w->dexcl;
+    d.nbodies = w->n;
+    d.njoints = (int32_t)w->joints.size();
+    d.ngens = (int32_t)w->gens.size();
+    d.nexcl = (int32_t)w->excl.size();
+    d.cell = hash_cell(w);
+    d.table = table_size(w);
+    d.sched = w->sched;
+    return d;
+}
+
+// carve the step scratch for the current capacities
+int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, uint32_t e, uint32_t r, uint32_t m,
+                   uint32_t it)
+{
+    if (n <= c->cap_n && b <= c->cap_b && wn <= c->cap_w && j <= c->cap_j && e <= c->cap_e && r <= c->cap_r &&
+        m <= c->cap_m && it <= c->cap_i && c->arena)
+        return R2D_OK;
+    auto grow = [](uint32_t have, uint32_t need) {
+        if (need <= have) return have;
+        uint64_t v = (uint64_t)need + need / 2 + 256;
+        return (uint32_t)std::min<uint64_t>(v, 0xffffff00ull);
+    };
+    c->cap_n = grow(c->cap_n, n); c->cap_b = grow(c->cap_b, b); c->cap_w = grow(c->cap_w, wn);
+    c->cap_j = grow(c->cap_j, j); c->cap_e = grow(c->cap_e, e); c->cap_r = grow(c->cap_r, r);
+    c->cap_m = grow(c->cap_m, m); c->cap_i = grow(c->cap_i, std::max(it, c->cap_m + c->cap_j));
+    size_t N = c->cap_n, B = c->cap_b, W = c->cap_w, E = c->cap_e, R = c->cap_r, M = c->cap_m, I = c->cap_i;
+    size_t off = 0;
+    std::vector<std::pair<void**, size_t>> parts;
+    StepDev& S = c->S;
+    auto add = [&](void** p, size_t bytes) { parts.push_back({p, bytes}); };
+    add((void**)&S.bworld, N * 4);
+    add((void**)&S.cx0, N * 8); add((void**)&S.cy0, N * 8);
+    add((void**)&S.cnx, N * 4); add((void**)&S.cny, N * 4);
+    add((void**)&S.ncell, (N + 1) * 4); add((void**)&S.cell_off, (N + 1) * 4);
+    add((void**)&S.cc, N * 4); add((void**)&S.cs, N * 4); add((void**)&S.sflag, N * 4);
+    add((void**)&S.bkt_cnt, (B + 1) * 4); add((void**)&S.bkt_off, (B + 1) * 4);
+    add((void**)&S.bkt_fill, (B + 1) * 4); add((void**)&S.bkt_len, (B + 1) * 4);
+    add((void**)&S.bkt_items, E * 4);
+    add((void**)&S.ent_body, E * 4); add((void**)&S.ent_bkt, E * 4);
+    add((void**)&S.ent_cnt, (E + 1) * 4); add((void**)&S.ent_off, (E + 1) * 4);
+    add((void**)&S.cand_a, R * 4); add((void**)&S.cand_b, R * 4);
+    add((void**)&S.cand_flag, (R + 1) * 4); add((void**)&S.cand_idx, (R + 1) * 4);
+    add((void**)&S.sat_nx, R * 4); add((void**)&S.sat_ny, R * 4); add((void**)&S.sat_info, R * 4);
+    add((void**)&S.m_ref, M * 4); add((void**)&S.m_inc, M * 4);
+    add((void**)&S.m_nx, M * 4); add((void**)&S.m_ny, M * 4); add((void**)&S.m_fric, M * 4);
+    add((void**)&S.m_prev1, M * 4); add((void**)&S.m_prev2, M * 4); add((void**)&S.m_valid, M * 4);
+    add((void**)&S.m_pt, (size_t)MP_NFIELDS * 2 * M * 4);
+    add((void**)&S.w_man_begin, (W + 1) * 4); add((void**)&S.w_item_base, (W + 1) * 4);
+    add((void**)&S.w_status, (W + 1) * 4);
+    add((void**)&S.it_r0, I * 4); add((void**)&S.it_r1, I * 4);
+    add((void**)&S.it_k0, I * 4); add((void**)&S.it_k1, I * 4);
+    add((void**)&S.deg, (N + 1) * 4); add((void**)&S.deg_off, (N + 1) * 4); add((void**)&S.deg_fill, (N + 1) * 4);
+    add((void**)&S.lists, 2 * I * 4);
+    add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
+    add((void**)&S.counters, 64 * 4);
+    add((void**)&S.partials, 1024 * 4);
+    for (auto& p : parts) off += align256(p.second);
+    if (c->arena) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->arena));
+        c->arena = nullptr;
+    }
+    if (hipMalloc(&c->arena, off) != hipSuccess) { c->arena = nullptr; c->arena_bytes = 0; return R2D_E_OOM; }
+    c->arena_bytes = off;
+    char* base = (char*)c->arena;
+    size_t o = 0;
+    for (auto& p : parts) { *p.first = base + o; o += align256(p.second); }
+    HIPCHK(hipMemsetAsync(c->arena, 0, off, c->stream));
+    S.cap_e = c->cap_e; S.cap_r = c->cap_r; S.cap_m = c->cap_m; S.cap_i = c->cap_i;
+    if (c->hstatus_cap < W + 1) {
+        if (c->hstatus) HIPCHK(hipHostFree(c->hstatus));
+        if (hipHostMalloc((void**)&c->hstatus, (W + 1) * 4) != hipSuccess) return R2D_E_OOM;
+        c->hstatus_cap = W + 1;
+    }
+    if (c->dworlds_cap < W) {
+        if (c->dworlds) HIPCHK(hipFree(c->dworlds));
+        if (hipMalloc(&c->dworlds, W * sizeof(WorldDev)) != hipSuccess) return R2D_E_OOM;
+        c->dworlds_cap = W;
+    }
+    return R2D_OK;
+}
+
+__global__ void k_zero_i32(int32_t* p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
+               size_t steps)
+{
+    if (nw == 0 || steps == 0) return R2D_OK;
+    // the reference raises InvalidRigidBodyId from the first constraint solve
+    bool solves = sub_steps > 0 && iters > 0;
+    for (size_t i = 0; i < nw; ++i) {
+        r2d_world* w = worlds[i];
+        if (!w || w->ctx != c) return R2D_E_INVALID_ARG;
+        if (solves)
+            for (const HJoint& h : w->joints)
+                if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
+    }
+    uint64_t N = 0, B = 0, J = 0;
+    uint32_t maxn = 0;
+    std::vector<WorldDev> descs(nw);
+    for (size_t i = 0; i < nw; ++i) {
+        r2d_world* w = worlds[i];
+        int rc;
+        if (!w->dev_valid && (rc = upload_bodies(w))) return rc;
+        if ((rc = upload_topology(w))) return rc;
+        WorldDev d = make_desc(w);
+        d.body_base = (uint32_t)N;
+        d.bucket_base = (uint32_t)B;
+        d.joint_base = (uint32_t)J;
+        descs[i] = d;
+        N += (uint64_t)w->n;
+        B += d.table;
+        J += w->joints.size();
+        maxn = std::max<uint32_t>(maxn, (uint32_t)w->n);
+    }
+    if (N >= 0x7fffffffull || B >= 0x7fffffffull) return R2D_E_CAPACITY;
+    uint32_t est_e = std::max<uint32_t>(c->cap_e, (uint32_t)(4 * N + 1024));
+    uint32_t est_r = std::max<uint32_t>(c->cap_r, (uint32_t)(8 * N + 1024));
+    uint32_t est_m = std::max<uint32_t>(c->cap_m, (uint32_t)(4 * N + 1024));
+    int rc = ensure_scratch(c, (uint32_t)N, (uint32_t)B, (uint32_t)nw, (uint32_t)J, est_e, est_r, est_m,
+                            (uint32_t)J + est_m);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->dworlds, descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
+    bool lds = solve_lds_bytes((int)maxn) <= 160 * 1024 - 1024;
+    unsigned threads = maxn <= 512 ? 256 : 1024;
+    if (const char* e = std::getenv("R2D_SOLVE_THREADS")) {
+        int v = std::atoi(e);
+        if (v >= 64 && v <= 1024 && v % 64 == 0) threads = (unsigned)v;
+    }
+    int first_err = R2D_OK;
+    size_t done = 0;
+    for (int attempt = 0; done < steps; ++attempt) {
+        if (attempt > 16) return R2D_E_CAPACITY;
+        StepDev S = c->S;
+        S.nb = (uint32_t)N; S.nworlds = (uint32_t)nw; S.nbuckets = (uint32_t)B; S.njoints = (uint32_t)J;
+        S.worlds = c->dworlds;
+        HIPCHK(hipMemsetAsync(S.counters, 0, 64 * 4, c->stream));
+        size_t todo = steps - done;
+        for (size_t s = 0; s < todo; ++s) {
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, c->stream, S.w_status, (uint32_t)nw + 1);
+            KernelTimer* tm = c->profiling ? &c->timer : nullptr;
+            launch_step(S, c->stream, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, threads, tm);
+            if (tm) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                for (int p = 0; p < 4; ++p) {
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, c->timer.ev[p], c->timer.ev[p + 1]));
+                    c->phase_ms[p] += ms;
+                }
+                c->phase_steps += 1;
+            }
+        }
+        HIPCHK(hipMemcpyAsync(c->hcounters, S.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->hstatus, S.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        uint32_t committed = c->hcounters[C_STEPS];
+        done += committed;
+        for (size_t i = 0; i < nw; ++i) {
+            if (c->hstatus[i] != 0 && first_err == R2D_OK)
+                first_err = c->hstatus[i] == 4 ? R2D_E_NONFINITE : c->hstatus[i] == 5 ? R2D_E_DEVICE : R2D_E_RANGE;
+        }
+        for (int k = 0; k < 4; ++k) c->last_counts[k] = c->hcounters[k];
+        if (c->hcounters[C_OVERFLOW]) {
+            uint32_t e = std::max(c->hcounters[C_ENTRIES], c->cap_e);
+            uint32_t r = std::max(c->hcounters[C_RAW], c->cap_r);
+            uint32_t m = std::max(c->hcounters[C_MANIFOLDS], c->cap_m);
+            uint32_t it = std::max(c->hcounters[C_ITEMS], c->cap_i);
+            // a count is only known once the counts before it fit; grow the first short buffer
+            if (c->hcounters[C_OVERFLOW] & 1u) e = c->hcounters[C_ENTRIES] * 2;
+            if (c->hcounters[C_OVERFLOW] & 2u) r = c->hcounters[C_RAW] * 2;
+            if (c->hcounters[C_OVERFLOW] & 4u) m = c->hcounters[C_MANIFOLDS] * 2;
+            if (c->hcounters[C_OVERFLOW] & 8u) it = c->hcounters[C_ITEMS] * 2;
+            rc = ensure_scratch(c, (uint32_t)N, (uint32_t)B, (uint32_t)nw, (uint32_t)J, e + 1, r + 1, m + 1, it + 1);
+            if (rc) return rc;
+            continue;
+        }
+    }
+    c->epoch += 1;
+    c->last_batch.assign(worlds, worlds + nw);
+    for (size_t i = 0; i < nw; ++i) {
+        worlds[i]->host_valid = false;
+        worlds[i]->epoch = c->epoch;
+        worlds[i]->batch_index = (uint32_t)i;
+    }
+    return first_err;
+}
+
+void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
+{
+    std::memset(o, 0, sizeof *o);
+    o->id = w->u(F_ID)[s];
+    uint32_t fl = w->u(F_FLAGS)[s];
+    o->type = (fl & BF_RECT) ? 1u : 0u;
+    o->is_static = (fl & BF_STATIC) ? 1u : 0u;
+    o->num_normals = o->type ? 4u : 1u;
+    o->pos_x = w->f(F_PX)[s]; o->pos_y = w->f(F_PY)[s];
+    o->mom_x = w->f(F_MX)[s]; o->mom_y = w->f(F_MY)[s];
+    o->force_x = w->f(F_FX)[s]; o->force_y = w->f(F_FY)[s];
+    o->mass = w->f(F_MASS)[s]; o->angle = w->f(F_ANG)[s]; o->ang_momentum = w->f(F_L)[s];
+    o->torque = w->f(F_TQ)[s]; o->inertia = w->f(F_INERTIA)[s]; o->mu = w->f(F_MU)[s];
+    o->aabb_x = w->f(F_AX)[s]; o->aabb_y = w->f(F_AY)[s];
+    o->aabb_half_width = w->f(F_AHW)[s]; o->aabb_half_height = w->f(F_AHH)[s];
+    o->radius = w->f(F_RAD)[s];
+    o->width = w->width[s]; o->height = w->height[s];
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
+{
+    if (!out) return R2D_E_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return R2D_E_NO_DEVICE;
+    if (device < 0 || device >= ndev) return R2D_E_INVALID_ARG;
+    HIPCHK(hipSetDevice(device));
+    r2d_ctx* c = new (std::nothrow) r2d_ctx();
+    if (!c) return R2D_E_OOM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return R2D_E_DEVICE; }
+    if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
+    for (int i = 0; i < KT_N; ++i) HIPCHK(hipEventCreate(&c->timer.ev[i]));
+    set_solve_smem_limit();
+    *out = c;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->dworlds) (void)hipFree(c->dworlds);
+    if (c->hcounters) (void)hipHostFree(c->hcounters);
+    if (c->hstatus) (void)hipHostFree(c->hstatus);
+    for (int i = 0; i < KT_N; ++i) (void)hipEventDestroy(c->timer.ev[i]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_sync(r2d_ctx* c)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_world_create(r2d_ctx* c, float cell, size_t mult, r2d_world** out)
+{
+    if (!c || !out) return R2D_E_INVALID_ARG;
+    r2d_world* w = new (std::nothrow) r2d_world();
+    if (!w) return R2D_E_OOM;
+    w->ctx = c;
+    w->cell_arg = cell;
+    w->mult_arg = mult;
+    *out = w;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_world_destroy(r2d_world* w)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    (void)hipStreamSynchronize(w->ctx->stream);
+    if (w->dblock) (void)hipFree(w->dblock);
+    if (w->djoints) (void)hipFree(w->djoints);
+    if (w->dgens) (void)hipFree(w->dgens);
+    if (w->dexcl) (void)hipFree(w->dexcl);
+    delete w;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_world_clear(r2d_world* w)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    std::fill(w->id2slot.begin(), w->id2slot.end(), -1);
+    w->n = 0;
+    w->joints.clear();
+    w->gens.clear();
+    w->excl.clear();
+    w->topo_dirty = true;
+    w->epoch = 0;
+    return R2D_OK;
+}
+
+static int make_body_common(r2d_world* w, const r2d_body_options* bo, bool rect, float a, float b, uint32_t* id_out)
+{
+    if (!w || !bo) return R2D_E_INVALID_ARG;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    float mass;
+    float inertia;
+    if (!rect) {
+        // makeDiscBody (lib.zig:73-82), Disc.init (Disc.zig:29-55)
+        float r = a;
+        mass = bo->mass_kind ? bo->mass_value : (float)M_PI * r * r * bo->mass_value;
+        inertia = 0.5f * mass * r * r;
+    } else {
+        // makeRectangleBody (lib.zig:84-93), Rectangle.init (Rectangle.zig:33-64)
+        mass = bo->mass_kind ? bo->mass_value : a * b * bo->mass_value;
+        inertia = mass * (a * a + b * b) / 12;
+    }
+    uint32_t id = append_body(w);
+    int32_t s = w->n - 1;
+    w->f(F_PX)[s] = bo->pos_x; w->f(F_PY)[s] = bo->pos_y; w->f(F_ANG)[s] = bo->angle;
+    w->f(F_FX)[s] = 0.f; w->f(F_FY)[s] = 0.f; w->f(F_TQ)[s] = 0.f;
+    w->f(F_MASS)[s] = mass; w->f(F_INERTIA)[s] = inertia; w->f(F_MU)[s] = bo->mu;
+    w->f(F_MX)[s] = bo->vel_x * mass; w->f(F_MY)[s] = bo->vel_y * mass;
+    w->f(F_L)[s] = bo->omega * inertia;
+    w->u(F_FLAGS)[s] = rect ? BF_RECT : 0u;
+    if (!rect) {
+        w->f(F_RAD)[s] = a; w->f(F_HW)[s] = 0.f; w->f(F_HH)[s] = 0.f;
+        w->width[s] = 0.f; w->height[s] = 0.f;
+        w->f(F_AX)[s] = bo->pos_x; w->f(F_AY)[s] = bo->pos_y; w->f(F_AHW)[s] = a; w->f(F_AHH)[s] = a;
+    } else {
+        float hw = a / 2, hh = b / 2;
+        w->f(F_RAD)[s] = 0.f; w->f(F_HW)[s] = hw; w->f(F_HH)[s] = hh;
+        w->width[s] = a; w->height[s] = b;
+        // Rectangle.updateAABB (Rectangle.zig:71-86)
+        v2 lv[4] = {mk(-hw, -hh), mk(-hw, hh), mk(hw, hh), mk(hw, -hh)};
+        float c = cosf_r(bo->angle), sn = sinf_r(bo->angle);
+        float ew = 0.f, eh = 0.f;
+        for (int k = 0; k < 4; ++k) {
+            v2 r = rotcs(lv[k], c, sn);
+            if (r.x > ew) ew = r.x;
+            if (r.y > eh) eh = r.y;
+        }
+        w->f(F_AX)[s] = bo->pos_x; w->f(F_AY)[s] = bo->pos_y; w->f(F_AHW)[s] = ew; w->f(F_AHH)[s] = eh;
+    }
+    if (id_out) *id_out = id;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_make_disc(r2d_world* w, const r2d_body_options* bo, float radius, uint32_t* id_out)
+{
+    return make_body_common(w, bo, false, radius, 0.f, id_out);
+}
+
+R2D_EXPORT int r2d_make_rect(r2d_world* w, const r2d_body_options* bo, float width, float height, uint32_t* id_out)
+{
+    return make_body_common(w, bo, true, width, height, id_out);
+}
+
+R2D_EXPORT int r2d_set_static(r2d_world* w, uint32_t id, int is_static)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    int32_t s = w->slot_of(id);
+    if (s < 0) return R2D_E_INVALID_ID;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    uint32_t& fl = w->u(F_FLAGS)[s];
+    fl = is_static ? (fl | BF_STATIC) : (fl & ~BF_STATIC);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_remove_body(r2d_world* w, uint32_t id)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    int32_t s = w->slot_of(id);
+    if (s < 0) return R2D_E_INVALID_ID;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    int32_t last = w->n - 1;
+    if (s != last) {
+        for (int fld = 0; fld < F_NFIELDS; ++fld) w->block[(size_t)fld * w->cap + s] = w->block[(size_t)fld * w->cap + last];
+        w->width[s] = w->width[last];
+        w->height[s] = w->height[last];
+        w->id2slot[w->u(F_ID)[s]] = s;
+    }
+    w->id2slot[id] = -1;
+    w->n -= 1;
+    w->topo_dirty = true;   // joint / spring slots move
+    w->epoch = 0;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_make_gravity(r2d_world* w, float g)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    HGen h{};
+    h.type = 0; h.gx = 0.0f; h.gy = -g;
+    w->gens.push_back(h);
+    w->topo_dirty = true;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_make_spring(r2d_world* w, uint32_t id, float k, float c, float ax, float ay)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    if (w->slot_of(id) < 0) return R2D_E_INVALID_ID;
+    HGen h{};
+    h.type = 1; h.id = id; h.k = k; h.c = c; h.ax = ax; h.ay = ay;
+    w->gens.push_back(h);
+    w->topo_dirty = true;
+    return R2D_OK;
+}
+
+static const r2d_constraint_params kDefaultParams = {INFINITY, -INFINITY, 10.0f};
+
+static int add_joint(r2d_world* w, const r2d_constraint_params* p, HJoint h, size_t* handle)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    if (!p) p = &kDefaultParams;
+    h.pmax = p->power_max; h.pmin = p->power_min; h.beta = p->beta;
+    w->joints.push_back(h);
+    w->topo_dirty = true;
+    if (handle) *handle = w->joints.size() - 1;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_make_distance_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id1, uint32_t id2,
+                                       float target, size_t* handle)
+{
+    HJoint h{};
+    h.type = 0; h.id1 = id1; h.id2 = id2; h.target = target;
+    return add_joint(w, p, h, handle);
+}
+
+R2D_EXPORT int r2d_make_offset_distance_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id1,
+                                              uint32_t id2, float r1x, float r1y, float r2x, float r2y,
+                                              float target, size_t* handle)
+{
+    HJoint h{};
+    h.type = 1; h.id1 = id1; h.id2 = id2; h.r1x = r1x; h.r1y = r1y; h.r2x = r2x; h.r2y = r2y; h.target = target;
+    return add_joint(w, p, h, handle);
+}
+
+R2D_EXPORT int r2d_make_fixed_position_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id, float tx,
+                                             float ty, size_t* handle)
+{
+    HJoint h{};
+    h.type = 2; h.id1 = id; h.tx = tx; h.ty = ty;
+    return add_joint(w, p, h, handle);
+}
+
+R2D_EXPORT int r2d_make_motor_joint(r2d_world* w, const r2d_constraint_params* p, uint32_t id, float omega,
+                                    size_t* handle)
+{
+    HJoint h{};
+    h.type = 3; h.id1 = id; h.target = omega;
+    return add_joint(w, p, h, handle);
+}
+
+R2D_EXPORT int r2d_exclude_collision_pair(r2d_world* w, uint32_t a, uint32_t b)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    uint64_t k = a < b ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
+    auto it = std::lower_bound(w->excl.begin(), w->excl.end(), k);
+    if (it == w->excl.end() || *it != k) w->excl.insert(it, k);
+    w->topo_dirty = true;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_process(r2d_world* w, float dt, size_t sub_steps, size_t iters)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    return step_batch(w->ctx, &w, 1, dt, sub_steps, iters, 1);
+}
+
+R2D_EXPORT int r2d_process_batch(r2d_ctx* c, r2d_world* const* worlds, size_t n, float dt, size_t sub_steps,
+                                 size_t iters, size_t steps)
+{
+    if (!c || (!worlds && n)) return R2D_E_INVALID_ARG;
+    return step_batch(c, worlds, n, dt, sub_steps, iters, steps);
+}
+
+R2D_EXPORT int r2d_num_bodies(r2d_world* w, size_t* n)
+{
+    if (!w || !n) return R2D_E_INVALID_ARG;
+    *n = (size_t)w->n;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_body_id_at(r2d_world* w, size_t i, uint32_t* id)
+{
+    if (!w || !id) return R2D_E_INVALID_ARG;
+    if (i >= (size_t)w->n) return R2D_E_RANGE;
+    *id = w->u(F_ID)[i];
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_get_body(r2d_world* w, uint32_t id, r2d_body_state* out)
+{
+    if (!w || !out) return R2D_E_INVALID_ARG;
+    int32_t s = w->slot_of(id);
+    if (s < 0) return R2D_E_INVALID_ID;
+    int rc = download_bodies(w);
+    if (rc) return rc;
+    fill_state(w, s, out);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_set_body(r2d_world* w, uint32_t id, const r2d_body_state* in)
+{
+    if (!w || !in) return R2D_E_INVALID_ARG;
+    int32_t s = w->slot_of(id);
+    if (s < 0) return R2D_E_INVALID_ID;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    w->f(F_PX)[s] = in->pos_x; w->f(F_PY)[s] = in->pos_y;
+    w->f(F_MX)[s] = in->mom_x; w->f(F_MY)[s] = in->mom_y;
+    w->f(F_FX)[s] = in->force_x; w->f(F_FY)[s] = in->force_y;
+    w->f(F_ANG)[s] = in->angle; w->f(F_L)[s] = in->ang_momentum; w->f(F_TQ)[s] = in->torque;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_get_bodies(r2d_world* w, r2d_body_state* out, size_t cap, size_t* n)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    int rc = download_bodies(w);
+    if (rc) return rc;
+    if (n) *n = (size_t)w->n;
+    if (!out) return R2D_OK;
+    size_t m = std::min(cap, (size_t)w->n);
+    for (size_t i = 0; i < m; ++i) fill_state(w, (int32_t)i, &out[i]);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t cap, size_t* n)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    r2d_ctx* c = w->ctx;
+    if (w->epoch == 0 || w->epoch != c->epoch) {
+        if (n) *n = 0;
+        return w->epoch == 0 ? R2D_OK : R2D_E_RANGE;
+    }
+    StepDev& S = c->S;
+    uint32_t b[2];
+    HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 8, hipMemcpyDeviceToHost));
+    size_t cnt = b[1] - b[0];
+    if (n) *n = cnt;
+    if (!out) return R2D_OK;
+    size_t m = std::min(cap, cnt);
+    if (m == 0) return R2D_OK;
+    std::vector<uint32_t> ref(m), inc(m), valid(m);
+    std::vector<float> nx(m), ny(m), fr(m), pt((size_t)MP_NFIELDS * 2 * m);
+    HIPCHK(hipMemcpy(ref.data(), S.m_ref + b[0], m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(inc.data(), S.m_inc + b[0], m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(valid.data(), S.m_valid + b[0], m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nx.data(), S.m_nx + b[0], m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ny.data(), S.m_ny + b[0], m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fr.data(), S.m_fric + b[0], m * 4, hipMemcpyDeviceToHost));
+    for (int f = 0; f < MP_NFIELDS * 2; ++f)
+        HIPCHK(hipMemcpy(pt.data() + (size_t)f * m, S.m_pt + (size_t)f * S.cap_m + b[0], m * 4, hipMemcpyDeviceToHost));
+    int rc = download_bodies(w);
+    if (rc) return rc;
+    uint32_t wb[1];
+    // batch body base of this world = first body's batch index
+    HIPCHK(hipMemcpy(wb, &c->dworlds[w->batch_index].body_base, 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < m; ++i) {
+        r2d_manifold_state& o = out[i];
+        std::memset(&o, 0, sizeof o);
+        o.ref_id = w->u(F_ID)[ref[i] - wb[0]];
+        o.inc_id = w->u(F_ID)[inc[i] - wb[0]];
+        o.normal_x = nx[i]; o.normal_y = ny[i];
+        o.tangent_x = ny[i]; o.tangent_y = -nx[i];
+        o.friction = fr[i];
+        for (int p = 0; p < 2; ++p) {
+            r2d_manifold_point& q = o.points[p];
+            q.valid = (valid[i] >> p) & 1u;
+            if (!q.valid) continue;
+            auto F = [&](int fld) { return pt[((size_t)fld * 2 + p) * m + i]; };
+            q.acc_pn = F(MP_ACC_PN); q.acc_pt = F(MP_ACC_PT);
+            q.ref_r_x = F(MP_R1X); q.ref_r_y = F(MP_R1Y); q.inc_r_x = F(MP_R2X); q.inc_r_y = F(MP_R2Y);
+            q.pos_x = F(MP_POSX); q.pos_y = F(MP_POSY);
+            q.depth = F(MP_DEPTH); q.original_depth = F(MP_ODEPTH);
+            q.mass_n = F(MP_MASS_N); q.mass_t = F(MP_MASS_T);
+        }
+    }
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_set_hash_params(r2d_world* w, int mode, float cell, size_t mult)
+{
+    if (!w || (mode != 0 && mode != 1)) return R2D_E_INVALID_ARG;
+    if (mode == 1 && (!(cell > 0.0f) || mult == 0)) return R2D_E_INVALID_ARG;
+    w->hash_mode = mode;
+    w->hash_cell = cell;
+    w->hash_mult = mult;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_set_schedule(r2d_world* w, int mode)
+{
+    if (!w || mode != 0) return R2D_E_INVALID_ARG;
+    w->sched = mode;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_set_profiling(r2d_ctx* c, int on)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    c->profiling = on != 0;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_phase_times(r2d_ctx* c, double* out_ms, uint64_t* steps)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    if (out_ms) for (int i = 0; i < 4; ++i) out_ms[i] = c->phase_ms[i];
+    if (steps) *steps = c->phase_steps;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    for (int i = 0; i < 4; ++i) c->phase_ms[i] = 0;
+    c->phase_steps = 0;
+    return R2D_OK;
+}
+
+__global__ void k_debug_eval(int op, const float* in, float* out, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float x = in[i];
+        float r;
+        switch (op) {
+        case 0: r = sinf_r(x); break;
+        case 1: r = cosf_r(x); break;
+        case 2: r = sqrt_rn(x); break;
+        case 3: r = 1.0f / x; break;
+        default: r = in[2 * i] / in[2 * i + 1]; break;
+        }
+        out[i] = r;
+    }
+}
+
+R2D_EXPORT int r2d_debug_eval(r2d_ctx* c, int op, const float* in, float* out, size_t n)
+{
+    if (!c || !in || !out || op < 0 || op > 4) return R2D_E_INVALID_ARG;
+    if (n == 0) return R2D_OK;
+    size_t nin = op == 4 ? 2 * n : n;
+    float *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, nin * 4) != hipSuccess) return R2D_E_OOM;
+    if (hipMalloc(&dout, n * 4) != hipSuccess) { (void)hipFree(din); return R2D_E_OOM; }
+    HIPCHK(hipMemcpyAsync(din, in, nin * 4, hipMemcpyHostToDevice, c->stream));
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_debug_eval, dim3(blocks), dim3(256), 0, c->stream, op, din, dout, (uint32_t)n);
+    HIPCHK(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_last_counts(r2d_ctx* c, uint64_t* out)
+{
+    if (!c || !out) return R2D_E_INVALID_ARG;
+    for (int i = 0; i < 4; ++i) out[i] = c->last_counts[i];
+    return R2D_OK;
+}

@@ -1,0 +1,121 @@
+// Device-side layout of resolve2d worlds and of one batched step.
+//
+// A world keeps its bodies as structure-of-arrays in HBM, indexed by the
+// reference's iteration order (the AutoArrayHashMap order of lib.zig:135,
+// including swapRemove moves).  A batched step views several worlds as one
+// flat "batch body index" space: batch index = world.body_base + slot.
+#pragma once
+
+#include <stdint.h>
+
+#include "r2d_math.h"
+
+namespace r2d {
+
+enum : uint32_t {
+    BF_STATIC = 1u,    // RigidBody.static (RigidBody.zig:85)
+    BF_RECT = 2u,      // RigidBody.Type.rectangle
+};
+
+// per-body scheduling flag: the body orders the solver's items
+enum : uint32_t { SF_RESOURCE = 1u };
+
+// Constraint record (Constraint.zig:27-31 parameters + per-type fields)
+struct Joint {
+    int32_t type;      // 0 distance, 1 offset distance, 2 fixed position, 3 motor
+    int32_t s1, s2;    // world-local body slots (s2 unused for 2/3)
+    float pmax, pmin, beta;
+    float r1x, r1y, r2x, r2y;
+    float target;      // target distance / target omega
+    float tx, ty;      // fixed-position target
+    float pad[3];
+};
+
+// ForceGen record (ForceGen.zig:12-19); type 1 = anchored spring extension
+struct Gen {
+    int32_t type;
+    int32_t slot;      // spring body slot, -1 when its body is gone
+    float gx, gy;      // DownwardsGravity.g_vec (DownwardsGravity.zig:22)
+    float k, c, ax, ay;
+};
+
+struct WorldDev {
+    // bodies (SoA, capacity >= nbodies)
+    float *px, *py, *ang, *mx, *my, *L, *fx, *fy, *tq;
+    float *mass, *inertia, *mu, *rad, *hw, *hh;
+    float *ax, *ay, *ahw, *ahh;      // stale AABB (aabb.zig:6-8)
+    uint32_t *id, *flags;
+    const Joint* joints;
+    const Gen* gens;
+    const uint64_t* excl;            // sorted (min_id << 32 | max_id)
+    int32_t nbodies, njoints, ngens, nexcl;
+    float cell;                      // spatial-hash cell width
+    uint32_t table;                  // spatial-hash table size (buckets)
+    uint32_t body_base, bucket_base, joint_base;
+    int32_t sched;                   // 0 reference order, 1 coloured
+};
+
+// device counters
+enum : int {
+    C_ENTRIES = 0,     // spatial-hash cell entries E
+    C_RAW = 1,         // raw candidates (a, b, cell) R
+    C_MANIFOLDS = 2,   // manifolds M
+    C_ITEMS = 3,       // solver items (joints + manifolds)
+    C_OVERFLOW = 4,    // capacity overflow flag (bit per buffer)
+    C_STEPS = 5,       // committed steps
+    C_COUNT = 8
+};
+
+// per-step scratch, sized by capacities chosen on the host
+struct StepDev {
+    uint32_t nb;             // batch bodies N
+    uint32_t nworlds;        // W
+    uint32_t nbuckets;       // B
+    uint32_t njoints;        // batch joints
+    uint32_t cap_e, cap_r, cap_m, cap_i;
+    const WorldDev* worlds;
+    uint32_t* bworld;        // [N] world index of each batch body
+    // broadphase, per body
+    int64_t *cx0, *cy0;      // first cell
+    uint32_t *cnx, *cny;     // cells per row / rows
+    uint32_t* ncell;         // [N+1] cells per body -> exclusive offsets
+    uint32_t* cell_off;      // [N+1]
+    float *cc, *cs;          // cos / sin of the angle at step start
+    uint32_t* sflag;         // scheduling flags
+    // spatial hash buckets
+    uint32_t *bkt_cnt, *bkt_off, *bkt_fill, *bkt_len;  // [B+1]
+    uint32_t* bkt_items;     // [cap_e]
+    // entries -> raw candidates
+    uint32_t *ent_body, *ent_bkt;  // [cap_e]
+    uint32_t *ent_cnt, *ent_off;   // [cap_e+1]
+    uint32_t *cand_a, *cand_b;     // [cap_r]
+    uint32_t *cand_flag, *cand_idx;      // [cap_r+1]
+    float *sat_nx, *sat_ny;
+    uint32_t* sat_info;      // normal id | ref-is-lower-slot << 2 | collides << 3
+    // manifolds (discovery order)
+    uint32_t *m_ref, *m_inc;           // batch body indices
+    float *m_nx, *m_ny, *m_fric, *m_prev1, *m_prev2;
+    uint32_t* m_valid;                 // bit p: point p present
+    float* m_pt;                       // [MP_NFIELDS fields][2 points][cap_m]
+    // per world
+    uint32_t *w_man_begin, *w_item_base;   // [W+1]
+    int32_t* w_status;
+    // solver items (joints then manifolds, per world)
+    int32_t *it_r0, *it_r1;
+    uint32_t *it_k0, *it_k1;
+    uint32_t *deg, *deg_off, *deg_fill;    // [N+1]
+    uint32_t* lists;                       // [2*cap_i]
+    // global-memory solver state (worlds too large for LDS)
+    float *g_im, *g_ii;
+    uint32_t* g_cnt;
+    uint32_t* counters;
+    uint32_t* partials;                    // scan scratch
+};
+
+// manifold point fields: m_pt[(field * 2 + point) * cap_m + m]
+enum : int {
+    MP_ACC_PN = 0, MP_ACC_PT, MP_R1X, MP_R1Y, MP_R2X, MP_R2Y, MP_POSX, MP_POSY,
+    MP_DEPTH, MP_ODEPTH, MP_MASS_N, MP_MASS_T, MP_NFIELDS
+};
+
+}  // namespace r2d
