@@ -1187,8 +1187,8 @@ void launch_step(const StepDev& S, hipStream_t st, float dt, uint32_t sub_steps,
     } else {
         hipLaunchKernelGGL(k_solve<false>, dim3(S.nworlds), dim3(solve_threads), 0, st, S, dt, sub_steps, iters);
     }
-    hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
     T(KT_END);
+    hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
 }
 
 void set_solve_smem_limit()
