@@ -25,6 +25,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def checked(rc, what):
+    from resolve2d_amd import _native
+    _native.check(rc, what)
+
+
 def build_worlds(ctx, n_worlds, seed0):
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld, apply_scene
@@ -76,6 +85,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--single-steps", type=int, default=300)
+    ap.add_argument("--tuned-hash", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: worlds are independent and would be sharded whole across GPUs "
@@ -92,6 +104,7 @@ def main():
     setup_s = time.perf_counter() - t0
     dt, S, K = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
 
+    log(f"built {args.worlds} worlds, {n_bodies} bodies in {setup_s:.1f} s")
     for _ in range(args.warmup):
         ctx.process_batch_handles(handles, dt, S, K, 1)
     ctx.sync()
@@ -114,6 +127,7 @@ def main():
 
     value = n_bodies * args.steps / elapsed
     lat_ms = np.array(lat) * 1e3
+    log(f"batched: {args.steps} steps in {elapsed:.3f} s")
 
     # plain: the same scenes, one world per launch
     from resolve2d_amd.engine import GpuWorld, apply_scene
@@ -121,16 +135,17 @@ def main():
     for sc in specs[:args.plain_worlds]:
         w = GpuWorld(ctx)
         apply_scene(sc, w)
-        w.process(dt, S, K)
+        checked(w.process(dt, S, K), "plain step")
         ctx.sync()
         a = time.perf_counter()
         for _ in range(args.plain_steps):
-            w.process(dt, S, K)
+            checked(w.process(dt, S, K), "plain step")
         ctx.sync()
         plain_t += time.perf_counter() - a
         plain_body_steps += count_bodies(sc) * args.plain_steps
         w.close()
     plain_value = plain_body_steps / plain_t
+    log("plain done")
 
     # roofline of the dominant kernel (solve): algorithmic bytes per launch
     # = 112 B per body (pose, momentum, force, torque, mass, inertia, flags, shape read;
@@ -143,8 +158,35 @@ def main():
     solve_s = per_step["solve"] / 1e3
     achieved = solve_bytes / solve_s / 1e9 if solve_s > 0 else 0.0
 
+    # configuration 2: one 4,096-body world, full 300-step horizon, one world per launch,
+    # reference order vs the coloured schedule
+    single = {}
+    if not args.no_single:
+        for mode, label in ((0, "reference_order"), (1, "colored")):
+            w = GpuWorld(ctx)
+            apply_scene(scenes.mixed(4096, seed=1, hash_cell=2.0 if args.tuned_hash else None), w)
+            if mode:
+                w.set_schedule(1)
+            ctx.sync()
+            lat1 = []
+            a0 = time.perf_counter()
+            for i in range(args.single_steps):
+                a = time.perf_counter()
+                checked(w.process(dt, S, K), f"single-world step ({label})")
+                lat1.append(time.perf_counter() - a)
+                if i % 100 == 99:
+                    log(f"single world {label}: {i + 1} steps")
+            ctx.sync()
+            t1 = time.perf_counter() - a0
+            single[label] = {"value": 4097 * args.single_steps / t1, "unit": "body-steps/s",
+                             "ms_per_step": t1 / args.single_steps * 1e3,
+                             "p50_step_ms": float(np.percentile(np.array(lat1) * 1e3, 50)),
+                             "p99_step_ms": float(np.percentile(np.array(lat1) * 1e3, 99))}
+            w.close()
+
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline")
         rate, used, tcpu = cpu_baseline(specs, 60, args.cpu_budget)
         cpu = {"value": rate, "unit": "body-steps/s", "cores": 1, "kind": "port",
                "sample": f"oracle f32 restatement, first {used} worlds x 60 steps ({tcpu:.1f} s)"}
@@ -171,6 +213,7 @@ def main():
         "plain": {"value": plain_value, "unit": "body-steps/s",
                   "sample": f"first {args.plain_worlds} worlds, {args.plain_steps} steps each, one world per launch"},
         "batched_vs_plain": value / plain_value,
+        "single_world_4096": single,
         "phase_ms_per_step": per_step,
         "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "dominant_phase": dom},

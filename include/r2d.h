@@ -209,6 +209,10 @@ int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
  * manifolds, solver items (out[4]). */
 int r2d_ctx_last_counts(r2d_ctx* ctx, uint64_t* out);
 
+/* Text of the last failure on the calling thread (which HIP call, which
+ * world), for R2D_E_DEVICE and friends. */
+int r2d_last_error(char* buf, size_t n);
+
 /* Test hook with no counterpart in the reference: evaluates the device's
  * scalar arithmetic on n inputs so tests can compare its bits with the CPU
  * restatement.  op 0 sin, 1 cos (the reference's rotate2 trig,

@@ -45,7 +45,7 @@ def load():
             "get_bodies": (None, [P, ctypes.POINTER(D)]),
             "set_body": (I, [P, U32, D, D, D, D, D, D, D, D, D]),
             "get_manifolds": (None, [P, ctypes.POINTER(D)]),
-            "set_trace": (None, [P, I]), "num_trace": (I64, [P]),
+            "set_trace": (None, [P, I]), "num_trace": (I64, [P]), "set_schedule": (None, [P, I]),
             "get_trace": (None, [P, ctypes.POINTER(U32)]),
             "narrow_pair": (I, [P, U32, U32, ctypes.POINTER(D)]),
         }
@@ -118,6 +118,9 @@ class OracleWorld:
 
     def clear(self):
         self._f("clear")(self.h)
+
+    def set_schedule(self, mode):
+        self._f("set_schedule")(self.h, mode)
 
     def process(self, dt, sub_steps, iters):
         return self._f("process_")(self.h, dt, sub_steps, iters)

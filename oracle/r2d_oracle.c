@@ -207,6 +207,7 @@ EXPORT float or_cosf(float x) { return r2dref_cosf(x); }
         }                                                                                               \
     }                                                                                                   \
     EXPORT void EP##set_trace(void *w, int on) { ((IP##world *)w)->trace_on = on; }                       \
+    EXPORT void EP##set_schedule(void *w, int mode) { ((IP##world *)w)->sched = mode; }                   \
     EXPORT int64_t EP##num_trace(void *w) { return ((IP##world *)w)->ntrace / 2; }                        \
     /* (id_a, id_b) pairs that reached SAT, in visiting order */                                        \
     EXPORT void EP##get_trace(void *vw, uint32_t *o)                                                     \

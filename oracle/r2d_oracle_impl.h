@@ -162,6 +162,9 @@ typedef struct {
     R2D_REAL cell_width_arg; uint64_t table_mult_arg; /* stored, unused (lib.zig:141-142) */
     /* hash parameters actually used (lib.zig:255 hard-codes 4.0 and 2*N) */
     R2D_REAL hash_cell; uint64_t hash_mult;
+    /* solver schedule: 0 reference order; 1 graph-coloured (an extension of
+     * this build, not of the reference: items run colour by colour) */
+    int32_t sched;
     /* broadphase trace for tests: candidate pairs visited in discovery order */
     int32_t trace_on;
     int32_t *trace; int64_t ntrace, captrace;
@@ -930,13 +933,86 @@ static void F(apply_gen)(const F(forcegen) *g, F(body) *b)
     b->force.y += (-g->k) * d.y - g->c * v.y;
 }
 
+/* Coloured schedule (extension, no reference counterpart).  Items are the
+ * constraints followed by the manifolds.  An item's ordering bodies are both
+ * bodies of a distance / offset joint (static or not), the body of a fixed /
+ * motor joint, and those bodies of a manifold that are dynamic, touched by a
+ * distance / offset joint, or hold a -0 momentum component (a contact's
+ * "+= 0" on any other static body changes nothing).  Greedy colouring in item
+ * order: smallest colour unused at the item's ordering bodies; past 128
+ * colours every item gets a colour of its own.  Writes the item permutation
+ * sorted by (colour, item). */
+static void F(color_items)(F(world) *w, int32_t *perm)
+{
+    int32_t N = w->nbodies, C = w->ncons, M = w->nmans, I = C + M;
+    uint8_t *jt = (uint8_t *)calloc((size_t)N + 1, 1);
+    for (int32_t c = 0; c < C; ++c) {
+        if (w->cons[c].type > 1) continue;
+        int32_t s1 = F(slot_of)(w, w->cons[c].id1), s2 = F(slot_of)(w, w->cons[c].id2);
+        if (s1 >= 0) jt[s1] = 1;
+        if (s2 >= 0) jt[s2] = 1;
+    }
+    uint64_t *mask = (uint64_t *)calloc(2 * (size_t)N + 2, sizeof(uint64_t));
+    uint32_t *col = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(I + 1));
+    uint32_t next_unique = 128;
+    for (int32_t it = 0; it < I; ++it) {
+        int32_t r[2] = { -1, -1 };
+        if (it < C) {
+            const F(constraint) *c = &w->cons[it];
+            r[0] = F(slot_of)(w, c->id1);
+            if (c->type <= 1) { int32_t s2 = F(slot_of)(w, c->id2); if (s2 != r[0]) r[1] = s2; }
+        } else {
+            const F(manifold) *m = &w->mans[it - C];
+            int32_t e[2] = { m->ref, m->inc };
+            for (int k = 0; k < 2; ++k) {
+                const F(body) *b = &w->bodies[e[k]];
+                int negz = (b->mom.x == 0 && signbit(b->mom.x)) || (b->mom.y == 0 && signbit(b->mom.y)) ||
+                           (b->L == 0 && signbit(b->L));
+                if (!b->is_static || jt[e[k]] || negz) r[k] = e[k];
+            }
+        }
+        uint64_t m0 = 0, m1 = 0;
+        for (int k = 0; k < 2; ++k) if (r[k] >= 0) { m0 |= mask[2 * r[k]]; m1 |= mask[2 * r[k] + 1]; }
+        uint32_t cc;
+        if (~m0) cc = (uint32_t)__builtin_ctzll(~m0);
+        else if (~m1) cc = 64u + (uint32_t)__builtin_ctzll(~m1);
+        else cc = next_unique++;
+        col[it] = cc;
+        for (int k = 0; k < 2; ++k) {
+            if (r[k] < 0) continue;
+            if (cc < 64) mask[2 * r[k]] |= 1ull << cc;
+            else if (cc < 128) mask[2 * r[k] + 1] |= 1ull << (cc - 64);
+        }
+    }
+    /* stable counting order by colour */
+    for (int32_t it = 0; it < I; ++it) perm[it] = it;
+    for (int32_t i = 1; i < I; ++i) {
+        int32_t v = perm[i], j = i;
+        while (j > 0 && col[perm[j - 1]] > col[v]) { perm[j] = perm[j - 1]; --j; }
+        perm[j] = v;
+    }
+    free(col); free(mask); free(jt);
+}
+
 /* Solver.process (lib.zig:189-251).  Returns 0, or -2 for
  * error.InvalidRigidBodyId (state is left as the reference leaves it). */
 static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t iters)
 {
     R2D_REAL f_sub = (R2D_REAL)(float)sub_steps;
     R2D_REAL sub_dt = dt / f_sub;
+    int32_t *perm = NULL;
+    if (w->sched == 1 && sub_steps > 0 && iters > 0) {
+        /* the coloured order is not the constraint order: validate ids up front */
+        for (int32_t c = 0; c < w->ncons; ++c) {
+            if (F(slot_of)(w, w->cons[c].id1) < 0) return -2;
+            if (w->cons[c].type <= 1 && F(slot_of)(w, w->cons[c].id2) < 0) return -2;
+        }
+    }
     F(update_manifolds)(w);
+    if (w->sched == 1) {
+        perm = (int32_t *)malloc(sizeof(int32_t) * (size_t)(w->ncons + w->nmans + 1));
+        F(color_items)(w, perm);
+    }
     for (uint64_t s = 0; s < sub_steps; ++s) {
         for (int32_t gi = 0; gi < w->ngens; ++gi)
             for (int32_t b = 0; b < w->nbodies; ++b) F(apply_gen)(&w->gens[gi], &w->bodies[b]);
@@ -954,6 +1030,18 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
             F(pre_step)(M, &w->bodies[M->ref], &w->bodies[M->inc]);
         }
         for (uint64_t k = 0; k < iters; ++k) {
+            if (perm) {
+                for (int32_t q = 0; q < w->ncons + w->nmans; ++q) {
+                    int32_t it = perm[q];
+                    if (it < w->ncons) {
+                        F(solve_constraint)(w, &w->cons[it], sub_dt);
+                    } else {
+                        F(manifold) *M = &w->mans[it - w->ncons];
+                        F(calc_impulses)(M, &w->bodies[M->ref], &w->bodies[M->inc], sub_dt);
+                    }
+                }
+                continue;
+            }
             for (int32_t c = 0; c < w->ncons; ++c) {
                 int rc = F(solve_constraint)(w, &w->cons[c], sub_dt);
                 if (rc) return rc;
@@ -974,6 +1062,7 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
             B->torque = 0;
         }
     }
+    free(perm);
     return 0;
 }
 

@@ -107,6 +107,7 @@ SIGNATURES = {
     "r2d_ctx_reset_phase_times": [P],
     "r2d_ctx_last_counts": [P, ctypes.POINTER(ctypes.c_uint64)],
     "r2d_debug_eval": [P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), SZ],
+    "r2d_last_error": [ctypes.c_char_p, SZ],
 }
 
 _lib = None
@@ -129,8 +130,16 @@ def load():
     return lib
 
 
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    if _lib is not None:
+        _lib.r2d_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
 def check(status, where):
     if status != R2D_OK:
+        where = f"{where} [{last_error()}]"
         if status == 2:
             raise InvalidRigidBodyId(status, where)
         raise R2DError(status, where)

@@ -23,10 +23,22 @@ using namespace r2d;
 
 #define R2D_EXPORT extern "C" __attribute__((visibility("default")))
 
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) return R2D_E_DEVICE;  \
+static thread_local char g_last_error[512] = "";
+
+static void note_error(const char* what, hipError_t e, int line)
+{
+    std::snprintf(g_last_error, sizeof g_last_error, "%s failed: %s (%d) at r2d_api.hip:%d", what,
+                  hipGetErrorString(e), (int)e, line);
+    if (std::getenv("R2D_DEBUG")) std::fprintf(stderr, "[r2d] %s\n", g_last_error);
+}
+
+#define HIPCHK(x)                                        \
+    do {                                                 \
+        hipError_t e_ = (x);                             \
+        if (e_ != hipSuccess) {                          \
+            note_error(#x, e_, __LINE__);                \
+            return R2D_E_DEVICE;                         \
+        }                                                \
     } while (0)
 
 namespace {
@@ -112,11 +124,19 @@ struct r2d_ctx {
     uint64_t epoch = 0;
     uint64_t last_counts[4] = {0, 0, 0, 0};
     std::vector<r2d_world*> last_batch;
+    // captured step: replayed while the batch layout and launch settings stay the same
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;          // whole step
+    hipGraphExec_t phase_exec[PH_N] = {};         // one per phase (timed steps)
+    std::vector<uint8_t> graph_key;
+    std::vector<WorldDev> uploaded_descs;
 };
 
 namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void destroy_graphs(r2d_ctx* c);
 
 int ensure_world_device(r2d_world* w)
 {
@@ -324,6 +344,10 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.it_k0, I * 4); add((void**)&S.it_k1, I * 4);
     add((void**)&S.deg, (N + 1) * 4); add((void**)&S.deg_off, (N + 1) * 4); add((void**)&S.deg_fill, (N + 1) * 4);
     add((void**)&S.lists, 2 * I * 4);
+    add((void**)&S.it_color, I * 4);
+    add((void**)&S.it_perm, I * 4);
+    add((void**)&S.chist, 130 * (W + 1) * 4);
+    add((void**)&S.cmask, 2 * N * 8);
     add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
     add((void**)&S.counters, 64 * 4);
     add((void**)&S.partials, 1024 * 4);
@@ -345,6 +369,8 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
         if (hipHostMalloc((void**)&c->hstatus, (W + 1) * 4) != hipSuccess) return R2D_E_OOM;
         c->hstatus_cap = W + 1;
     }
+    c->uploaded_descs.clear();
+    destroy_graphs(c);
     if (c->dworlds_cap < W) {
         if (c->dworlds) HIPCHK(hipFree(c->dworlds));
         if (hipMalloc(&c->dworlds, W * sizeof(WorldDev)) != hipSuccess) return R2D_E_OOM;
@@ -353,9 +379,33 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     return R2D_OK;
 }
 
-__global__ void k_zero_i32(int32_t* p, uint32_t n)
+void destroy_graphs(r2d_ctx* c)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    for (int ph = 0; ph < PH_N; ++ph) {
+        if (c->phase_exec[ph]) (void)hipGraphExecDestroy(c->phase_exec[ph]);
+        c->phase_exec[ph] = nullptr;
+    }
+    c->graph_key.clear();
+}
+
+template <class Fn>
+bool capture_graph(r2d_ctx* c, hipGraphExec_t* out, Fn&& enqueue)
+{
+    hipGraph_t g = nullptr;
+    *out = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    enqueue();
+    bool ok = hipStreamEndCapture(c->stream, &g) == hipSuccess && g;
+    if (ok) ok = hipGraphInstantiate(out, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    if (!ok) *out = nullptr;
+    (void)hipGetLastError();
+    return ok;
 }
 
 int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
@@ -396,7 +446,11 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
     int rc = ensure_scratch(c, (uint32_t)N, (uint32_t)B, (uint32_t)nw, (uint32_t)J, est_e, est_r, est_m,
                             (uint32_t)J + est_m);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->dworlds, descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
+    if (c->uploaded_descs.size() != descs.size() ||
+        std::memcmp(c->uploaded_descs.data(), descs.data(), descs.size() * sizeof(WorldDev)) != 0) {
+        HIPCHK(hipMemcpyAsync(c->dworlds, descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
+        c->uploaded_descs = descs;
+    }
     bool lds = solve_lds_bytes((int)maxn) <= 160 * 1024 - 1024;
     unsigned threads = maxn <= 512 ? 256 : 1024;
     if (const char* e = std::getenv("R2D_SOLVE_THREADS")) {
@@ -412,10 +466,42 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
         S.worlds = c->dworlds;
         HIPCHK(hipMemsetAsync(S.counters, 0, 64 * 4, c->stream));
         size_t todo = steps - done;
+        KernelTimer* tm = c->profiling ? &c->timer : nullptr;
+        auto eager_phase = [&](int ph) {
+            launch_phase(S, c->stream, ph, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, threads);
+        };
+        bool graph_ok = false;
+        if (c->use_graph) {
+            // key: everything the captured launches bake in
+            std::vector<uint8_t> key;
+            auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + n); };
+            put(&S, sizeof S);
+            put(descs.data(), descs.size() * sizeof(WorldDev));
+            float kdt = dt; uint64_t ks = sub_steps, ki = iters; uint32_t kt = threads; int kl = lds;
+            put(&kdt, 4); put(&ks, 8); put(&ki, 8); put(&kt, 4); put(&kl, 4); put(&maxn, 4);
+            if (key != c->graph_key) {
+                destroy_graphs(c);
+                bool ok = capture_graph(c, &c->graph_exec, [&]() { for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph); });
+                for (int ph = 0; ok && ph < PH_N; ++ph)
+                    ok = capture_graph(c, &c->phase_exec[ph], [&]() { eager_phase(ph); });
+                if (ok) c->graph_key.swap(key);
+                else { destroy_graphs(c); c->use_graph = false; }   // capture unsupported: launch eagerly
+            }
+            graph_ok = c->graph_exec != nullptr;
+        }
         for (size_t s = 0; s < todo; ++s) {
-            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, c->stream, S.w_status, (uint32_t)nw + 1);
-            KernelTimer* tm = c->profiling ? &c->timer : nullptr;
-            launch_step(S, c->stream, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, threads, tm);
+            if (!tm) {
+                if (graph_ok) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+                else for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph);
+            } else {
+                // timed step: events on the stream between the phases
+                for (int ph = 0; ph < PH_N; ++ph) {
+                    tm->mark(ph, c->stream);
+                    if (graph_ok) HIPCHK(hipGraphLaunch(c->phase_exec[ph], c->stream));
+                    else eager_phase(ph);
+                }
+                tm->mark(KT_END, c->stream);
+            }
             if (tm) {
                 HIPCHK(hipStreamSynchronize(c->stream));
                 for (int p = 0; p < 4; ++p) {
@@ -432,6 +518,8 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
         uint32_t committed = c->hcounters[C_STEPS];
         done += committed;
         for (size_t i = 0; i < nw; ++i) {
+            if (c->hstatus[i] == 5)
+                std::snprintf(g_last_error, sizeof g_last_error, "solver schedule stalled in world %zu", i);
             if (c->hstatus[i] != 0 && first_err == R2D_OK)
                 first_err = c->hstatus[i] == 4 ? R2D_E_NONFINITE : c->hstatus[i] == 5 ? R2D_E_DEVICE : R2D_E_RANGE;
         }
@@ -497,6 +585,7 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
     for (int i = 0; i < KT_N; ++i) HIPCHK(hipEventCreate(&c->timer.ev[i]));
     set_solve_smem_limit();
+    if (const char* e = std::getenv("R2D_NO_GRAPH")) c->use_graph = !(e[0] == '1');
     *out = c;
     return R2D_OK;
 }
@@ -505,6 +594,7 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
 {
     if (!c) return R2D_E_INVALID_ARG;
     (void)hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
     if (c->arena) (void)hipFree(c->arena);
     if (c->dworlds) (void)hipFree(c->dworlds);
     if (c->hcounters) (void)hipHostFree(c->hcounters);
@@ -860,7 +950,7 @@ R2D_EXPORT int r2d_set_hash_params(r2d_world* w, int mode, float cell, size_t mu
 
 R2D_EXPORT int r2d_set_schedule(r2d_world* w, int mode)
 {
-    if (!w || mode != 0) return R2D_E_INVALID_ARG;
+    if (!w || (mode != 0 && mode != 1)) return R2D_E_INVALID_ARG;
     w->sched = mode;
     return R2D_OK;
 }
@@ -919,6 +1009,13 @@ R2D_EXPORT int r2d_debug_eval(r2d_ctx* c, int op, const float* in, float* out, s
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(din);
     (void)hipFree(dout);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_last_error(char* buf, size_t n)
+{
+    if (!buf || n == 0) return R2D_E_INVALID_ARG;
+    std::snprintf(buf, n, "%s", g_last_error);
     return R2D_OK;
 }
 

@@ -105,6 +105,10 @@ struct StepDev {
     uint32_t *it_k0, *it_k1;
     uint32_t *deg, *deg_off, *deg_fill;    // [N+1]
     uint32_t* lists;                       // [2*cap_i]
+    uint32_t* it_color;                    // [cap_i] colour (0 in reference order)
+    uint32_t* it_perm;                     // [cap_i] world-local items in execution order
+    uint32_t* chist;                       // [130*W] colour histogram per world
+    uint64_t* cmask;                       // [2*N] colours used per body (coloured schedule)
     // global-memory solver state (worlds too large for LDS)
     float *g_im, *g_ii;
     uint32_t* g_cnt;

@@ -726,8 +726,60 @@ __global__ void k_list_fill(StepDev S)
     }
 }
 
-// rank of each item among the items sharing one of its resources, in the
-// reference's sequential order (constraints, then manifolds in discovery order)
+// Coloured schedule: greedy colouring of each world's items in item order;
+// an item takes the smallest colour unused by earlier items sharing one of
+// its ordering bodies.  Worlds in reference order keep colour 0 everywhere.
+__global__ void k_color(StepDev S)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    for (uint32_t w = grid_tid(); w < S.nworlds; w += grid_stride()) {
+        const WorldDev& D = S.worlds[w];
+        uint32_t i0 = S.w_item_base[w], i1 = S.w_item_base[w + 1];
+        if (D.sched == 0) {
+            for (uint32_t it = i0; it < i1; ++it) { S.it_color[it] = 0u; S.it_perm[it] = it - i0; }
+            continue;
+        }
+        for (int b = 0; b < D.nbodies; ++b) { S.cmask[2 * (D.body_base + b)] = 0; S.cmask[2 * (D.body_base + b) + 1] = 0; }
+        uint32_t overflow_color = 128u;
+        for (uint32_t it = i0; it < i1; ++it) {
+            int32_t r0 = S.it_r0[it], r1 = S.it_r1[it];
+            uint64_t m0 = 0, m1 = 0;
+            if (r0 >= 0) { m0 |= S.cmask[2 * r0]; m1 |= S.cmask[2 * r0 + 1]; }
+            if (r1 >= 0) { m0 |= S.cmask[2 * r1]; m1 |= S.cmask[2 * r1 + 1]; }
+            uint32_t c;
+            if (~m0) c = (uint32_t)__builtin_ctzll(~m0);
+            else if (~m1) c = 64u + (uint32_t)__builtin_ctzll(~m1);
+            else c = overflow_color++;   // a colour of its own is always independent
+            S.it_color[it] = c;
+            if (c < 64) {
+                uint64_t bit = 1ull << c;
+                if (r0 >= 0) S.cmask[2 * r0] |= bit;
+                if (r1 >= 0) S.cmask[2 * r1] |= bit;
+            } else if (c < 128) {
+                uint64_t bit = 1ull << (c - 64);
+                if (r0 >= 0) S.cmask[2 * r0 + 1] |= bit;
+                if (r1 >= 0) S.cmask[2 * r1 + 1] |= bit;
+            }
+        }
+        // execution order = (colour, item): a stable counting sort; colours past
+        // 127 are unique and increase with the item index, so they share the
+        // last bin in item order.  Workers walk this order, the same order the
+        // per-body ranks use, so the earliest unfinished item always has a worker.
+        uint32_t* h = S.chist + 130 * w;
+        for (int c = 0; c < 130; ++c) h[c] = 0u;
+        for (uint32_t it = i0; it < i1; ++it) h[umin(S.it_color[it], 128u) + 1] += 1u;
+        for (int c = 1; c < 130; ++c) h[c] += h[c - 1];
+        for (uint32_t it = i0; it < i1; ++it) {
+            uint32_t b = umin(S.it_color[it], 128u);
+            S.it_perm[i0 + h[b]] = it - i0;
+            h[b] += 1u;
+        }
+    }
+}
+
+// rank of each item among the items sharing one of its resources: in the
+// reference's sequential order (constraints, then manifolds in discovery
+// order), or colour by colour for the coloured schedule
 __global__ void k_list_rank(StepDev S)
 {
     if (S.counters[C_OVERFLOW]) return;
@@ -736,8 +788,9 @@ __global__ void k_list_rank(StepDev S)
         uint32_t* l = S.lists + o;
         for (uint32_t i = 1; i < n; ++i) {
             uint32_t v = l[i];
+            uint64_t kv = ((uint64_t)S.it_color[v >> 1] << 32) | v;
             uint32_t j = i;
-            while (j > 0 && l[j - 1] > v) { l[j] = l[j - 1]; --j; }
+            while (j > 0 && (((uint64_t)S.it_color[l[j - 1] >> 1] << 32) | l[j - 1]) > kv) { l[j] = l[j - 1]; --j; }
             l[j] = v;
         }
         for (uint32_t p = 0; p < n; ++p) {
@@ -915,7 +968,7 @@ __device__ __forceinline__ void solve_joint(const WorldDev& D, const ST& st, con
 // as a dataflow: an item may run once every earlier item (in the reference's
 // order) that touches one of its ordering bodies has run, which gives the
 // same bits as the sequential loop.
-constexpr uint32_t SPIN_LIMIT = 1u << 24;
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
 
 template <bool LDS>
 __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
@@ -1065,7 +1118,7 @@ __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t su
                 if (!have) {
                     if (q >= total) break;
                     k = q / nitems;
-                    j = q - k * nitems;
+                    j = S.it_perm[ibase + (q - k * nitems)];
                     uint32_t item = ibase + j;
                     r0 = S.it_r0[item]; r1 = S.it_r1[item];
                     if (r0 >= 0) { r0 -= (int32_t)base; t0 = k * st.deg[r0] + S.it_k0[item]; }
@@ -1146,49 +1199,58 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs, unsigned cap = 4096)
 
 size_t solve_lds_bytes(int nbodies) { return (size_t)nbodies * 7u * 4u; }
 
-void launch_step(const StepDev& S, hipStream_t st, float dt, uint32_t sub_steps, uint32_t iters,
-                 uint32_t max_bodies, bool lds, unsigned solve_threads, KernelTimer* tm)
+void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_t sub_steps, uint32_t iters,
+                  uint32_t max_bodies, bool lds, unsigned solve_threads)
 {
     const unsigned BS = 256;
     const unsigned GRID = 1024;   // grid-stride kernels over device-sized ranges
-    auto T = [&](int id) { if (tm) tm->mark(id, st); };
-    T(KT_BROAD_BEGIN);
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, S.counters);
-    hipLaunchKernelGGL(k_fill_bworld, dim3(S.nworlds), dim3(256), 0, st, S);
-    hipLaunchKernelGGL(k_prep, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
-    scan_u32(S.ncell, S.cell_off, nullptr, S.nb, S.nb, &S.counters[C_ENTRIES], S.partials, st);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_cnt, S.nbuckets + 1);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_fill, S.nbuckets + 1);
-    hipLaunchKernelGGL(k_entries, dim3(GRID), dim3(BS), 0, st, S);
-    scan_u32(S.bkt_cnt, S.bkt_off, nullptr, S.nbuckets, S.nbuckets, nullptr, S.partials, st);
-    hipLaunchKernelGGL(k_bucket_fill, dim3(GRID), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(S.nbuckets, BS)), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_ent_cnt, dim3(GRID), dim3(BS), 0, st, S);
-    scan_u32(S.ent_cnt, S.ent_off, &S.counters[C_ENTRIES], 0, S.cap_e, &S.counters[C_RAW], S.partials, st);
-    hipLaunchKernelGGL(k_cand_emit, dim3(GRID), dim3(BS), 0, st, S);
-    T(KT_NARROW_BEGIN);
-    hipLaunchKernelGGL(k_narrow, dim3(GRID), dim3(BS), 0, st, S);
-    scan_u32(S.cand_flag, S.cand_idx, &S.counters[C_RAW], 0, S.cap_r, &S.counters[C_MANIFOLDS], S.partials, st);
-    hipLaunchKernelGGL(k_man_build, dim3(GRID), dim3(BS), 0, st, S);
-    T(KT_SCHED_BEGIN);
-    hipLaunchKernelGGL(k_world_ranges, dim3(blocks_for(S.nworlds + 1, BS)), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_joint_mark, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg, S.nb + 1);
-    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg_fill, S.nb + 1);
-    hipLaunchKernelGGL(k_items_joints, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_items_mans, dim3(GRID), dim3(BS), 0, st, S);
-    scan_u32(S.deg, S.deg_off, nullptr, S.nb, S.nb, nullptr, S.partials, st);
-    hipLaunchKernelGGL(k_list_fill, dim3(GRID), dim3(BS), 0, st, S);
-    hipLaunchKernelGGL(k_list_rank, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
-    T(KT_SOLVE_BEGIN);
-    if (lds) {
-        size_t sm = solve_lds_bytes((int)max_bodies);
-        hipLaunchKernelGGL(k_solve<true>, dim3(S.nworlds), dim3(solve_threads), sm, st, S, dt, sub_steps, iters);
-    } else {
-        hipLaunchKernelGGL(k_solve<false>, dim3(S.nworlds), dim3(solve_threads), 0, st, S, dt, sub_steps, iters);
+    switch (phase) {
+    case PH_BROAD:
+        hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, S.counters);
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nworlds + 1, BS)), dim3(BS), 0, st,
+                           reinterpret_cast<uint32_t*>(S.w_status), S.nworlds + 1);
+        hipLaunchKernelGGL(k_fill_bworld, dim3(S.nworlds), dim3(256), 0, st, S);
+        hipLaunchKernelGGL(k_prep, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        scan_u32(S.ncell, S.cell_off, nullptr, S.nb, S.nb, &S.counters[C_ENTRIES], S.partials, st);
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_cnt,
+                           S.nbuckets + 1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_fill,
+                           S.nbuckets + 1);
+        hipLaunchKernelGGL(k_entries, dim3(GRID), dim3(BS), 0, st, S);
+        scan_u32(S.bkt_cnt, S.bkt_off, nullptr, S.nbuckets, S.nbuckets, nullptr, S.partials, st);
+        hipLaunchKernelGGL(k_bucket_fill, dim3(GRID), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(S.nbuckets, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_ent_cnt, dim3(GRID), dim3(BS), 0, st, S);
+        scan_u32(S.ent_cnt, S.ent_off, &S.counters[C_ENTRIES], 0, S.cap_e, &S.counters[C_RAW], S.partials, st);
+        hipLaunchKernelGGL(k_cand_emit, dim3(GRID), dim3(BS), 0, st, S);
+        break;
+    case PH_NARROW:
+        hipLaunchKernelGGL(k_narrow, dim3(GRID), dim3(BS), 0, st, S);
+        scan_u32(S.cand_flag, S.cand_idx, &S.counters[C_RAW], 0, S.cap_r, &S.counters[C_MANIFOLDS], S.partials, st);
+        hipLaunchKernelGGL(k_man_build, dim3(GRID), dim3(BS), 0, st, S);
+        break;
+    case PH_SCHED:
+        hipLaunchKernelGGL(k_world_ranges, dim3(blocks_for(S.nworlds + 1, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_joint_mark, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg, S.nb + 1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg_fill, S.nb + 1);
+        hipLaunchKernelGGL(k_items_joints, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_items_mans, dim3(GRID), dim3(BS), 0, st, S);
+        scan_u32(S.deg, S.deg_off, nullptr, S.nb, S.nb, nullptr, S.partials, st);
+        hipLaunchKernelGGL(k_color, dim3(blocks_for(S.nworlds, 64)), dim3(64), 0, st, S);
+        hipLaunchKernelGGL(k_list_fill, dim3(GRID), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_list_rank, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        break;
+    default:
+        if (lds) {
+            size_t sm = solve_lds_bytes((int)max_bodies);
+            hipLaunchKernelGGL(k_solve<true>, dim3(S.nworlds), dim3(solve_threads), sm, st, S, dt, sub_steps, iters);
+        } else {
+            hipLaunchKernelGGL(k_solve<false>, dim3(S.nworlds), dim3(solve_threads), 0, st, S, dt, sub_steps, iters);
+        }
+        hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
+        break;
     }
-    T(KT_END);
-    hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
 }
 
 void set_solve_smem_limit()

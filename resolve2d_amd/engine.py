@@ -98,6 +98,9 @@ class GpuWorld:
     def set_hash(self, cell, mult):
         N.check(self.lib.r2d_set_hash_params(self.handle, 1, cell, mult), "r2d_set_hash_params")
 
+    def set_schedule(self, mode):
+        N.check(self.lib.r2d_set_schedule(self.handle, mode), "r2d_set_schedule")
+
     def process(self, dt, sub_steps, iters):
         return self.lib.r2d_process(self.handle, dt, sub_steps, iters)
 
