@@ -130,6 +130,7 @@ struct r2d_ctx {
     hipGraphExec_t phase_exec[PH_N] = {};         // one per phase (timed steps)
     std::vector<uint8_t> graph_key;
     std::vector<WorldDev> uploaded_descs;
+    int solve_variant = -1;                       // -1: choose by world size
 };
 
 namespace {
@@ -452,11 +453,14 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
         c->uploaded_descs = descs;
     }
     bool lds = solve_lds_bytes((int)maxn) <= 160 * 1024 - 1024;
-    unsigned threads = maxn <= 512 ? 256 : 1024;
-    if (const char* e = std::getenv("R2D_SOLVE_THREADS")) {
+    // solve variant: context override (autotuner), environment, then size heuristic
+    int variant = maxn <= 600 ? 1 : 0;
+    if (c->solve_variant >= 0) variant = c->solve_variant;
+    if (const char* e = std::getenv("R2D_SOLVE_VARIANT")) {
         int v = std::atoi(e);
-        if (v >= 64 && v <= 1024 && v % 64 == 0) threads = (unsigned)v;
+        if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
+    const unsigned threads = solve_variant_threads(variant);
     int first_err = R2D_OK;
     size_t done = 0;
     for (int attempt = 0; done < steps; ++attempt) {
@@ -468,7 +472,7 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
         size_t todo = steps - done;
         KernelTimer* tm = c->profiling ? &c->timer : nullptr;
         auto eager_phase = [&](int ph) {
-            launch_phase(S, c->stream, ph, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, threads);
+            launch_phase(S, c->stream, ph, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, variant);
         };
         bool graph_ok = false;
         if (c->use_graph) {
@@ -477,8 +481,8 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
             auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + n); };
             put(&S, sizeof S);
             put(descs.data(), descs.size() * sizeof(WorldDev));
-            float kdt = dt; uint64_t ks = sub_steps, ki = iters; uint32_t kt = threads; int kl = lds;
-            put(&kdt, 4); put(&ks, 8); put(&ki, 8); put(&kt, 4); put(&kl, 4); put(&maxn, 4);
+            float kdt = dt; uint64_t ks = sub_steps, ki = iters; uint32_t kt = threads; int kl = lds, kv = variant;
+            put(&kdt, 4); put(&ks, 8); put(&ki, 8); put(&kt, 4); put(&kl, 4); put(&maxn, 4); put(&kv, 4);
             if (key != c->graph_key) {
                 destroy_graphs(c);
                 bool ok = capture_graph(c, &c->graph_exec, [&]() { for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph); });
@@ -1016,6 +1020,13 @@ R2D_EXPORT int r2d_last_error(char* buf, size_t n)
 {
     if (!buf || n == 0) return R2D_E_INVALID_ARG;
     std::snprintf(buf, n, "%s", g_last_error);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_set_solve_variant(r2d_ctx* c, int variant)
+{
+    if (!c || variant < -1 || variant >= SOLVE_VARIANTS) return R2D_E_INVALID_ARG;
+    c->solve_variant = variant;
     return R2D_OK;
 }
 

@@ -21,9 +21,15 @@ size_t solve_lds_bytes(int nbodies);
 // the four phases of one step, enqueued in this order on one stream
 enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
+// solve-kernel variants (register slots per thread x workgroup size); 0 is
+// the no-slot 1024-thread kernel for large worlds
+constexpr int SOLVE_VARIANTS = 5;
+unsigned solve_variant_threads(int v);
+int solve_variant_slots(int v);
+
 // enqueue one phase of a batched step on stream st
 void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_t sub_steps, uint32_t iters,
-                  uint32_t max_bodies, bool lds, unsigned solve_threads);
+                  uint32_t max_bodies, bool lds, int solve_variant);
 
 void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_const, uint32_t cap,
               uint32_t* total_dev, uint32_t* partials, hipStream_t st);

@@ -968,10 +968,181 @@ __device__ __forceinline__ void solve_joint(const WorldDev& D, const ST& st, con
 // as a dataflow: an item may run once every earlier item (in the reference's
 // order) that touches one of its ordering bodies has run, which gives the
 // same bits as the sequential loop.
+
+// --------------------------------------------------- register-resident items
+// One thread owns the items at execution positions tid, tid + blockDim, ...
+// (at most NSLOT of them) and keeps each item's contact geometry, effective
+// masses and accumulated impulses in registers for the whole step.
+struct Slot {
+    uint32_t flags;          // SL_* bits
+    uint32_t j;              // world-local item index
+    int32_t r0, r1;          // ordering bodies (world-local) or -1
+    uint32_t rk0, rk1;       // ranks at r0 / r1
+    uint32_t dg0, dg1;       // item counts of r0 / r1 (targets advance by these per iteration)
+    uint32_t l1, l2;         // manifold bodies (ref, inc), world-local
+    uint32_t m;              // manifold index in the step's arrays
+    float nx, ny, fric, prev1, prev2;
+    float r1x[2], r1y[2], r2x[2], r2y[2], depth[2], odepth[2], mn[2], mt[2], apn[2], apt[2];
+};
+enum : uint32_t { SL_USED = 1u, SL_JOINT = 2u, SL_P0 = 4u, SL_P1 = 8u, SL_W1 = 16u, SL_W2 = 32u, SL_S1 = 64u, SL_S2 = 128u };
+
+// CollisionManifold.preStep effective masses (collision.zig:117-132) for one point
+template <class ST>
+__device__ __forceinline__ void slot_masses(Slot& z, const ST& st, int p)
+{
+    float im1 = st.im[z.l1], im2 = st.im[z.l2], ii1 = st.ii[z.l1], ii2 = st.ii[z.l2];
+    float inv_mass = im1 + im2;
+    v2 n = mk(z.nx, z.ny), t = rot90cw(n);
+    v2 r1 = mk(z.r1x[p], z.r1y[p]), r2 = mk(z.r2x[p], z.r2y[p]);
+    float r1n = cross2(r1, n), r2n = cross2(r2, n);
+    float kn = inv_mass + ii1 * (r1n * r1n) + ii2 * (r2n * r2n);
+    float r1t = cross2(r1, t), r2t = cross2(r2, t);
+    float kt = inv_mass + ii1 * (r1t * r1t) + ii2 * (r2t * r2t);
+    z.mn[p] = kn > 0.0f ? 1.0f / kn : 0.0f;
+    z.mt[p] = kt > 0.0f ? 1.0f / kt : 0.0f;
+}
+
+template <class ST>
+__device__ __forceinline__ void slot_load(Slot& z, const StepDev& S, const WorldDev& D, const ST& st, uint32_t pos,
+                                          uint32_t ibase, uint32_t nj, uint32_t mbeg, uint32_t base)
+{
+    const size_t cm = S.cap_m;
+    const float* P = S.m_pt;
+    z.j = S.it_perm[ibase + pos];
+    uint32_t item = ibase + z.j;
+    z.r0 = S.it_r0[item];
+    z.r1 = S.it_r1[item];
+    z.flags = SL_USED;
+    if (z.r0 >= 0) { z.r0 -= (int32_t)base; z.rk0 = S.it_k0[item]; z.dg0 = st.deg[z.r0]; }
+    if (z.r1 >= 0) { z.r1 -= (int32_t)base; z.rk1 = S.it_k1[item]; z.dg1 = st.deg[z.r1]; }
+    if (z.j < nj) { z.flags |= SL_JOINT; return; }
+    uint32_t m = mbeg + (z.j - nj);
+    z.m = m;
+    z.l1 = S.m_ref[m] - base;
+    z.l2 = S.m_inc[m] - base;
+    if (S.sflag[base + z.l1] & SF_RESOURCE) z.flags |= SL_W1;
+    if (S.sflag[base + z.l2] & SF_RESOURCE) z.flags |= SL_W2;
+    if (D.flags[z.l1] & BF_STATIC) z.flags |= SL_S1;
+    if (D.flags[z.l2] & BF_STATIC) z.flags |= SL_S2;
+    z.nx = S.m_nx[m];
+    z.ny = S.m_ny[m];
+    z.fric = sqrt_rn(D.mu[z.l1] * D.mu[z.l2]);
+    z.prev1 = S.m_prev1[m];
+    z.prev2 = S.m_prev2[m];
+    uint32_t valid = S.m_valid[m];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (valid & (1u << p)) z.flags |= (SL_P0 << p);
+        z.r1x[p] = P[(MP_R1X * 2 + p) * cm + m];
+        z.r1y[p] = P[(MP_R1Y * 2 + p) * cm + m];
+        z.r2x[p] = P[(MP_R2X * 2 + p) * cm + m];
+        z.r2y[p] = P[(MP_R2Y * 2 + p) * cm + m];
+        z.depth[p] = P[(MP_DEPTH * 2 + p) * cm + m];
+        z.odepth[p] = P[(MP_ODEPTH * 2 + p) * cm + m];
+        z.apn[p] = 0.0f;
+        z.apt[p] = 0.0f;
+        slot_masses(z, st, p);
+    }
+}
+
+// updateTGSDepth (collision.zig:73-100): points whose depth left its original
+// value are moved with the bodies' rotation since the last sub-step; the
+// effective masses then follow the new lever arms (preStep runs every sub-step)
+template <class ST>
+__device__ __forceinline__ void slot_tgs(Slot& z, const WorldDev& D, const ST& st)
+{
+    float a1 = D.ang[z.l1], a2 = D.ang[z.l2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (!(z.flags & (SL_P0 << p))) continue;
+        if (approx(z.depth[p], z.odepth[p], 1e-4f)) continue;
+        v2 rr1 = rotate2(mk(z.r1x[p], z.r1y[p]), a1 - z.prev1);
+        v2 rr2 = rotate2(mk(z.r2x[p], z.r2y[p]), a2 - z.prev2);
+        v2 q1 = add2(rr1, mk(D.px[z.l1], D.py[z.l1]));
+        v2 q2 = add2(rr2, mk(D.px[z.l2], D.py[z.l2]));
+        float dd = dot2(mk(z.nx, z.ny), sub2(q2, q1));
+        z.depth[p] = dd + z.odepth[p];
+        z.r1x[p] = rr1.x; z.r1y[p] = rr1.y; z.r2x[p] = rr2.x; z.r2y[p] = rr2.y;
+        slot_masses(z, st, p);
+    }
+    z.prev1 = a1;
+    z.prev2 = a2;
+}
+
+// CollisionManifold.calculateImpulses (collision.zig:135-218) on a register slot
+template <class ST>
+__device__ __forceinline__ void slot_impulses(Slot& z, const ST& st, float dt)
+{
+    const uint32_t l1 = z.l1, l2 = z.l2;
+    float im1 = st.im[l1], ii1 = st.ii[l1], im2 = st.im[l2], ii2 = st.ii[l2];
+    v2 vl1 = scale2(mk(st.mx[l1], st.my[l1]), im1);
+    float om1 = st.L[l1] * ii1;
+    v2 vl2 = scale2(mk(st.mx[l2], st.my[l2]), im2);
+    float om2 = st.L[l2] * ii2;
+    v2 n = mk(z.nx, z.ny);
+    v2 t = rot90cw(n);
+    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
+    float rot1 = 0.f, rot2 = 0.f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (!(z.flags & (SL_P0 << p))) continue;
+        float depth = z.depth[p];
+        if (depth >= 0.f) { z.apn[p] = 0.f; z.apt[p] = 0.f; continue; }
+        v2 r1 = mk(z.r1x[p], z.r1y[p]), r2 = mk(z.r2x[p], z.r2y[p]);
+        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
+        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
+        v2 dv = sub2(v1, v2v);
+        float bias = BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
+        float num = dot2(dv, n) + bias;
+        float pn = num * z.mn[p];
+        if (pn < MIN_MANIFOLD_IMPULSE) continue;
+        num = dot2(dv, t);
+        float pt = num * z.mt[p];
+        float acc_pn = z.apn[p], acc_pt = z.apt[p];
+        float new_pn = maxn(0.0f, acc_pn + pn);
+        float applied_pn = new_pn - acc_pn;
+        acc_pn = new_pn;
+        float max_pt = z.fric * absf(acc_pn);
+        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
+        float applied_pt = new_pt - acc_pt;
+        z.apn[p] = acc_pn;
+        z.apt[p] = new_pt;
+        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
+        if (!(z.flags & SL_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
+        if (!(z.flags & SL_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
+    }
+    if (z.flags & SL_W1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
+    if (z.flags & SL_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
+}
+
+__device__ __forceinline__ void slot_store(const Slot& z, const StepDev& S)
+{
+    if (!(z.flags & SL_USED) || (z.flags & SL_JOINT)) return;
+    const size_t cm = S.cap_m;
+    float* P = S.m_pt;
+    const uint32_t m = z.m;
+    S.m_fric[m] = z.fric;
+    S.m_prev1[m] = z.prev1;
+    S.m_prev2[m] = z.prev2;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (!(z.flags & (SL_P0 << p))) continue;
+        P[(MP_ACC_PN * 2 + p) * cm + m] = z.apn[p];
+        P[(MP_ACC_PT * 2 + p) * cm + m] = z.apt[p];
+        P[(MP_R1X * 2 + p) * cm + m] = z.r1x[p];
+        P[(MP_R1Y * 2 + p) * cm + m] = z.r1y[p];
+        P[(MP_R2X * 2 + p) * cm + m] = z.r2x[p];
+        P[(MP_R2Y * 2 + p) * cm + m] = z.r2y[p];
+        P[(MP_DEPTH * 2 + p) * cm + m] = z.depth[p];
+        P[(MP_MASS_N * 2 + p) * cm + m] = z.mn[p];
+        P[(MP_MASS_T * 2 + p) * cm + m] = z.mt[p];
+    }
+}
+
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
 
-template <bool LDS>
-__global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
+template <bool LDS, int NSLOT, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
@@ -1011,6 +1182,22 @@ __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t su
         st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
     }
     __syncthreads();
+
+    // register path when every item of this world has an owner slot
+    const bool use_reg = NSLOT > 0 && nitems <= (uint32_t)NSLOT * blockDim.x;
+    Slot sl[NSLOT > 0 ? NSLOT : 1];
+    uint32_t nslot = 0;
+    if (use_reg) {
+#pragma unroll
+        for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
+            sl[z].flags = 0u;
+            uint32_t pos = threadIdx.x + (uint32_t)z * blockDim.x;
+            if (pos < nitems) {
+                slot_load(sl[z], S, D, st, pos, ibase, nj, mbeg, base);
+                nslot = (uint32_t)z + 1u;
+            }
+        }
+    }
 
     const uint32_t M = S.w_man_begin[w + 1] - mbeg;
     float* P = S.m_pt;
@@ -1065,7 +1252,12 @@ __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t su
         }
         __syncthreads();
         // updateTGSDepth + preStep (collision.zig:73-133)
-        for (uint32_t mm = threadIdx.x; mm < M; mm += blockDim.x) {
+        if (use_reg) {
+#pragma unroll
+            for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z)
+                if ((sl[z].flags & SL_USED) && !(sl[z].flags & SL_JOINT)) slot_tgs(sl[z], D, st);
+        }
+        for (uint32_t mm = threadIdx.x; !use_reg && mm < M; mm += blockDim.x) {
             uint32_t m = mbeg + mm;
             uint32_t l1 = S.m_ref[m] - base, l2 = S.m_inc[m] - base;
             v2 n = mk(S.m_nx[m], S.m_ny[m]);
@@ -1105,7 +1297,42 @@ __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t su
         }
         __syncthreads();
         // K iterations of (constraints, manifolds) as one ordered dataflow
-        {
+        if (use_reg) {
+            uint32_t k = 0, cz = 0, spins = 0;
+            if (nslot == 0) k = iters;
+            while (k < iters) {
+                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                bool ready = false;
+#pragma unroll
+                for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
+                    if ((uint32_t)z == cz) {
+                        ready = true;
+                        if (sl[z].r0 >= 0) ready = ld_acq(&st.cnt[sl[z].r0]) == k * sl[z].dg0 + sl[z].rk0;
+                        if (ready && sl[z].r1 >= 0) ready = ld_acq(&st.cnt[sl[z].r1]) == k * sl[z].dg1 + sl[z].rk1;
+                    }
+                }
+                if (ready) {
+#pragma unroll
+                    for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
+                        if ((uint32_t)z == cz) {
+                            if (sl[z].flags & SL_JOINT) solve_joint(D, st, D.joints[sl[z].j], sub_dt);
+                            else slot_impulses(sl[z], st, sub_dt);
+                            if (sl[z].r0 >= 0) st_rel(&st.cnt[sl[z].r0], k * sl[z].dg0 + sl[z].rk0 + 1u);
+                            if (sl[z].r1 >= 0) st_rel(&st.cnt[sl[z].r1], k * sl[z].dg1 + sl[z].rk1 + 1u);
+                        }
+                    }
+                    spins = 0;
+                    if (++cz == nslot) { cz = 0; ++k; }
+                } else {
+                    if (++spins > SPIN_LIMIT) {
+                        __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        S.w_status[w] = 5;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        } else {
             const uint32_t total = iters * nitems;
             uint32_t q = threadIdx.x;
             uint32_t k = 0, j = 0;
@@ -1168,6 +1395,10 @@ __global__ __launch_bounds__(1024) void k_solve(StepDev S, float dt, uint32_t su
         }
         __syncthreads();
     }
+    if (use_reg) {
+#pragma unroll
+        for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) slot_store(sl[z], S);
+    }
     if (LDS) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
@@ -1199,8 +1430,26 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs, unsigned cap = 4096)
 
 size_t solve_lds_bytes(int nbodies) { return (size_t)nbodies * 7u * 4u; }
 
+typedef void (*SolveFn)(StepDev, float, uint32_t, uint32_t);
+struct SolveVariant {
+    SolveFn lds_fn;
+    unsigned threads;
+    int slots;
+};
+// solve kernel variants: register slots per thread x workgroup size
+static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
+    {k_solve<true, 0, 1024>, 1024, 0},
+    {k_solve<true, 4, 256>, 256, 4},
+    {k_solve<true, 2, 512>, 512, 2},
+    {k_solve<true, 2, 256>, 256, 2},
+    {k_solve<true, 0, 256>, 256, 0},
+};
+
+unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
+int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
+
 void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_t sub_steps, uint32_t iters,
-                  uint32_t max_bodies, bool lds, unsigned solve_threads)
+                  uint32_t max_bodies, bool lds, int solve_variant)
 {
     const unsigned BS = 256;
     const unsigned GRID = 1024;   // grid-stride kernels over device-sized ranges
@@ -1241,21 +1490,25 @@ void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_
         hipLaunchKernelGGL(k_list_fill, dim3(GRID), dim3(BS), 0, st, S);
         hipLaunchKernelGGL(k_list_rank, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
         break;
-    default:
+    default: {
+        const SolveVariant& v = kSolveVariants[solve_variant < SOLVE_VARIANTS ? solve_variant : 0];
         if (lds) {
             size_t sm = solve_lds_bytes((int)max_bodies);
-            hipLaunchKernelGGL(k_solve<true>, dim3(S.nworlds), dim3(solve_threads), sm, st, S, dt, sub_steps, iters);
+            hipLaunchKernelGGL(v.lds_fn, dim3(S.nworlds), dim3(v.threads), sm, st, S, dt, sub_steps, iters);
         } else {
-            hipLaunchKernelGGL(k_solve<false>, dim3(S.nworlds), dim3(solve_threads), 0, st, S, dt, sub_steps, iters);
+            hipLaunchKernelGGL((k_solve<false, 0, 1024>), dim3(S.nworlds), dim3(1024), 0, st, S, dt, sub_steps, iters);
         }
         hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
         break;
+    }
     }
 }
 
 void set_solve_smem_limit()
 {
-    hipFuncSetAttribute((const void*)k_solve<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    for (int v = 0; v < SOLVE_VARIANTS; ++v)
+        (void)hipFuncSetAttribute((const void*)kSolveVariants[v].lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
 }
 
 }  // namespace r2d

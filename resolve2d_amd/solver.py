@@ -53,6 +53,9 @@ class Context:
         N.check(self.lib.r2d_process_batch(self.handle, arr, len(handles), dt, sub_steps, collision_iters, steps),
                 "r2d_process_batch")
 
+    def set_solve_variant(self, variant: int):
+        N.check(self.lib.r2d_ctx_set_solve_variant(self.handle, variant), "r2d_ctx_set_solve_variant")
+
     def set_profiling(self, on: bool):
         N.check(self.lib.r2d_ctx_set_profiling(self.handle, 1 if on else 0), "r2d_ctx_set_profiling")
 

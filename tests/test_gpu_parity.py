@@ -88,3 +88,31 @@ def test_batched_equals_single(gpu_ctx, oracle):
     for w, r in zip(batch, refs):
         assert bits_equal(w.bodies(), r.bodies()), first_diff(w.bodies(), r.bodies())
         w.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
+    """Every launch configuration of the solve kernel (register slots x
+    workgroup size, including the fallback for worlds with more items than
+    slots) gives the oracle's bits."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    specs = [scenes.batched_world(21, n_min=40, n_max=60), scenes.batched_world(22),
+             scenes.pyramid(rows=16), scenes.mixed(900, seed=3, width=20.0, height=30.0, ground_w=30.0)]
+    gw = [GpuWorld(gpu_ctx) for _ in specs]
+    ow = [oracle.OracleWorld("f32") for _ in specs]
+    for sc, g, o in zip(specs, gw, ow):
+        apply_scene(sc, g)
+        apply_scene(sc, o)
+    gpu_ctx.set_solve_variant(variant)
+    try:
+        gpu_ctx.process_batch_handles([g.handle for g in gw], scenes.DT, 4, 4, 60)
+    finally:
+        gpu_ctx.set_solve_variant(-1)
+    for o in ow:
+        for _ in range(60):
+            assert o.process(scenes.DT, 4, 4) == 0
+    for i, (g, o) in enumerate(zip(gw, ow)):
+        assert bits_equal(g.bodies(), o.bodies()), (variant, i, first_diff(g.bodies(), o.bodies()))
+        assert bits_equal(g.manifolds(), o.manifolds()), (variant, i)
+        g.close()
