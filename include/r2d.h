@@ -206,10 +206,9 @@ int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
 int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
 /* Launch configuration of the solve kernel: -1 (default) picks by world
- * size; 0..4 select a variant (register slots per thread x workgroup size:
- * 0: none x 1024, 1: 4 x 256, 2: 2 x 512, 3: 2 x 256, 4: none x 256).  Every
- * variant gives the same bits; the autotuner (resolve2d_amd/autotune.py)
- * picks the fastest for a batch shape. */
+ * size; 0..6 select the register-slot kernel's slots x workgroup size and the
+ * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner
+ * (resolve2d_amd/autotune.py) picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);
 /* Totals of the last step of the context: entries, raw candidates,
  * manifolds, solver items (out[4]). */

@@ -332,7 +332,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.bkt_items, E * 4);
     add((void**)&S.ent_body, E * 4); add((void**)&S.ent_bkt, E * 4);
     add((void**)&S.ent_cnt, (E + 1) * 4); add((void**)&S.ent_off, (E + 1) * 4);
-    add((void**)&S.cand_a, R * 4); add((void**)&S.cand_b, R * 4);
+    add((void**)&S.cand_a, R * 4); add((void**)&S.cand_b, R * 4); add((void**)&S.cand_k, R * 4);
     add((void**)&S.cand_flag, (R + 1) * 4); add((void**)&S.cand_idx, (R + 1) * 4);
     add((void**)&S.sat_nx, R * 4); add((void**)&S.sat_ny, R * 4); add((void**)&S.sat_info, R * 4);
     add((void**)&S.m_ref, M * 4); add((void**)&S.m_inc, M * 4);
@@ -347,6 +347,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.lists, 2 * I * 4);
     add((void**)&S.it_color, I * 4);
     add((void**)&S.it_perm, I * 4);
+    add((void**)&S.recs, I * 128);
     add((void**)&S.chist, 130 * (W + 1) * 4);
     add((void**)&S.cmask, 2 * N * 8);
     add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
@@ -460,7 +461,7 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
         int v = std::atoi(e);
         if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
-    const unsigned threads = solve_variant_threads(variant);
+    const unsigned threads = solve_variant_threads(variant) + 64u * (unsigned)solve_variant_slots(variant);
     int first_err = R2D_OK;
     size_t done = 0;
     for (int attempt = 0; done < steps; ++attempt) {

@@ -88,7 +88,7 @@ struct StepDev {
     // entries -> raw candidates
     uint32_t *ent_body, *ent_bkt;  // [cap_e]
     uint32_t *ent_cnt, *ent_off;   // [cap_e+1]
-    uint32_t *cand_a, *cand_b;     // [cap_r]
+    uint32_t *cand_a, *cand_b, *cand_k;   // [cap_r] lower body, partner, cell index in the lower body
     uint32_t *cand_flag, *cand_idx;      // [cap_r+1]
     float *sat_nx, *sat_ny;
     uint32_t* sat_info;      // normal id | ref-is-lower-slot << 2 | collides << 3
@@ -109,6 +109,7 @@ struct StepDev {
     uint32_t* it_perm;                     // [cap_i] world-local items in execution order
     uint32_t* chist;                       // [130*W] colour histogram per world
     uint64_t* cmask;                       // [2*N] colours used per body (coloured schedule)
+    uint4* recs;                           // [8*cap_i] solver item records (128 B each)
     // global-memory solver state (worlds too large for LDS)
     float *g_im, *g_ii;
     uint32_t* g_cnt;

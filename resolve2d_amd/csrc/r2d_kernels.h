@@ -21,9 +21,8 @@ size_t solve_lds_bytes(int nbodies);
 // the four phases of one step, enqueued in this order on one stream
 enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
-// solve-kernel variants (register slots per thread x workgroup size); 0 is
-// the no-slot 1024-thread kernel for large worlds
-constexpr int SOLVE_VARIANTS = 5;
+// solve-kernel variants (workgroup size x waves per SIMD; table in r2d_kernels.hip)
+constexpr int SOLVE_VARIANTS = 7;
 unsigned solve_variant_threads(int v);
 int solve_variant_slots(int v);
 

@@ -452,7 +452,8 @@ __global__ void k_cand_emit(StepDev S)
         uint32_t a = S.ent_body[e], gb = S.ent_bkt[e];
         const uint32_t* it = S.bkt_items + S.bkt_off[gb];
         uint32_t o = S.ent_off[e], c = S.ent_cnt[e];
-        for (uint32_t j = 0; j < c; ++j) { S.cand_a[o + j] = a; S.cand_b[o + j] = it[j]; }
+        uint32_t k = e - S.cell_off[a];
+        for (uint32_t j = 0; j < c; ++j) { S.cand_a[o + j] = a; S.cand_b[o + j] = it[j]; S.cand_k[o + j] = k; }
     }
 }
 
@@ -465,13 +466,23 @@ __global__ void k_narrow(StepDev S)
     if (S.counters[C_OVERFLOW]) return;
     uint32_t R = dev_count(&S.counters[C_RAW], S.cap_r);
     for (uint32_t q = grid_tid(); q < R; q += grid_stride()) {
-        uint32_t a = S.cand_a[q], b = S.cand_b[q];
-        uint32_t seg = S.ent_off[S.cell_off[a]];
+        uint32_t a = S.cand_a[q], b = S.cand_b[q], k = S.cand_k[q];
         uint32_t flag = 0, info = 0;
         float nx = 0.f, ny = 0.f;
+        // b first appears in a's query list at cell k unless an earlier cell of a
+        // hashes to a bucket b also occupies (buckets are deduplicated, so
+        // within one bucket b appears once)
         bool first = true;
-        for (uint32_t p = seg; p < q; ++p)
-            if (S.cand_b[p] == b) { first = false; break; }
+        {
+            const uint32_t* ba = S.ent_bkt + S.cell_off[a];
+            const uint32_t* bb = S.ent_bkt + S.cell_off[b];
+            uint32_t nb = S.ncell[b];
+            for (uint32_t j = 0; j < nb && first; ++j) {
+                uint32_t h = bb[j];
+                for (uint32_t kk = 0; kk < k; ++kk)
+                    if (ba[kk] == h) { first = false; break; }
+            }
+        }
         if (first) {
             uint32_t w = S.bworld[a];
             const WorldDev& D = S.worlds[w];
@@ -816,67 +827,8 @@ __device__ __forceinline__ void st_rel(uint32_t* p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// CollisionManifold.calculateImpulses (collision.zig:135-218)
 template <class ST>
-__device__ __forceinline__ void solve_manifold(const StepDev& S, const WorldDev& D, const ST& st, uint32_t m,
-                                               uint32_t l1, uint32_t l2, bool w1, bool w2, float dt)
-{
-    float* P = S.m_pt;
-    const size_t cm = S.cap_m;
-    float im1 = st.im[l1], ii1 = st.ii[l1], im2 = st.im[l2], ii2 = st.ii[l2];
-    v2 vl1 = scale2(mk(st.mx[l1], st.my[l1]), im1);
-    float om1 = st.L[l1] * ii1;
-    v2 vl2 = scale2(mk(st.mx[l2], st.my[l2]), im2);
-    float om2 = st.L[l2] * ii2;
-    v2 n = mk(S.m_nx[m], S.m_ny[m]);
-    v2 t = rot90cw(n);
-    float fric = S.m_fric[m];
-    uint32_t valid = S.m_valid[m];
-    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
-    float rot1 = 0.f, rot2 = 0.f;
-    const uint32_t st1 = D.flags[l1] & BF_STATIC;
-    const uint32_t st2 = D.flags[l2] & BF_STATIC;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        if (!(valid & (1u << p))) continue;
-        float depth = P[(MP_DEPTH * 2 + p) * cm + m];
-        if (depth >= 0.f) {
-            P[(MP_ACC_PN * 2 + p) * cm + m] = 0.f;
-            P[(MP_ACC_PT * 2 + p) * cm + m] = 0.f;
-            continue;
-        }
-        v2 r1 = mk(P[(MP_R1X * 2 + p) * cm + m], P[(MP_R1Y * 2 + p) * cm + m]);
-        v2 r2 = mk(P[(MP_R2X * 2 + p) * cm + m], P[(MP_R2Y * 2 + p) * cm + m]);
-        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
-        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
-        v2 dv = sub2(v1, v2v);
-        float bias = BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
-        float num = dot2(dv, n) + bias;
-        float pn = num * P[(MP_MASS_N * 2 + p) * cm + m];
-        if (pn < MIN_MANIFOLD_IMPULSE) continue;
-        num = dot2(dv, t);
-        float pt = num * P[(MP_MASS_T * 2 + p) * cm + m];
-        float acc_pn = P[(MP_ACC_PN * 2 + p) * cm + m];
-        float acc_pt = P[(MP_ACC_PT * 2 + p) * cm + m];
-        float new_pn = maxn(0.0f, acc_pn + pn);
-        float applied_pn = new_pn - acc_pn;
-        acc_pn = new_pn;
-        float max_pt = fric * absf(acc_pn);
-        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
-        float applied_pt = new_pt - acc_pt;
-        acc_pt = new_pt;
-        P[(MP_ACC_PN * 2 + p) * cm + m] = acc_pn;
-        P[(MP_ACC_PT * 2 + p) * cm + m] = acc_pt;
-        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
-        if (!st1) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
-        if (!st2) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
-    }
-    if (w1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
-    if (w2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
-}
-
-template <class ST>
-__device__ __forceinline__ void solve_joint(const WorldDev& D, const ST& st, const Joint& J, float dt)
+__device__ __noinline__ void solve_joint(const WorldDev& D, const ST st, const Joint& J, float dt)
 {
     if (J.type == 0) {
         // DistanceJoint.solve (DistanceJoint.zig:40-75)
@@ -969,186 +921,266 @@ __device__ __forceinline__ void solve_joint(const WorldDev& D, const ST& st, con
 // order) that touches one of its ordering bodies has run, which gives the
 // same bits as the sequential loop.
 
-// --------------------------------------------------- register-resident items
-// One thread owns the items at execution positions tid, tid + blockDim, ...
-// (at most NSLOT of them) and keeps each item's contact geometry, effective
-// masses and accumulated impulses in registers for the whole step.
-struct Slot {
-    uint32_t flags;          // SL_* bits
-    uint32_t j;              // world-local item index
-    int32_t r0, r1;          // ordering bodies (world-local) or -1
-    uint32_t rk0, rk1;       // ranks at r0 / r1
-    uint32_t dg0, dg1;       // item counts of r0 / r1 (targets advance by these per iteration)
-    uint32_t l1, l2;         // manifold bodies (ref, inc), world-local
-    uint32_t m;              // manifold index in the step's arrays
-    float nx, ny, fric, prev1, prev2;
-    float r1x[2], r1y[2], r2x[2], r2y[2], depth[2], odepth[2], mn[2], mt[2], apn[2], apt[2];
+// ------------------------------------------------------- solver item records
+// Each solver item of a step gets one 128-byte record at its execution
+// position, built in parallel at the first sub-step: the ordering bodies and
+// ranks, and for a manifold its normal, friction, both points' lever arms,
+// depths, effective masses (identical every sub-step unless updateTGSDepth
+// moves the point) and accumulated impulses.  A worker reads its next record
+// while it still waits on the current one, so a link of a dependency chain
+// touches only LDS body state.
+enum : uint32_t {
+    RF_JOINT = 1u, RF_P0 = 2u, RF_P1 = 4u, RF_W1 = 8u, RF_W2 = 16u, RF_S1 = 32u, RF_S2 = 64u,
+    RF_TGS = 128u            // a point's depth is not its original depth: updateTGSDepth is live
 };
-enum : uint32_t { SL_USED = 1u, SL_JOINT = 2u, SL_P0 = 4u, SL_P1 = 8u, SL_W1 = 16u, SL_W2 = 32u, SL_S1 = 64u, SL_S2 = 128u };
+constexpr uint32_t NONE = 0xffffffffu;
+
+// words: 0 flags, 1 body a (manifold ref / joint r0), 2 body b (inc / r1),
+// 3 index (manifold in the step arrays / world-local joint), 4-5 ranks at the
+// ordering bodies, 6-8 normal + friction, 9-15 point 0 (r1x r1y r2x r2y depth
+// mass_n mass_t), 16-22 point 1, 24-27 impulses (pn0 pt0 pn1 pt1)
+struct Rec {
+    uint32_t flags, a, b, idx, rk0, rk1;
+    float nx, ny, fric;
+    float p[2][7];
+    float acc[4];
+};
+
+__device__ __forceinline__ void rec_load(Rec& r, const uint4* q)
+{
+    uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3], v4 = q[4], v5 = q[5], v6 = q[6];
+    r.flags = v0.x; r.a = v0.y; r.b = v0.z; r.idx = v0.w;
+    r.rk0 = v1.x; r.rk1 = v1.y;
+    r.nx = __uint_as_float(v1.z); r.ny = __uint_as_float(v1.w);
+    r.fric = __uint_as_float(v2.x);
+    r.p[0][0] = __uint_as_float(v2.y); r.p[0][1] = __uint_as_float(v2.z); r.p[0][2] = __uint_as_float(v2.w);
+    r.p[0][3] = __uint_as_float(v3.x); r.p[0][4] = __uint_as_float(v3.y); r.p[0][5] = __uint_as_float(v3.z);
+    r.p[0][6] = __uint_as_float(v3.w);
+    r.p[1][0] = __uint_as_float(v4.x); r.p[1][1] = __uint_as_float(v4.y); r.p[1][2] = __uint_as_float(v4.z);
+    r.p[1][3] = __uint_as_float(v4.w); r.p[1][4] = __uint_as_float(v5.x); r.p[1][5] = __uint_as_float(v5.y);
+    r.p[1][6] = __uint_as_float(v5.z);
+    r.acc[0] = __uint_as_float(v6.x); r.acc[1] = __uint_as_float(v6.y);
+    r.acc[2] = __uint_as_float(v6.z); r.acc[3] = __uint_as_float(v6.w);
+}
+
+__device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
+{
+    q[0] = make_uint4(r.flags, r.a, r.b, r.idx);
+    q[1] = make_uint4(r.rk0, r.rk1, __float_as_uint(r.nx), __float_as_uint(r.ny));
+    q[2] = make_uint4(__float_as_uint(r.fric), __float_as_uint(r.p[0][0]), __float_as_uint(r.p[0][1]),
+                      __float_as_uint(r.p[0][2]));
+    q[3] = make_uint4(__float_as_uint(r.p[0][3]), __float_as_uint(r.p[0][4]), __float_as_uint(r.p[0][5]),
+                      __float_as_uint(r.p[0][6]));
+    q[4] = make_uint4(__float_as_uint(r.p[1][0]), __float_as_uint(r.p[1][1]), __float_as_uint(r.p[1][2]),
+                      __float_as_uint(r.p[1][3]));
+    q[5] = make_uint4(__float_as_uint(r.p[1][4]), __float_as_uint(r.p[1][5]), __float_as_uint(r.p[1][6]), 0u);
+    q[6] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
+                      __float_as_uint(r.acc[3]));
+}
+
+__device__ __forceinline__ int32_t rec_r0(const Rec& r)
+{
+    if (r.flags & RF_JOINT) return r.a == NONE ? -1 : (int32_t)r.a;
+    return (r.flags & RF_W1) ? (int32_t)r.a : -1;
+}
+__device__ __forceinline__ int32_t rec_r1(const Rec& r)
+{
+    if (r.flags & RF_JOINT) return r.b == NONE ? -1 : (int32_t)r.b;
+    return (r.flags & RF_W2) ? (int32_t)r.b : -1;
+}
 
 // CollisionManifold.preStep effective masses (collision.zig:117-132) for one point
 template <class ST>
-__device__ __forceinline__ void slot_masses(Slot& z, const ST& st, int p)
+__device__ __forceinline__ void rec_masses(Rec& r, const ST& st, int p)
 {
-    float im1 = st.im[z.l1], im2 = st.im[z.l2], ii1 = st.ii[z.l1], ii2 = st.ii[z.l2];
+    float im1 = st.im[r.a], im2 = st.im[r.b], ii1 = st.ii[r.a], ii2 = st.ii[r.b];
     float inv_mass = im1 + im2;
-    v2 n = mk(z.nx, z.ny), t = rot90cw(n);
-    v2 r1 = mk(z.r1x[p], z.r1y[p]), r2 = mk(z.r2x[p], z.r2y[p]);
+    v2 n = mk(r.nx, r.ny), t = rot90cw(n);
+    v2 r1 = mk(r.p[p][0], r.p[p][1]), r2 = mk(r.p[p][2], r.p[p][3]);
     float r1n = cross2(r1, n), r2n = cross2(r2, n);
     float kn = inv_mass + ii1 * (r1n * r1n) + ii2 * (r2n * r2n);
     float r1t = cross2(r1, t), r2t = cross2(r2, t);
     float kt = inv_mass + ii1 * (r1t * r1t) + ii2 * (r2t * r2t);
-    z.mn[p] = kn > 0.0f ? 1.0f / kn : 0.0f;
-    z.mt[p] = kt > 0.0f ? 1.0f / kt : 0.0f;
+    r.p[p][5] = kn > 0.0f ? 1.0f / kn : 0.0f;
+    r.p[p][6] = kt > 0.0f ? 1.0f / kt : 0.0f;
 }
 
+// record of the item at execution position `pos` (first sub-step)
 template <class ST>
-__device__ __forceinline__ void slot_load(Slot& z, const StepDev& S, const WorldDev& D, const ST& st, uint32_t pos,
+__device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldDev& D, const ST& st, uint32_t pos,
                                           uint32_t ibase, uint32_t nj, uint32_t mbeg, uint32_t base)
 {
     const size_t cm = S.cap_m;
     const float* P = S.m_pt;
-    z.j = S.it_perm[ibase + pos];
-    uint32_t item = ibase + z.j;
-    z.r0 = S.it_r0[item];
-    z.r1 = S.it_r1[item];
-    z.flags = SL_USED;
-    if (z.r0 >= 0) { z.r0 -= (int32_t)base; z.rk0 = S.it_k0[item]; z.dg0 = st.deg[z.r0]; }
-    if (z.r1 >= 0) { z.r1 -= (int32_t)base; z.rk1 = S.it_k1[item]; z.dg1 = st.deg[z.r1]; }
-    if (z.j < nj) { z.flags |= SL_JOINT; return; }
-    uint32_t m = mbeg + (z.j - nj);
-    z.m = m;
-    z.l1 = S.m_ref[m] - base;
-    z.l2 = S.m_inc[m] - base;
-    if (S.sflag[base + z.l1] & SF_RESOURCE) z.flags |= SL_W1;
-    if (S.sflag[base + z.l2] & SF_RESOURCE) z.flags |= SL_W2;
-    if (D.flags[z.l1] & BF_STATIC) z.flags |= SL_S1;
-    if (D.flags[z.l2] & BF_STATIC) z.flags |= SL_S2;
-    z.nx = S.m_nx[m];
-    z.ny = S.m_ny[m];
-    z.fric = sqrt_rn(D.mu[z.l1] * D.mu[z.l2]);
-    z.prev1 = S.m_prev1[m];
-    z.prev2 = S.m_prev2[m];
+    uint32_t j = S.it_perm[ibase + pos];
+    uint32_t item = ibase + j;
+    r.rk0 = S.it_k0[item];
+    r.rk1 = S.it_k1[item];
+    r.nx = r.ny = r.fric = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int f = 0; f < 7; ++f) r.p[p][f] = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) r.acc[f] = 0.0f;
+    if (j < nj) {
+        int32_t r0 = S.it_r0[item], r1 = S.it_r1[item];
+        r.flags = RF_JOINT;
+        r.a = r0 >= 0 ? (uint32_t)(r0 - (int32_t)base) : NONE;
+        r.b = r1 >= 0 ? (uint32_t)(r1 - (int32_t)base) : NONE;
+        r.idx = j;
+        return;
+    }
+    uint32_t m = mbeg + (j - nj);
+    r.idx = m;
+    r.a = S.m_ref[m] - base;
+    r.b = S.m_inc[m] - base;
+    r.flags = 0u;
+    if (S.sflag[base + r.a] & SF_RESOURCE) r.flags |= RF_W1;
+    if (S.sflag[base + r.b] & SF_RESOURCE) r.flags |= RF_W2;
+    if (D.flags[r.a] & BF_STATIC) r.flags |= RF_S1;
+    if (D.flags[r.b] & BF_STATIC) r.flags |= RF_S2;
+    r.nx = S.m_nx[m];
+    r.ny = S.m_ny[m];
+    r.fric = sqrt_rn(D.mu[r.a] * D.mu[r.b]);   // preStep (collision.zig:114)
     uint32_t valid = S.m_valid[m];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (valid & (1u << p)) z.flags |= (SL_P0 << p);
-        z.r1x[p] = P[(MP_R1X * 2 + p) * cm + m];
-        z.r1y[p] = P[(MP_R1Y * 2 + p) * cm + m];
-        z.r2x[p] = P[(MP_R2X * 2 + p) * cm + m];
-        z.r2y[p] = P[(MP_R2Y * 2 + p) * cm + m];
-        z.depth[p] = P[(MP_DEPTH * 2 + p) * cm + m];
-        z.odepth[p] = P[(MP_ODEPTH * 2 + p) * cm + m];
-        z.apn[p] = 0.0f;
-        z.apt[p] = 0.0f;
-        slot_masses(z, st, p);
+        if (!(valid & (1u << p))) continue;
+        r.flags |= (RF_P0 << p);
+        r.p[p][0] = P[(MP_R1X * 2 + p) * cm + m];
+        r.p[p][1] = P[(MP_R1Y * 2 + p) * cm + m];
+        r.p[p][2] = P[(MP_R2X * 2 + p) * cm + m];
+        r.p[p][3] = P[(MP_R2Y * 2 + p) * cm + m];
+        r.p[p][4] = P[(MP_DEPTH * 2 + p) * cm + m];
+        if (!approx(r.p[p][4], P[(MP_ODEPTH * 2 + p) * cm + m], 1e-4f)) r.flags |= RF_TGS;
+        rec_masses(r, st, p);
     }
 }
 
-// updateTGSDepth (collision.zig:73-100): points whose depth left its original
-// value are moved with the bodies' rotation since the last sub-step; the
-// effective masses then follow the new lever arms (preStep runs every sub-step)
+// updateTGSDepth (collision.zig:73-100) for a record whose points may move:
+// rotate lever arms by the bodies' rotation since the previous sub-step,
+// recompute depth and (preStep) the effective masses
 template <class ST>
-__device__ __forceinline__ void slot_tgs(Slot& z, const WorldDev& D, const ST& st)
+__device__ __forceinline__ void rec_tgs(Rec& r, const StepDev& S, const WorldDev& D, const ST& st, bool = false)
 {
-    float a1 = D.ang[z.l1], a2 = D.ang[z.l2];
+    const uint32_t m = r.idx;
+    const size_t cm = S.cap_m;
+    const float* P = S.m_pt;
+    float a1 = D.ang[r.a], a2 = D.ang[r.b];
+    float prev1 = S.m_prev1[m], prev2 = S.m_prev2[m];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (!(z.flags & (SL_P0 << p))) continue;
-        if (approx(z.depth[p], z.odepth[p], 1e-4f)) continue;
-        v2 rr1 = rotate2(mk(z.r1x[p], z.r1y[p]), a1 - z.prev1);
-        v2 rr2 = rotate2(mk(z.r2x[p], z.r2y[p]), a2 - z.prev2);
-        v2 q1 = add2(rr1, mk(D.px[z.l1], D.py[z.l1]));
-        v2 q2 = add2(rr2, mk(D.px[z.l2], D.py[z.l2]));
-        float dd = dot2(mk(z.nx, z.ny), sub2(q2, q1));
-        z.depth[p] = dd + z.odepth[p];
-        z.r1x[p] = rr1.x; z.r1y[p] = rr1.y; z.r2x[p] = rr2.x; z.r2y[p] = rr2.y;
-        slot_masses(z, st, p);
+        if (!(r.flags & (RF_P0 << p))) continue;
+        float od = P[(MP_ODEPTH * 2 + p) * cm + m];
+        if (approx(r.p[p][4], od, 1e-4f)) continue;
+        v2 rr1 = rotate2(mk(r.p[p][0], r.p[p][1]), a1 - prev1);
+        v2 rr2 = rotate2(mk(r.p[p][2], r.p[p][3]), a2 - prev2);
+        v2 q1 = add2(rr1, mk(D.px[r.a], D.py[r.a]));
+        v2 q2 = add2(rr2, mk(D.px[r.b], D.py[r.b]));
+        float dd = dot2(mk(r.nx, r.ny), sub2(q2, q1));
+        r.p[p][4] = dd + od;
+        r.p[p][0] = rr1.x; r.p[p][1] = rr1.y; r.p[p][2] = rr2.x; r.p[p][3] = rr2.y;
+        rec_masses(r, st, p);
     }
-    z.prev1 = a1;
-    z.prev2 = a2;
+    S.m_prev1[m] = a1;
+    S.m_prev2[m] = a2;
 }
 
-// CollisionManifold.calculateImpulses (collision.zig:135-218) on a register slot
+// CollisionManifold.calculateImpulses (collision.zig:135-218) on a record
 template <class ST>
-__device__ __forceinline__ void slot_impulses(Slot& z, const ST& st, float dt)
+__device__ __forceinline__ void rec_impulses(Rec& r, const ST& st, float dt)
 {
-    const uint32_t l1 = z.l1, l2 = z.l2;
+    const uint32_t l1 = r.a, l2 = r.b;
     float im1 = st.im[l1], ii1 = st.ii[l1], im2 = st.im[l2], ii2 = st.ii[l2];
     v2 vl1 = scale2(mk(st.mx[l1], st.my[l1]), im1);
     float om1 = st.L[l1] * ii1;
     v2 vl2 = scale2(mk(st.mx[l2], st.my[l2]), im2);
     float om2 = st.L[l2] * ii2;
-    v2 n = mk(z.nx, z.ny);
+    v2 n = mk(r.nx, r.ny);
     v2 t = rot90cw(n);
     v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
     float rot1 = 0.f, rot2 = 0.f;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (!(z.flags & (SL_P0 << p))) continue;
-        float depth = z.depth[p];
-        if (depth >= 0.f) { z.apn[p] = 0.f; z.apt[p] = 0.f; continue; }
-        v2 r1 = mk(z.r1x[p], z.r1y[p]), r2 = mk(z.r2x[p], z.r2y[p]);
+        if (!(r.flags & (RF_P0 << p))) continue;
+        float depth = r.p[p][4];
+        if (depth >= 0.f) { r.acc[2 * p] = 0.f; r.acc[2 * p + 1] = 0.f; continue; }
+        v2 r1 = mk(r.p[p][0], r.p[p][1]), r2 = mk(r.p[p][2], r.p[p][3]);
         v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
         v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
         v2 dv = sub2(v1, v2v);
         float bias = BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
         float num = dot2(dv, n) + bias;
-        float pn = num * z.mn[p];
+        float pn = num * r.p[p][5];
         if (pn < MIN_MANIFOLD_IMPULSE) continue;
         num = dot2(dv, t);
-        float pt = num * z.mt[p];
-        float acc_pn = z.apn[p], acc_pt = z.apt[p];
+        float pt = num * r.p[p][6];
+        float acc_pn = r.acc[2 * p], acc_pt = r.acc[2 * p + 1];
         float new_pn = maxn(0.0f, acc_pn + pn);
         float applied_pn = new_pn - acc_pn;
         acc_pn = new_pn;
-        float max_pt = z.fric * absf(acc_pn);
+        float max_pt = r.fric * absf(acc_pn);
         float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
         float applied_pt = new_pt - acc_pt;
-        z.apn[p] = acc_pn;
-        z.apt[p] = new_pt;
+        r.acc[2 * p] = acc_pn;
+        r.acc[2 * p + 1] = new_pt;
         v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
-        if (!(z.flags & SL_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
-        if (!(z.flags & SL_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
+        if (!(r.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
+        if (!(r.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
     }
-    if (z.flags & SL_W1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
-    if (z.flags & SL_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
+    if (r.flags & RF_W1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
+    if (r.flags & RF_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
 }
 
-__device__ __forceinline__ void slot_store(const Slot& z, const StepDev& S)
+// record -> the step's manifold arrays (what r2d_get_manifolds reports)
+__device__ __forceinline__ void rec_writeback(const Rec& r, const StepDev& S)
 {
-    if (!(z.flags & SL_USED) || (z.flags & SL_JOINT)) return;
+    if (r.flags & RF_JOINT) return;
     const size_t cm = S.cap_m;
     float* P = S.m_pt;
-    const uint32_t m = z.m;
-    S.m_fric[m] = z.fric;
-    S.m_prev1[m] = z.prev1;
-    S.m_prev2[m] = z.prev2;
+    const uint32_t m = r.idx;
+    S.m_fric[m] = r.fric;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (!(z.flags & (SL_P0 << p))) continue;
-        P[(MP_ACC_PN * 2 + p) * cm + m] = z.apn[p];
-        P[(MP_ACC_PT * 2 + p) * cm + m] = z.apt[p];
-        P[(MP_R1X * 2 + p) * cm + m] = z.r1x[p];
-        P[(MP_R1Y * 2 + p) * cm + m] = z.r1y[p];
-        P[(MP_R2X * 2 + p) * cm + m] = z.r2x[p];
-        P[(MP_R2Y * 2 + p) * cm + m] = z.r2y[p];
-        P[(MP_DEPTH * 2 + p) * cm + m] = z.depth[p];
-        P[(MP_MASS_N * 2 + p) * cm + m] = z.mn[p];
-        P[(MP_MASS_T * 2 + p) * cm + m] = z.mt[p];
+        if (!(r.flags & (RF_P0 << p))) continue;
+        P[(MP_ACC_PN * 2 + p) * cm + m] = r.acc[2 * p];
+        P[(MP_ACC_PT * 2 + p) * cm + m] = r.acc[2 * p + 1];
+        P[(MP_MASS_N * 2 + p) * cm + m] = r.p[p][5];
+        P[(MP_MASS_T * 2 + p) * cm + m] = r.p[p][6];
+        P[(MP_R1X * 2 + p) * cm + m] = r.p[p][0];
+        P[(MP_R1Y * 2 + p) * cm + m] = r.p[p][1];
+        P[(MP_R2X * 2 + p) * cm + m] = r.p[p][2];
+        P[(MP_R2Y * 2 + p) * cm + m] = r.p[p][3];
+        P[(MP_DEPTH * 2 + p) * cm + m] = r.p[p][4];
     }
 }
 
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
+constexpr size_t SLOT_LDS_MAX = 150 * 1024;
+// LDS of the register-slot kernel: 13 words of body state per body
+__host__ __device__ inline size_t slot_lds_bytes(int nbodies) { return (size_t)nbodies * 13u * 4u; }
 
-template <bool LDS, int NSLOT, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
+// One workgroup steps one world: S sub-steps of forces, integration,
+// pre-step and K solver iterations (lib.zig:199-250).  Solver items run as a
+// dataflow: an item may run once every earlier item (in the schedule's order)
+// that touches one of its ordering bodies has run, which gives the same bits
+// as the sequential loop.  Workers own execution positions tid, tid+blockDim,
+// ... and take them in (iteration, position) order, the order the ranks use,
+// so the earliest unfinished item always has a worker.
+template <bool LDS, int MAXT, int WAVES = 1>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
+void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = blockIdx.x;
     if (S.w_status[w] != 0) return;
+    // worlds whose items fit the register-slot kernel were stepped by it
+    if (S.w_item_base[w + 1] - S.w_item_base[w] <= slot_cap && S.worlds[w].nbodies < 0xffff &&
+        slot_lds_bytes(S.worlds[w].nbodies) <= SLOT_LDS_MAX)
+        return;
     if (threadIdx.x == 0) s_abort = 0u;
     const WorldDev& D = S.worlds[w];
     const int n = D.nbodies;
@@ -1159,6 +1191,7 @@ __global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t su
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const float fsub = (float)sub_steps;
     const float sub_dt = dt / fsub;
+    uint4* recs = S.recs + (size_t)ibase * 8;
 
     SolverState<LDS> st;
     if (LDS) {
@@ -1182,26 +1215,13 @@ __global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t su
         st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
     }
     __syncthreads();
-
-    // register path when every item of this world has an owner slot
-    const bool use_reg = NSLOT > 0 && nitems <= (uint32_t)NSLOT * blockDim.x;
-    Slot sl[NSLOT > 0 ? NSLOT : 1];
-    uint32_t nslot = 0;
-    if (use_reg) {
-#pragma unroll
-        for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
-            sl[z].flags = 0u;
-            uint32_t pos = threadIdx.x + (uint32_t)z * blockDim.x;
-            if (pos < nitems) {
-                slot_load(sl[z], S, D, st, pos, ibase, nj, mbeg, base);
-                nslot = (uint32_t)z + 1u;
-            }
-        }
+    // records (first sub-step's preStep for every manifold)
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
+        Rec r;
+        rec_build(r, S, D, st, pos, ibase, nj, mbeg, base);
+        rec_store(r, recs + (size_t)pos * 8);
     }
 
-    const uint32_t M = S.w_man_begin[w + 1] - mbeg;
-    float* P = S.m_pt;
-    const size_t cm = S.cap_m;
     for (uint32_t s = 0; s < sub_steps; ++s) {
         // forces (lib.zig:200-205), AABB refresh + velocity integration (lib.zig:207-216)
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1251,125 +1271,80 @@ __global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t su
             st.cnt[i] = 0u;
         }
         __syncthreads();
-        // updateTGSDepth + preStep (collision.zig:73-133)
-        if (use_reg) {
-#pragma unroll
-            for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z)
-                if ((sl[z].flags & SL_USED) && !(sl[z].flags & SL_JOINT)) slot_tgs(sl[z], D, st);
-        }
-        for (uint32_t mm = threadIdx.x; !use_reg && mm < M; mm += blockDim.x) {
-            uint32_t m = mbeg + mm;
-            uint32_t l1 = S.m_ref[m] - base, l2 = S.m_inc[m] - base;
-            v2 n = mk(S.m_nx[m], S.m_ny[m]);
-            uint32_t valid = S.m_valid[m];
-            float a1 = D.ang[l1], a2 = D.ang[l2];
-            float prev1 = S.m_prev1[m], prev2 = S.m_prev2[m];
-            for (int p = 0; p < 2; ++p) {
-                if (!(valid & (1u << p))) continue;
-                float depth = P[(MP_DEPTH * 2 + p) * cm + m], od = P[(MP_ODEPTH * 2 + p) * cm + m];
-                if (approx(depth, od, 1e-4f)) continue;
-                v2 r1 = mk(P[(MP_R1X * 2 + p) * cm + m], P[(MP_R1Y * 2 + p) * cm + m]);
-                v2 r2 = mk(P[(MP_R2X * 2 + p) * cm + m], P[(MP_R2Y * 2 + p) * cm + m]);
-                v2 rr1 = rotate2(r1, a1 - prev1), rr2 = rotate2(r2, a2 - prev2);
-                v2 q1 = add2(rr1, mk(D.px[l1], D.py[l1]));
-                v2 q2 = add2(rr2, mk(D.px[l2], D.py[l2]));
-                float dd = dot2(n, sub2(q2, q1));
-                P[(MP_DEPTH * 2 + p) * cm + m] = dd + od;
-                P[(MP_R1X * 2 + p) * cm + m] = rr1.x; P[(MP_R1Y * 2 + p) * cm + m] = rr1.y;
-                P[(MP_R2X * 2 + p) * cm + m] = rr2.x; P[(MP_R2Y * 2 + p) * cm + m] = rr2.y;
-            }
-            S.m_prev1[m] = a1; S.m_prev2[m] = a2;
-            float im1 = st.im[l1], im2 = st.im[l2], ii1 = st.ii[l1], ii2 = st.ii[l2];
-            float inv_mass = im1 + im2;
-            v2 t = rot90cw(n);
-            S.m_fric[m] = sqrt_rn(D.mu[l1] * D.mu[l2]);
-            for (int p = 0; p < 2; ++p) {
-                if (!(valid & (1u << p))) continue;
-                v2 r1 = mk(P[(MP_R1X * 2 + p) * cm + m], P[(MP_R1Y * 2 + p) * cm + m]);
-                v2 r2 = mk(P[(MP_R2X * 2 + p) * cm + m], P[(MP_R2Y * 2 + p) * cm + m]);
-                float r1n = cross2(r1, n), r2n = cross2(r2, n);
-                float kn = inv_mass + ii1 * (r1n * r1n) + ii2 * (r2n * r2n);
-                float r1t = cross2(r1, t), r2t = cross2(r2, t);
-                float kt = inv_mass + ii1 * (r1t * r1t) + ii2 * (r2t * r2t);
-                P[(MP_MASS_N * 2 + p) * cm + m] = kn > 0.0f ? 1.0f / kn : 0.0f;
-                P[(MP_MASS_T * 2 + p) * cm + m] = kt > 0.0f ? 1.0f / kt : 0.0f;
+        // updateTGSDepth (collision.zig:73-100): live only for records flagged at
+        // build time; every manifold's previous angles end as the last sub-step's
+        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
+            uint4* q = recs + (size_t)pos * 8;
+            uint4 h = q[0];
+            if (h.x & RF_JOINT) continue;
+            if (h.x & RF_TGS) {
+                Rec r;
+                rec_load(r, q);
+                rec_tgs(r, S, D, st);
+                rec_store(r, q);
+            } else if (s + 1 == sub_steps) {
+                S.m_prev1[h.w] = D.ang[h.y];
+                S.m_prev2[h.w] = D.ang[h.z];
             }
         }
         __syncthreads();
         // K iterations of (constraints, manifolds) as one ordered dataflow
-        if (use_reg) {
-            uint32_t k = 0, cz = 0, spins = 0;
-            if (nslot == 0) k = iters;
-            while (k < iters) {
-                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                bool ready = false;
-#pragma unroll
-                for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
-                    if ((uint32_t)z == cz) {
-                        ready = true;
-                        if (sl[z].r0 >= 0) ready = ld_acq(&st.cnt[sl[z].r0]) == k * sl[z].dg0 + sl[z].rk0;
-                        if (ready && sl[z].r1 >= 0) ready = ld_acq(&st.cnt[sl[z].r1]) == k * sl[z].dg1 + sl[z].rk1;
-                    }
-                }
-                if (ready) {
-#pragma unroll
-                    for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) {
-                        if ((uint32_t)z == cz) {
-                            if (sl[z].flags & SL_JOINT) solve_joint(D, st, D.joints[sl[z].j], sub_dt);
-                            else slot_impulses(sl[z], st, sub_dt);
-                            if (sl[z].r0 >= 0) st_rel(&st.cnt[sl[z].r0], k * sl[z].dg0 + sl[z].rk0 + 1u);
-                            if (sl[z].r1 >= 0) st_rel(&st.cnt[sl[z].r1], k * sl[z].dg1 + sl[z].rk1 + 1u);
-                        }
-                    }
-                    spins = 0;
-                    if (++cz == nslot) { cz = 0; ++k; }
-                } else {
-                    if (++spins > SPIN_LIMIT) {
-                        __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        S.w_status[w] = 5;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
+        {
+            const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
+            uint32_t pcur = p_begin, kcur = (p_begin < nitems) ? 0u : iters;
+            Rec cur, nxt;
+            uint32_t cur_pos = 0, cur_k = 0, nxt_pos = 0, nxt_k = 0;
+            bool have = false, has_nxt = false;
+            if (kcur < iters) {
+                rec_load(nxt, recs + (size_t)pcur * 8);
+                nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
+                pcur += p_step;
+                if (pcur >= nitems) { pcur = p_begin; ++kcur; }
             }
-        } else {
-            const uint32_t total = iters * nitems;
-            uint32_t q = threadIdx.x;
-            uint32_t k = 0, j = 0;
             int32_t r0 = -1, r1 = -1;
             uint32_t t0 = 0, t1 = 0;
-            bool have = false;
             uint32_t spins = 0;
             while (true) {
                 if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 if (!have) {
-                    if (q >= total) break;
-                    k = q / nitems;
-                    j = S.it_perm[ibase + (q - k * nitems)];
-                    uint32_t item = ibase + j;
-                    r0 = S.it_r0[item]; r1 = S.it_r1[item];
-                    if (r0 >= 0) { r0 -= (int32_t)base; t0 = k * st.deg[r0] + S.it_k0[item]; }
-                    if (r1 >= 0) { r1 -= (int32_t)base; t1 = k * st.deg[r1] + S.it_k1[item]; }
+                    if (!has_nxt) break;
+                    cur = nxt;
+                    cur_pos = nxt_pos;
+                    cur_k = nxt_k;
+                    has_nxt = false;
                     have = true;
+                    // prefetch the following position; the same position in the
+                    // next iteration reads impulses this record is about to write,
+                    // so that case is forwarded from registers instead
+                    if (kcur < iters) {
+                        if (pcur != cur_pos) rec_load(nxt, recs + (size_t)pcur * 8);
+                        nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
+                        pcur += p_step;
+                        if (pcur >= nitems) { pcur = p_begin; ++kcur; }
+                    }
+                    r0 = rec_r0(cur);
+                    r1 = rec_r1(cur);
+                    if (r0 >= 0) t0 = cur_k * st.deg[r0] + cur.rk0;
+                    if (r1 >= 0) t1 = cur_k * st.deg[r1] + cur.rk1;
                 }
                 bool ready = true;
                 if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
                 if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
                 if (ready) {
-                    if (j < nj) {
-                        solve_joint(D, st, D.joints[j], sub_dt);
+                    if (cur.flags & RF_JOINT) {
+                        solve_joint(D, st, D.joints[cur.idx], sub_dt);
                     } else {
-                        uint32_t m = mbeg + (j - nj);
-                        uint32_t l1 = S.m_ref[m] - base, l2 = S.m_inc[m] - base;
-                        bool wr1 = (S.sflag[base + l1] & SF_RESOURCE) != 0;
-                        bool wr2 = (S.sflag[base + l2] & SF_RESOURCE) != 0;
-                        solve_manifold(S, D, st, m, l1, l2, wr1, wr2, sub_dt);
+                        rec_impulses(cur, st, sub_dt);
+                        recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
+                                                                   __float_as_uint(cur.acc[1]),
+                                                                   __float_as_uint(cur.acc[2]),
+                                                                   __float_as_uint(cur.acc[3]));
                     }
                     if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
                     if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                    if (has_nxt && nxt_pos == cur_pos) nxt = cur;
                     have = false;
                     spins = 0;
-                    q += blockDim.x;
                 } else {
                     // every item's predecessors precede it, so some item is always
                     // ready; a stall this long means corrupt ordering data: stop
@@ -1395,14 +1370,379 @@ __global__ __launch_bounds__(MAXT) void k_solve(StepDev S, float dt, uint32_t su
         }
         __syncthreads();
     }
-    if (use_reg) {
-#pragma unroll
-        for (int z = 0; z < (NSLOT > 0 ? NSLOT : 1); ++z) slot_store(sl[z], S);
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
+        Rec r;
+        rec_load(r, recs + (size_t)pos * 8);
+        rec_writeback(r, S);
     }
     if (LDS) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
         }
+    }
+}
+
+// ------------------------------------------------------ register-slot solver
+// For worlds with at most NSLOT * MAXT solver items (and < 65,535 bodies) each
+// thread keeps all of its items -- execution positions tid, tid + MAXT, ... --
+// in registers for the whole step, and the body state a chain link reads
+// (momentum, inverse masses, pose, cos / sin of the angle, inertia, torque)
+// sits in LDS, so the dependency-chain loop issues no global memory operation:
+// a wave's vector-memory wait is wave-wide, and any load inside the loop would
+// put HBM latency on every chain link.
+struct Slot {
+    uint32_t flags;          // RF_* bits | joint type << 24 | SLOT_USED; 0 = unused
+    uint32_t pair;           // manifold: ref | inc << 16; joint: r0 | r1 << 16 (0xffff = none)
+    uint32_t ranks;          // rank at r0 | rank at r1 << 16
+    uint32_t idx;            // manifold index / joint body slots (s1 | s2 << 16)
+    float nx, ny, fric;
+    float p[2][7];           // manifold: r1x r1y r2x r2y depth mass_n mass_t per point
+                             // joint: pmax pmin beta r1x r1y r2x r2y | target tx ty
+    float acc[4];            // pn0 pt0 pn1 pt1
+};
+constexpr uint32_t SLOT_USED = 0x80000000u;
+
+struct SlotState {
+    float *mx, *my, *L, *im, *ii;
+    uint32_t *cnt, *deg;
+    float *px, *py, *c, *s, *I, *tq;
+};
+
+__device__ __forceinline__ void slot_from_rec(Slot& z, const Rec& r)
+{
+    z.flags = r.flags | SLOT_USED;
+    z.pair = (r.a & 0xffffu) | (r.b << 16);
+    z.ranks = (r.rk0 & 0xffffu) | (r.rk1 << 16);
+    z.idx = r.idx;
+    z.nx = r.nx; z.ny = r.ny; z.fric = r.fric;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int f = 0; f < 7; ++f) z.p[p][f] = r.p[p][f];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) z.acc[f] = r.acc[f];
+}
+
+__device__ __forceinline__ void rec_from_slot(Rec& r, const Slot& z)
+{
+    r.flags = z.flags & ~SLOT_USED;
+    uint32_t a = z.pair & 0xffffu, b = z.pair >> 16;
+    r.a = (a == 0xffffu) ? NONE : a;
+    r.b = (b == 0xffffu) ? NONE : b;
+    r.rk0 = z.ranks & 0xffffu; r.rk1 = z.ranks >> 16;
+    r.idx = z.idx;
+    r.nx = z.nx; r.ny = z.ny; r.fric = z.fric;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int f = 0; f < 7; ++f) r.p[p][f] = z.p[p][f];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) r.acc[f] = z.acc[f];
+}
+
+// joint slot: parameters from the world's joint record
+__device__ __forceinline__ void slot_joint(Slot& z, const Joint& J, const WorldDev& D)
+{
+    z.flags |= (uint32_t)J.type << 24;
+    if (D.flags[J.s1] & BF_STATIC) z.flags |= RF_S1;
+    z.idx = ((uint32_t)J.s1 & 0xffffu) | ((uint32_t)(J.s2 < 0 ? 0xffff : J.s2) << 16);
+    z.p[0][0] = J.pmax; z.p[0][1] = J.pmin; z.p[0][2] = J.beta;
+    z.p[0][3] = J.r1x; z.p[0][4] = J.r1y; z.p[0][5] = J.r2x; z.p[0][6] = J.r2y;
+    z.p[1][0] = J.target; z.p[1][1] = J.tx; z.p[1][2] = J.ty;
+}
+
+// Constraint.solve for a joint slot; the arithmetic of solve_joint with pose,
+// cos / sin and inertia read from LDS
+__device__ __forceinline__ void slot_solve_joint(const Slot& z, const SlotState& st, float dt)
+{
+    const uint32_t type = (z.flags >> 24) & 0x7fu;
+    const uint32_t a = z.idx & 0xffffu, b = z.idx >> 16;
+    const float pmax = z.p[0][0], pmin = z.p[0][1], beta = z.p[0][2], target = z.p[1][0];
+    if (type == 0) {
+        // DistanceJoint.solve (DistanceJoint.zig:40-75)
+        float w1 = st.im[a], w2 = st.im[b];
+        v2 v1 = scale2(mk(st.mx[a], st.my[a]), w1);
+        v2 v2v = scale2(mk(st.mx[b], st.my[b]), w2);
+        v2 rv = sub2(v2v, v1);
+        v2 pa = mk(st.px[a], st.py[a]), pb = mk(st.px[b], st.py[b]);
+        v2 n = normalize2(sub2(pb, pa));
+        float dist_err = length2(sub2(pb, pa)) - target;
+        float pc = beta * dist_err / maxn(target, 0.001f);
+        float Jv = -(dot2(rv, n) + pc) / (w1 + w2);
+        float ndv = dot2(n, rv);
+        float den = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] -= dp.x; st.my[a] -= dp.y;
+        st.mx[b] += dp.x; st.my[b] += dp.y;
+    } else if (type == 1) {
+        // OffsetDistanceJoint.solve (OffsetDistanceJoint.zig:44-105)
+        v2 r1 = mk(z.p[0][3], z.p[0][4]), r2 = mk(z.p[0][5], z.p[0][6]);
+        v2 rr1 = rotcs(r1, st.c[a], st.s[a]), rr2 = rotcs(r2, st.c[b], st.s[b]);
+        v2 a1 = add2(rr1, mk(st.px[a], st.py[a]));
+        v2 a2 = add2(rr2, mk(st.px[b], st.py[b]));
+        v2 n = normalize2(sub2(a2, a1));
+        float im1 = st.im[a], im2 = st.im[b], ii1 = st.ii[a], ii2 = st.ii[b];
+        v2 vl1 = scale2(mk(st.mx[a], st.my[a]), im1);
+        float w1 = st.L[a] * ii1;
+        v2 vl2 = scale2(mk(st.mx[b], st.my[b]), im2);
+        float w2 = st.L[b] * ii2;
+        v2 v1 = add2(vl1, mk(-rr1.y * w1, rr1.x * w1));
+        v2 v2v = add2(vl2, mk(-rr2.y * w2, rr2.x * w2));
+        v2 dv = sub2(v2v, v1);
+        float dist_err = length2(sub2(a2, a1)) - target;
+        float pc = beta * dist_err / maxn(target, GRAD_DIV_LIMIT);
+        float num = dot2(n, dv) + pc;
+        float r1xn = cross2(rr1, n), r2xn = cross2(rr2, n);
+        float den = im1 + im2 + r1xn * r1xn * ii1 + r2xn * r2xn * ii2;
+        float Jv = -num / den;
+        float ndv = dot2(n, dv);
+        float den2 = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
+        Jv = clampn(Jv, pmin * den2, pmax * den2);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] -= dp.x; st.my[a] -= dp.y;
+        st.L[a] -= Jv * r1xn;
+        st.mx[b] += dp.x; st.my[b] += dp.y;
+        st.L[b] += Jv * r2xn;
+    } else if (type == 2) {
+        // FixedPositionJoint.solve (FixedPositionJoint.zig:38-72)
+        if (z.flags & RF_S1) return;
+        float w = st.im[a];
+        v2 v = scale2(mk(st.mx[a], st.my[a]), w);
+        v2 delta = sub2(mk(z.p[1][1], z.p[1][2]), mk(st.px[a], st.py[a]));
+        float dist = length2(delta);
+        if (dist < ALLOWED_CONSTRAINT_VALUE) return;
+        v2 n = scale2(delta, 1.0f / dist);
+        float bias = beta * dist;
+        float rv = dot2(v, n);
+        float Jv = -(rv - bias) / w;
+        float den = 1.0f / maxn(absf(rv), ALLOWED_CONSTRAINT_VALUE);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] += dp.x; st.my[a] += dp.y;
+    } else {
+        // MotorJoint.solve (MotorJoint.zig:38-64)
+        if (z.flags & RF_S1) return;
+        float I = st.I[a];
+        float omega = st.L[a] / I;
+        float C = omega - target;
+        if (absf(C) < ALLOWED_CONSTRAINT_VALUE) return;
+        float bias = beta * C;
+        float tq = st.tq[a];
+        float Jv = tq - bias;
+        float rel = tq / I;
+        float den = 1.0f / maxn(absf(rel), ALLOWED_CONSTRAINT_VALUE);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        st.L[a] += Jv * dt;
+    }
+}
+
+template <int NSLOT, int MAXT, int WAVES>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
+void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
+{
+    extern __shared__ uint32_t smem[];
+    __shared__ uint32_t s_abort;
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t w = blockIdx.x;
+    if (S.w_status[w] != 0) return;
+    const uint32_t ibase = S.w_item_base[w];
+    const uint32_t nitems = S.w_item_base[w + 1] - ibase;
+    const WorldDev& D = S.worlds[w];
+    const int n = D.nbodies;
+    if (nitems > (uint32_t)NSLOT * MAXT || n >= 0xffff || slot_lds_bytes(n) > SLOT_LDS_MAX) return;
+    if (threadIdx.x == 0) s_abort = 0u;
+    const uint32_t base = D.body_base;
+    const uint32_t nj = D.njoints;
+    const uint32_t mbeg = S.w_man_begin[w];
+    const float fsub = (float)sub_steps;
+    const float sub_dt = dt / fsub;
+
+    SlotState st;
+    st.mx = (float*)smem;
+    st.my = st.mx + n;
+    st.L = st.my + n;
+    st.im = st.L + n;
+    st.ii = st.im + n;
+    st.cnt = (uint32_t*)(st.ii + n);
+    st.deg = st.cnt + n;
+    st.px = (float*)(st.deg + n);
+    st.py = st.px + n;
+    st.c = st.py + n;
+    st.s = st.c + n;
+    st.I = st.s + n;
+    st.tq = st.I + n;
+    for (int i = threadIdx.x; i < n; i += MAXT) {
+        bool stat = (D.flags[i] & BF_STATIC) != 0;
+        st.mx[i] = D.mx[i]; st.my[i] = D.my[i]; st.L[i] = D.L[i];
+        st.im[i] = stat ? 0.0f : 1.0f / D.mass[i];
+        st.ii[i] = stat ? 0.0f : 1.0f / D.inertia[i];
+        st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
+        st.I[i] = D.inertia[i];
+    }
+    __syncthreads();
+    Slot sl[NSLOT];
+    uint32_t nslot = 0;
+#pragma unroll
+    for (int z = 0; z < NSLOT; ++z) {
+        sl[z].flags = 0u;
+        uint32_t pos = threadIdx.x + (uint32_t)z * MAXT;
+        if (pos < nitems) {
+            Rec r;
+            rec_build(r, S, D, st, pos, ibase, nj, mbeg, base);
+            slot_from_rec(sl[z], r);
+            if (r.flags & RF_JOINT) slot_joint(sl[z], D.joints[r.idx], D);
+            nslot = (uint32_t)z + 1u;
+        }
+    }
+
+    for (uint32_t s = 0; s < sub_steps; ++s) {
+        // forces (lib.zig:200-205), AABB refresh + velocity integration (lib.zig:207-216)
+        for (int i = threadIdx.x; i < n; i += MAXT) {
+            uint32_t f = D.flags[i];
+            bool stat = (f & BF_STATIC) != 0;
+            float fx = D.fx[i], fy = D.fy[i];
+            float px = D.px[i], py = D.py[i], a = D.ang[i];
+            for (int g = 0; g < D.ngens; ++g) {
+                const Gen& G = D.gens[g];
+                if (stat) continue;
+                if (G.type == 0) {
+                    fx += G.gx * D.mass[i];
+                    fy += G.gy * D.mass[i];
+                } else if (G.slot == i) {
+                    float inv_m = 1.0f / D.mass[i];
+                    v2 v = scale2(mk(st.mx[i], st.my[i]), inv_m);
+                    v2 d = sub2(mk(px, py), mk(G.ax, G.ay));
+                    fx += (-G.k) * d.x - G.c * v.x;
+                    fy += (-G.k) * d.y - G.c * v.y;
+                }
+            }
+            // cos / sin of the angle: offset joints read them in the solve
+            float c = 0.0f, sn = 0.0f;
+            if (nj || (s + 1 == sub_steps && (f & BF_RECT))) { c = cosf_r(a); sn = sinf_r(a); }
+            if (s + 1 == sub_steps) {
+                float hw, hh;
+                if (f & BF_RECT) {
+                    float x = D.hw[i], y = D.hh[i];
+                    v2 lv[4] = {mk(-x, -y), mk(-x, y), mk(x, y), mk(x, -y)};
+                    hw = 0.f; hh = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        v2 r = rotcs(lv[k], c, sn);
+                        if (r.x > hw) hw = r.x;
+                        if (r.y > hh) hh = r.y;
+                    }
+                } else {
+                    hw = D.rad[i]; hh = D.rad[i];
+                }
+                D.ax[i] = px; D.ay[i] = py; D.ahw[i] = hw; D.ahh[i] = hh;
+            }
+            if (!stat) {
+                st.mx[i] += fx * sub_dt;
+                st.my[i] += fy * sub_dt;
+                st.L[i] += D.tq[i] * sub_dt;
+            }
+            D.fx[i] = fx; D.fy[i] = fy;
+            st.px[i] = px; st.py[i] = py; st.c[i] = c; st.s[i] = sn;
+            st.tq[i] = D.tq[i];
+            st.cnt[i] = 0u;
+        }
+        __syncthreads();
+        // updateTGSDepth (collision.zig:73-100)
+#pragma unroll
+        for (int z = 0; z < NSLOT; ++z) {
+            uint32_t fl = sl[z].flags;
+            if (!fl || (fl & RF_JOINT)) continue;
+            if (fl & RF_TGS) {
+                Rec r;
+                rec_from_slot(r, sl[z]);
+                rec_tgs(r, S, D, st, true);
+                slot_from_rec(sl[z], r);
+            } else if (s + 1 == sub_steps) {
+                S.m_prev1[sl[z].idx] = D.ang[sl[z].pair & 0xffffu];
+                S.m_prev2[sl[z].idx] = D.ang[sl[z].pair >> 16];
+            }
+        }
+        // K iterations as an ordered dataflow over register-resident items
+        {
+            uint32_t k = nslot ? 0u : iters, cz = 0, spins = 0;
+            bool have = false;
+            Slot cur;
+            int32_t r0 = -1, r1 = -1;
+            uint32_t t0 = 0, t1 = 0;
+            while (k < iters) {
+                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                if (!have) {
+#pragma unroll
+                    for (int z = 0; z < NSLOT; ++z)
+                        if ((uint32_t)z == cz) cur = sl[z];
+                    have = true;
+                    uint32_t a = cur.pair & 0xffffu, b = cur.pair >> 16;
+                    if (cur.flags & RF_JOINT) {
+                        r0 = a == 0xffffu ? -1 : (int32_t)a;
+                        r1 = b == 0xffffu ? -1 : (int32_t)b;
+                    } else {
+                        r0 = (cur.flags & RF_W1) ? (int32_t)a : -1;
+                        r1 = (cur.flags & RF_W2) ? (int32_t)b : -1;
+                    }
+                    if (r0 >= 0) t0 = k * st.deg[r0] + (cur.ranks & 0xffffu);
+                    if (r1 >= 0) t1 = k * st.deg[r1] + (cur.ranks >> 16);
+                }
+                bool ready = true;
+                if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
+                if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
+                if (ready) {
+                    if (cur.flags & RF_JOINT) {
+                        slot_solve_joint(cur, st, sub_dt);
+                    } else {
+                        Rec r;
+                        rec_from_slot(r, cur);
+                        rec_impulses(r, st, sub_dt);
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) cur.acc[f] = r.acc[f];
+#pragma unroll
+                        for (int z = 0; z < NSLOT; ++z)
+                            if ((uint32_t)z == cz)
+#pragma unroll
+                                for (int f = 0; f < 4; ++f) sl[z].acc[f] = cur.acc[f];
+                    }
+                    if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
+                    if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                    have = false;
+                    spins = 0;
+                    if (++cz == nslot) { cz = 0; ++k; }
+                } else {
+                    if (++spins > SPIN_LIMIT) {
+                        __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        S.w_status[w] = 5;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __syncthreads();
+        // position integration (lib.zig:238-249)
+        for (int i = threadIdx.x; i < n; i += MAXT) {
+            if (D.flags[i] & BF_STATIC) continue;
+            float qd = sub_dt / D.mass[i];
+            D.px[i] += st.mx[i] * qd;
+            D.py[i] += st.my[i] * qd;
+            D.ang[i] += st.L[i] * sub_dt / D.inertia[i];
+            D.fx[i] = 0.0f; D.fy[i] = 0.0f;
+            D.tq[i] = 0.0f;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int z = 0; z < NSLOT; ++z) {
+        if (!sl[z].flags || (sl[z].flags & RF_JOINT)) continue;
+        Rec r;
+        rec_from_slot(r, sl[z]);
+        rec_writeback(r, S);
+    }
+    for (int i = threadIdx.x; i < n; i += MAXT) {
+        D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
     }
 }
 
@@ -1430,19 +1770,25 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs, unsigned cap = 4096)
 
 size_t solve_lds_bytes(int nbodies) { return (size_t)nbodies * 7u * 4u; }
 
-typedef void (*SolveFn)(StepDev, float, uint32_t, uint32_t);
+typedef void (*SlotFn)(StepDev, float, uint32_t, uint32_t);
+typedef void (*RecFn)(StepDev, float, uint32_t, uint32_t, uint32_t);
 struct SolveVariant {
-    SolveFn lds_fn;
-    unsigned threads;
+    SlotFn slot_fn;      // register-slot kernel (nullptr: none)
+    unsigned slot_threads;
     int slots;
+    RecFn rec_fn;        // record kernel for the worlds the slot kernel leaves
+    unsigned threads;
 };
-// solve kernel variants: register slots per thread x workgroup size
+// solve variants: the register-slot kernel's slots x workgroup size, and the
+// record kernel's workgroup size for worlds with more items than slots
 static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
-    {k_solve<true, 0, 1024>, 1024, 0},
-    {k_solve<true, 4, 256>, 256, 4},
-    {k_solve<true, 2, 512>, 512, 2},
-    {k_solve<true, 2, 256>, 256, 2},
-    {k_solve<true, 0, 256>, 256, 0},
+    {nullptr, 0, 0, k_solve<true, 1024>, 1024},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256},
+    {nullptr, 0, 0, k_solve<true, 256, 2>, 256},
 };
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
@@ -1492,11 +1838,18 @@ void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_
         break;
     default: {
         const SolveVariant& v = kSolveVariants[solve_variant < SOLVE_VARIANTS ? solve_variant : 0];
+        size_t sm = solve_lds_bytes((int)max_bodies);
+        uint32_t cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
+        bool slot_ok = v.slot_fn && slot_lds_bytes((int)max_bodies) <= SLOT_LDS_MAX;
+        if (!slot_ok) cap = 0u;
+        if (lds && slot_ok)
+            hipLaunchKernelGGL(v.slot_fn, dim3(S.nworlds), dim3(v.slot_threads), slot_lds_bytes((int)max_bodies), st,
+                               S, dt, sub_steps, iters);
         if (lds) {
-            size_t sm = solve_lds_bytes((int)max_bodies);
-            hipLaunchKernelGGL(v.lds_fn, dim3(S.nworlds), dim3(v.threads), sm, st, S, dt, sub_steps, iters);
+            hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), sm, st, S, dt, sub_steps, iters, cap);
         } else {
-            hipLaunchKernelGGL((k_solve<false, 0, 1024>), dim3(S.nworlds), dim3(1024), 0, st, S, dt, sub_steps, iters);
+            hipLaunchKernelGGL((k_solve<false, 1024>), dim3(S.nworlds), dim3(1024), 0, st, S, dt, sub_steps, iters,
+                               0u);
         }
         hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
         break;
@@ -1506,9 +1859,13 @@ void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_
 
 void set_solve_smem_limit()
 {
-    for (int v = 0; v < SOLVE_VARIANTS; ++v)
-        (void)hipFuncSetAttribute((const void*)kSolveVariants[v].lds_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+    for (int v = 0; v < SOLVE_VARIANTS; ++v) {
+        if (kSolveVariants[v].slot_fn)
+            (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)kSolveVariants[v].rec_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
+    }
 }
 
 }  // namespace r2d

@@ -111,7 +111,8 @@ R2D_HD uint64_t mulhi64(uint64_t a, uint64_t b)
 }
 
 // Huge |x| (>= 2^28*pi/2): exact reduction with a 128-bit window of 2/pi.
-R2D_HD int rem_pio2_huge(float x, double* y)
+// Out of line: it is rare and large.
+static __host__ __device__ __noinline__ int rem_pio2_huge(float x, double* y)
 {
     // bits of 2/pi after the binary point, 32 per word
     const uint32_t tp[12] = {0xa2f9836eu, 0x4e441529u, 0xfc2757d1u, 0xf534ddc0u, 0xdb629599u, 0x3c439041u,

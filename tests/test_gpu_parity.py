@@ -90,15 +90,15 @@ def test_batched_equals_single(gpu_ctx, oracle):
         w.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", list(range(7)))
 def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
-    """Every launch configuration of the solve kernel (register slots x
-    workgroup size, including the fallback for worlds with more items than
-    slots) gives the oracle's bits."""
+    """Every launch configuration of the solve kernel (workgroup size per
+    world) gives the oracle's bits."""
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld, apply_scene
     specs = [scenes.batched_world(21, n_min=40, n_max=60), scenes.batched_world(22),
-             scenes.pyramid(rows=16), scenes.mixed(900, seed=3, width=20.0, height=30.0, ground_w=30.0)]
+             scenes.pyramid(rows=16), scenes.mixed(900, seed=3, width=20.0, height=30.0, ground_w=30.0),
+             scenes.car_demo()]
     gw = [GpuWorld(gpu_ctx) for _ in specs]
     ow = [oracle.OracleWorld("f32") for _ in specs]
     for sc, g, o in zip(specs, gw, ow):
