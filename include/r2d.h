@@ -41,7 +41,9 @@ typedef enum r2d_status {
     R2D_E_DEVICE = 5,        /* HIP runtime failure                                      */
     R2D_E_NO_DEVICE = 6,     /* no usable GPU                                            */
     R2D_E_RANGE = 7,         /* index out of range (r2d_body_id_at), too many cells     */
-    R2D_E_CAPACITY = 8       /* request larger than a fixed limit                        */
+    R2D_E_CAPACITY = 8,      /* request larger than a fixed limit                        */
+    R2D_E_BUSY = 9           /* the world is held by a server (r2d_serve_submit) until its
+                                request completes; the context already has a server      */
 } r2d_status;
 
 typedef struct r2d_ctx r2d_ctx;
@@ -206,7 +208,7 @@ int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
 int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
 /* Launch configuration of the solve kernel: -1 (default) picks by world
- * size; 0..6 select the register-slot kernel's slots x workgroup size and the
+ * size; 0..7 select the register-slot kernel's slots x workgroup size and the
  * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner
  * (resolve2d_amd/autotune.py) picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);
@@ -224,6 +226,89 @@ int r2d_last_error(char* buf, size_t n);
  * src/core/nmath.zig:129-136), 2 sqrt, 3 reciprocal, 4 division of input
  * pairs (in[2i] / in[2i+1]; `in` holds 2n floats). */
 int r2d_debug_eval(r2d_ctx* ctx, int op, const float* in, float* out, size_t n);
+
+/* ------------------------------------------------------- bulk ingestion */
+/* n bodies in one call, in order (ids as n calls of r2d_make_disc /
+ * r2d_make_rect would assign them, EntityFactory.appendBody lib.zig:66-71):
+ * kind[i] 0 = disc (dim_a = radius), 1 = rectangle (dim_a x dim_b).
+ * ids_out may be NULL. */
+int r2d_make_bodies(r2d_world* w, size_t n, const r2d_body_options* bo, const int32_t* kind, const float* dim_a,
+                    const float* dim_b, uint32_t* ids_out);
+
+/* ------------------------------------------------------------- serving */
+/* No counterpart in the reference (it steps one world per call).  A server
+ * owns one context's batched step: requests -- a built world and a number of
+ * steps -- are admitted in arrival order while the sum of their footprints
+ * (r2d_world_footprint) fits hbm_budget_bytes and fewer than max_worlds
+ * run; each r2d_serve_tick collects the batch step launched by the previous
+ * tick, retires finished requests, admits waiting ones and launches one step
+ * of every running world as one batch (continuous batching).  A world's
+ * state stays on the device from admission to completion; while a request
+ * holds it, edits of the world return R2D_E_BUSY (reads are allowed).  Each
+ * world's result is bit-identical to stepping it alone. */
+typedef struct r2d_server r2d_server;
+
+enum { R2D_REQ_QUEUED = 0, R2D_REQ_RUNNING = 1, R2D_REQ_DONE = 2, R2D_REQ_FAILED = 3 };
+
+typedef struct {
+    size_t hbm_budget_bytes;    /* admission budget over resident footprints */
+    uint32_t max_worlds;        /* batch slots */
+    float dt;                   /* Solver.process arguments of every step */
+    uint32_t sub_steps;
+    uint32_t collision_iters;
+} r2d_serve_options;
+
+typedef struct {
+    uint32_t admitted;          /* requests admitted by this tick */
+    uint32_t completed;         /* requests finished by the step it collected */
+    uint32_t queued;            /* waiting for admission */
+    uint32_t running;           /* admitted and not finished */
+    uint32_t launched_worlds;   /* worlds in the batch step this tick launched */
+    uint64_t launched_bodies;
+    size_t resident_bytes;
+    double step_gpu_ms;         /* device time of the collected batch step */
+    double step_wall_ms;        /* its launch-to-collection wall time */
+} r2d_serve_tick_info;
+
+typedef struct {
+    int32_t state;              /* R2D_REQ_* */
+    int32_t status;             /* r2d_status of a failed request */
+    uint32_t steps_done;
+    uint32_t steps_requested;
+    size_t footprint_bytes;
+    double wait_ms;             /* submit -> admission (so far, while queued) */
+    double total_ms;            /* submit -> completion (so far, while not done) */
+    double max_step_ms;         /* slowest device step it took part in */
+} r2d_request_status;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, queued, running;
+    uint64_t batch_steps;       /* batched launches collected */
+    uint64_t scene_steps;       /* world-steps over them */
+    uint64_t body_steps;
+    double gpu_ms;              /* device time of all batch steps */
+    double wall_ms;             /* launch-to-collection wall time of all batch steps */
+    double p50_step_ms, p99_step_ms;    /* device step time seen by each world-step */
+    double p50_batch_ms, p99_batch_ms;  /* device time per batch step */
+    size_t resident_bytes, peak_resident_bytes;
+} r2d_serve_summary;
+
+/* Bytes a world is charged at admission: its device state plus its share of
+ * the step scratch at the library's initial per-body estimates. */
+int r2d_world_footprint(r2d_world* w, size_t* bytes);
+int r2d_serve_create(r2d_ctx* ctx, const r2d_serve_options* opt, r2d_server** out);
+/* Waits for the launched step; worlds of unfinished requests are released. */
+int r2d_serve_destroy(r2d_server* sv);
+/* Queues a request; R2D_E_CAPACITY if the world can never fit the budget,
+ * R2D_E_INVALID_ID if a constraint names a removed body. */
+int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, uint64_t* ticket);
+/* Does not wait for the step it launches: host work between ticks overlaps
+ * the device.  info may be NULL. */
+int r2d_serve_tick(r2d_server* sv, r2d_serve_tick_info* info);
+/* Ticks until every request has finished. */
+int r2d_serve_drain(r2d_server* sv);
+int r2d_serve_poll(r2d_server* sv, uint64_t ticket, r2d_request_status* out);
+int r2d_serve_stats(r2d_server* sv, r2d_serve_summary* out);
 
 #ifdef __cplusplus
 }

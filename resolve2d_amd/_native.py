@@ -14,8 +14,10 @@ R2D_OK = 0
 STATUS_NAMES = {
     0: "R2D_OK", 1: "R2D_E_INVALID_ARG", 2: "R2D_E_INVALID_ID", 3: "R2D_E_OOM",
     4: "R2D_E_NONFINITE", 5: "R2D_E_DEVICE", 6: "R2D_E_NO_DEVICE", 7: "R2D_E_RANGE",
-    8: "R2D_E_CAPACITY",
+    8: "R2D_E_CAPACITY", 9: "R2D_E_BUSY",
 }
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class R2DError(RuntimeError):
@@ -65,6 +67,32 @@ class ManifoldState(ctypes.Structure):
                 ("friction", ctypes.c_float), ("points", ManifoldPoint * 2)]
 
 
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_worlds", ctypes.c_uint32), ("dt", ctypes.c_float),
+                ("sub_steps", ctypes.c_uint32), ("collision_iters", ctypes.c_uint32)]
+
+
+class ServeTickInfo(ctypes.Structure):
+    _fields_ = [("admitted", ctypes.c_uint32), ("completed", ctypes.c_uint32), ("queued", ctypes.c_uint32),
+                ("running", ctypes.c_uint32), ("launched_worlds", ctypes.c_uint32),
+                ("launched_bodies", ctypes.c_uint64), ("resident_bytes", ctypes.c_size_t),
+                ("step_gpu_ms", ctypes.c_double), ("step_wall_ms", ctypes.c_double)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("steps_done", ctypes.c_uint32),
+                ("steps_requested", ctypes.c_uint32), ("footprint_bytes", ctypes.c_size_t),
+                ("wait_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("max_step_ms", ctypes.c_double)]
+
+
+class ServeSummary(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "rejected", "queued", "running",
+                                               "batch_steps", "scene_steps", "body_steps")] + \
+               [(n, ctypes.c_double) for n in ("gpu_ms", "wall_ms", "p50_step_ms", "p99_step_ms", "p50_batch_ms",
+                                               "p99_batch_ms")] + \
+               [("resident_bytes", ctypes.c_size_t), ("peak_resident_bytes", ctypes.c_size_t)]
+
+
 P = ctypes.c_void_p
 U32P = ctypes.POINTER(ctypes.c_uint32)
 SZP = ctypes.POINTER(ctypes.c_size_t)
@@ -109,6 +137,16 @@ SIGNATURES = {
     "r2d_debug_eval": [P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), SZ],
     "r2d_last_error": [ctypes.c_char_p, SZ],
     "r2d_ctx_set_solve_variant": [P, I32],
+    "r2d_make_bodies": [P, SZ, ctypes.POINTER(BodyOptions), ctypes.POINTER(ctypes.c_int32),
+                        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), U32P],
+    "r2d_world_footprint": [P, SZP],
+    "r2d_serve_create": [P, ctypes.POINTER(ServeOptions), ctypes.POINTER(P)],
+    "r2d_serve_destroy": [P],
+    "r2d_serve_submit": [P, P, U32, ctypes.POINTER(ctypes.c_uint64)],
+    "r2d_serve_tick": [P, ctypes.POINTER(ServeTickInfo)],
+    "r2d_serve_drain": [P],
+    "r2d_serve_poll": [P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)],
+    "r2d_serve_stats": [P, ctypes.POINTER(ServeSummary)],
 }
 
 _lib = None

@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <vector>
 
@@ -93,6 +95,7 @@ struct r2d_world {
     bool host_valid = true;   // host mirror is current
     bool dev_valid = false;   // device body block is current
     bool topo_dirty = true;   // joints / generators / exclusions need upload
+    bool in_service = false;  // held by a server until its request completes
     // manifolds of the last step this world took part in
     uint64_t epoch = 0;
     uint32_t batch_index = 0;
@@ -105,6 +108,10 @@ struct r2d_world {
 struct r2d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[2] = {nullptr, nullptr};      // side streams of the solve phase
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;   // brackets of the last launched batch
+    r2d_server* server = nullptr;
     // step scratch
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -129,8 +136,11 @@ struct r2d_ctx {
     hipGraphExec_t graph_exec = nullptr;          // whole step
     hipGraphExec_t phase_exec[PH_N] = {};         // one per phase (timed steps)
     std::vector<uint8_t> graph_key;
+    uint64_t graph_uses = 0;                      // launches with the current key
     std::vector<WorldDev> uploaded_descs;
     int solve_variant = -1;                       // -1: choose by world size
+
+    Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}}; }
 };
 
 namespace {
@@ -138,6 +148,7 @@ namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 void destroy_graphs(r2d_ctx* c);
+int server_collect(r2d_server* sv);
 
 int ensure_world_device(r2d_world* w)
 {
@@ -188,6 +199,7 @@ int download_bodies(r2d_world* w)
 // host edits need the current state; afterwards the device copy is stale
 int begin_edit(r2d_world* w)
 {
+    if (w->in_service) return R2D_E_BUSY;
     int rc = download_bodies(w);
     if (rc) return rc;
     w->dev_valid = false;
@@ -410,10 +422,25 @@ bool capture_graph(r2d_ctx* c, hipGraphExec_t* out, Fn&& enqueue)
     return ok;
 }
 
-int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
-               size_t steps)
+// One batched call in flight: prepared (uploads, scratch, plan), launched
+// (the steps enqueued plus asynchronous copies of the counters), then waited
+// for (overflowed scratch is grown and the uncommitted steps relaunched).
+struct Batch {
+    std::vector<r2d_world*> worlds;
+    std::vector<WorldDev> descs;
+    uint64_t N = 0, B = 0, J = 0;
+    SolvePlan plan;
+    float dt = 0.f;
+    size_t sub = 0, iters = 0;
+    size_t steps = 0, done = 0;
+    int attempts = 0;
+    int first_err = R2D_OK;
+    std::vector<int32_t> world_status;   // device status per world of the last wait
+};
+
+int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
+                  size_t steps, Batch& b)
 {
-    if (nw == 0 || steps == 0) return R2D_OK;
     // the reference raises InvalidRigidBodyId from the first constraint solve
     bool solves = sub_steps > 0 && iters > 0;
     for (size_t i = 0; i < nw; ++i) {
@@ -423,110 +450,129 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
             for (const HJoint& h : w->joints)
                 if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
     }
-    uint64_t N = 0, B = 0, J = 0;
-    uint32_t maxn = 0;
-    std::vector<WorldDev> descs(nw);
+    b = Batch();
+    b.worlds.assign(worlds, worlds + nw);
+    b.descs.resize(nw);
+    b.dt = dt; b.sub = sub_steps; b.iters = iters; b.steps = steps;
+    std::vector<uint32_t> sizes(nw);
     for (size_t i = 0; i < nw; ++i) {
         r2d_world* w = worlds[i];
         int rc;
         if (!w->dev_valid && (rc = upload_bodies(w))) return rc;
         if ((rc = upload_topology(w))) return rc;
         WorldDev d = make_desc(w);
-        d.body_base = (uint32_t)N;
-        d.bucket_base = (uint32_t)B;
-        d.joint_base = (uint32_t)J;
-        descs[i] = d;
-        N += (uint64_t)w->n;
-        B += d.table;
-        J += w->joints.size();
-        maxn = std::max<uint32_t>(maxn, (uint32_t)w->n);
+        d.body_base = (uint32_t)b.N;
+        d.bucket_base = (uint32_t)b.B;
+        d.joint_base = (uint32_t)b.J;
+        b.descs[i] = d;
+        b.N += (uint64_t)w->n;
+        b.B += d.table;
+        b.J += w->joints.size();
+        sizes[i] = (uint32_t)w->n;
     }
-    if (N >= 0x7fffffffull || B >= 0x7fffffffull) return R2D_E_CAPACITY;
-    uint32_t est_e = std::max<uint32_t>(c->cap_e, (uint32_t)(4 * N + 1024));
-    uint32_t est_r = std::max<uint32_t>(c->cap_r, (uint32_t)(8 * N + 1024));
-    uint32_t est_m = std::max<uint32_t>(c->cap_m, (uint32_t)(4 * N + 1024));
-    int rc = ensure_scratch(c, (uint32_t)N, (uint32_t)B, (uint32_t)nw, (uint32_t)J, est_e, est_r, est_m,
-                            (uint32_t)J + est_m);
+    if (b.N >= 0x7fffffffull || b.B >= 0x7fffffffull) return R2D_E_CAPACITY;
+    uint32_t est_e = std::max<uint32_t>(c->cap_e, (uint32_t)(4 * b.N + 1024));
+    uint32_t est_r = std::max<uint32_t>(c->cap_r, (uint32_t)(8 * b.N + 1024));
+    uint32_t est_m = std::max<uint32_t>(c->cap_m, (uint32_t)(4 * b.N + 1024));
+    int rc = ensure_scratch(c, (uint32_t)b.N, (uint32_t)b.B, (uint32_t)nw, (uint32_t)b.J, est_e, est_r, est_m,
+                            (uint32_t)b.J + est_m);
     if (rc) return rc;
-    if (c->uploaded_descs.size() != descs.size() ||
-        std::memcmp(c->uploaded_descs.data(), descs.data(), descs.size() * sizeof(WorldDev)) != 0) {
-        HIPCHK(hipMemcpyAsync(c->dworlds, descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
-        c->uploaded_descs = descs;
-    }
-    bool lds = solve_lds_bytes((int)maxn) <= 160 * 1024 - 1024;
-    // solve variant: context override (autotuner), environment, then size heuristic
-    int variant = maxn <= 600 ? 1 : 0;
+    // solve variant: context override (autotuner), environment, then the
+    // default: the register-slot kernel for small worlds beside a record
+    // kernel sized for the largest world of the batch
+    uint32_t maxn = sizes.empty() ? 0u : *std::max_element(sizes.begin(), sizes.end());
+    int variant = maxn <= 600 ? 1 : 7;
     if (c->solve_variant >= 0) variant = c->solve_variant;
     if (const char* e = std::getenv("R2D_SOLVE_VARIANT")) {
         int v = std::atoi(e);
         if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
-    const unsigned threads = solve_variant_threads(variant) + 64u * (unsigned)solve_variant_slots(variant);
-    int first_err = R2D_OK;
-    size_t done = 0;
-    for (int attempt = 0; done < steps; ++attempt) {
-        if (attempt > 16) return R2D_E_CAPACITY;
-        StepDev S = c->S;
-        S.nb = (uint32_t)N; S.nworlds = (uint32_t)nw; S.nbuckets = (uint32_t)B; S.njoints = (uint32_t)J;
-        S.worlds = c->dworlds;
-        HIPCHK(hipMemsetAsync(S.counters, 0, 64 * 4, c->stream));
-        size_t todo = steps - done;
-        KernelTimer* tm = c->profiling ? &c->timer : nullptr;
-        auto eager_phase = [&](int ph) {
-            launch_phase(S, c->stream, ph, dt, (uint32_t)sub_steps, (uint32_t)iters, maxn, lds, variant);
-        };
-        bool graph_ok = false;
-        if (c->use_graph) {
-            // key: everything the captured launches bake in
-            std::vector<uint8_t> key;
-            auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + n); };
-            put(&S, sizeof S);
-            put(descs.data(), descs.size() * sizeof(WorldDev));
-            float kdt = dt; uint64_t ks = sub_steps, ki = iters; uint32_t kt = threads; int kl = lds, kv = variant;
-            put(&kdt, 4); put(&ks, 8); put(&ki, 8); put(&kt, 4); put(&kl, 4); put(&maxn, 4); put(&kv, 4);
-            if (key != c->graph_key) {
-                destroy_graphs(c);
-                bool ok = capture_graph(c, &c->graph_exec, [&]() { for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph); });
-                for (int ph = 0; ok && ph < PH_N; ++ph)
-                    ok = capture_graph(c, &c->phase_exec[ph], [&]() { eager_phase(ph); });
-                if (ok) c->graph_key.swap(key);
-                else { destroy_graphs(c); c->use_graph = false; }   // capture unsupported: launch eagerly
-            }
-            graph_ok = c->graph_exec != nullptr;
+    b.plan = plan_solve(variant, sizes.data(), nw);
+    return R2D_OK;
+}
+
+int batch_launch(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    if (c->uploaded_descs.size() != nw ||
+        std::memcmp(c->uploaded_descs.data(), b.descs.data(), nw * sizeof(WorldDev)) != 0) {
+        HIPCHK(hipMemcpyAsync(c->dworlds, b.descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
+        c->uploaded_descs = b.descs;
+    }
+    StepDev S = c->S;
+    S.nb = (uint32_t)b.N; S.nworlds = (uint32_t)nw; S.nbuckets = (uint32_t)b.B; S.njoints = (uint32_t)b.J;
+    S.worlds = c->dworlds;
+    const Streams ss = c->streams();
+    const uint32_t sub = (uint32_t)b.sub, iters = (uint32_t)b.iters;
+    auto eager_phase = [&](int ph) { launch_phase(S, ss, ph, b.dt, sub, iters, b.plan); };
+    KernelTimer* tm = c->profiling ? &c->timer : nullptr;
+    hipGraphExec_t* want = nullptr;   // graphs this launch replays
+    if (c->use_graph) {
+        // key: everything the captured launches bake in (the world descriptors
+        // are device data, uploaded above, and are not part of it)
+        std::vector<uint8_t> key;
+        auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + n); };
+        put(&S, sizeof S);
+        put(&b.plan, sizeof b.plan);
+        float kdt = b.dt; uint32_t ks = sub, ki = iters;
+        put(&kdt, 4); put(&ks, 4); put(&ki, 4);
+        if (key != c->graph_key) { destroy_graphs(c); c->graph_key.swap(key); c->graph_uses = 0; }
+        c->graph_uses += 1;
+        // capturing costs more than launching once: capture a shape that repeats
+        // (a second launch with the same key, or several steps in this one)
+        const bool repeat = c->graph_uses >= 2 || b.steps - b.done >= 2;
+        bool ok = repeat;
+        if (ok && !tm && !c->graph_exec)
+            ok = capture_graph(c, &c->graph_exec, [&]() { for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph); });
+        for (int ph = 0; ok && tm && ph < PH_N; ++ph)
+            if (!c->phase_exec[ph]) ok = capture_graph(c, &c->phase_exec[ph], [&]() { eager_phase(ph); });
+        if (repeat && !ok) { destroy_graphs(c); c->use_graph = false; }   // capture unsupported: launch eagerly
+        else if (ok) want = tm ? c->phase_exec : &c->graph_exec;
+    }
+    HIPCHK(hipMemsetAsync(S.counters, 0, 64 * 4, c->stream));
+    HIPCHK(hipEventRecord(c->ev_begin, c->stream));
+    const size_t todo = b.steps - b.done;
+    for (size_t s = 0; s < todo; ++s) {
+        if (!tm) {
+            if (want) HIPCHK(hipGraphLaunch(*want, c->stream));
+            else for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph);
+            continue;
         }
-        for (size_t s = 0; s < todo; ++s) {
-            if (!tm) {
-                if (graph_ok) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
-                else for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph);
-            } else {
-                // timed step: events on the stream between the phases
-                for (int ph = 0; ph < PH_N; ++ph) {
-                    tm->mark(ph, c->stream);
-                    if (graph_ok) HIPCHK(hipGraphLaunch(c->phase_exec[ph], c->stream));
-                    else eager_phase(ph);
-                }
-                tm->mark(KT_END, c->stream);
-            }
-            if (tm) {
-                HIPCHK(hipStreamSynchronize(c->stream));
-                for (int p = 0; p < 4; ++p) {
-                    float ms = 0.f;
-                    HIPCHK(hipEventElapsedTime(&ms, c->timer.ev[p], c->timer.ev[p + 1]));
-                    c->phase_ms[p] += ms;
-                }
-                c->phase_steps += 1;
-            }
+        // timed step: events on the stream between the phases
+        for (int ph = 0; ph < PH_N; ++ph) {
+            tm->mark(ph, c->stream);
+            if (want) HIPCHK(hipGraphLaunch(want[ph], c->stream));
+            else eager_phase(ph);
         }
-        HIPCHK(hipMemcpyAsync(c->hcounters, S.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(c->hstatus, S.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
+        tm->mark(KT_END, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < 4; ++p) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->timer.ev[p], c->timer.ev[p + 1]));
+            c->phase_ms[p] += ms;
+        }
+        c->phase_steps += 1;
+    }
+    HIPCHK(hipMemcpyAsync(c->hcounters, S.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hstatus, S.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->ev_end, c->stream));
+    b.attempts += 1;
+    return R2D_OK;
+}
+
+int batch_wait(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    for (;;) {
+        HIPCHK(hipEventSynchronize(c->ev_end));
         uint32_t committed = c->hcounters[C_STEPS];
-        done += committed;
+        b.done += committed;
+        b.world_status.assign(c->hstatus, c->hstatus + nw);
         for (size_t i = 0; i < nw; ++i) {
             if (c->hstatus[i] == 5)
                 std::snprintf(g_last_error, sizeof g_last_error, "solver schedule stalled in world %zu", i);
-            if (c->hstatus[i] != 0 && first_err == R2D_OK)
-                first_err = c->hstatus[i] == 4 ? R2D_E_NONFINITE : c->hstatus[i] == 5 ? R2D_E_DEVICE : R2D_E_RANGE;
+            if (c->hstatus[i] != 0 && b.first_err == R2D_OK)
+                b.first_err = c->hstatus[i] == 4 ? R2D_E_NONFINITE : c->hstatus[i] == 5 ? R2D_E_DEVICE : R2D_E_RANGE;
         }
         for (int k = 0; k < 4; ++k) c->last_counts[k] = c->hcounters[k];
         if (c->hcounters[C_OVERFLOW]) {
@@ -539,19 +585,49 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
             if (c->hcounters[C_OVERFLOW] & 2u) r = c->hcounters[C_RAW] * 2;
             if (c->hcounters[C_OVERFLOW] & 4u) m = c->hcounters[C_MANIFOLDS] * 2;
             if (c->hcounters[C_OVERFLOW] & 8u) it = c->hcounters[C_ITEMS] * 2;
-            rc = ensure_scratch(c, (uint32_t)N, (uint32_t)B, (uint32_t)nw, (uint32_t)J, e + 1, r + 1, m + 1, it + 1);
+            int rc = ensure_scratch(c, (uint32_t)b.N, (uint32_t)b.B, (uint32_t)nw, (uint32_t)b.J, e + 1, r + 1,
+                                    m + 1, it + 1);
             if (rc) return rc;
-            continue;
         }
+        if (b.done >= b.steps) break;
+        if (b.attempts > 16) return R2D_E_CAPACITY;
+        int rc = batch_launch(c, b);
+        if (rc) return rc;
     }
     c->epoch += 1;
-    c->last_batch.assign(worlds, worlds + nw);
+    c->last_batch = b.worlds;
     for (size_t i = 0; i < nw; ++i) {
-        worlds[i]->host_valid = false;
-        worlds[i]->epoch = c->epoch;
-        worlds[i]->batch_index = (uint32_t)i;
+        b.worlds[i]->host_valid = false;
+        b.worlds[i]->epoch = c->epoch;
+        b.worlds[i]->batch_index = (uint32_t)i;
     }
-    return first_err;
+    return b.first_err;
+}
+
+// device time of the last launched batch (its final launch when it was relaunched)
+float batch_gpu_ms(r2d_ctx* c)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_begin, c->ev_end) != hipSuccess) ms = 0.f;
+    return ms;
+}
+
+int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
+               size_t steps)
+{
+    if (nw == 0 || steps == 0) return R2D_OK;
+    for (size_t i = 0; i < nw; ++i)
+        if (worlds[i] && worlds[i]->in_service) return R2D_E_BUSY;
+    // the step scratch is the context's: finish the server's launched batch first
+    if (c->server) {
+        int rc = server_collect(c->server);
+        if (rc) return rc;
+    }
+    Batch b;
+    int rc = batch_prepare(c, worlds, nw, dt, sub_steps, iters, steps, b);
+    if (rc) return rc;
+    if ((rc = batch_launch(c, b))) return rc;
+    return batch_wait(c, b);
 }
 
 void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
@@ -573,6 +649,176 @@ void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
     o->width = w->width[s]; o->height = w->height[s];
 }
 
+// ---------------------------------------------------------------- serving
+// Admission estimate of one resident world: its device block and topology
+// plus its share of the step scratch at the initial capacity estimates
+// (per body: cell ranges, 4 cell entries, 8 raw candidates, 4 manifolds and
+// 4 solver items; per hash bucket 16 B; per joint one solver item).
+constexpr size_t kScratchPerBody = 88 + 4 * 20 + 8 * 32 + 4 * 128 + 4 * 160;
+constexpr size_t kScratchPerBucket = 16;
+constexpr size_t kScratchPerJoint = 160;
+constexpr size_t kScratchPerWorld = 12 + 130 * 4 + sizeof(WorldDev);
+
+size_t world_footprint(const r2d_world* w)
+{
+    size_t n = (size_t)w->n;
+    size_t dev = (size_t)F_NFIELDS * std::max<size_t>(std::max(w->cap, w->dcap), 16) * 4 +
+                 std::max<size_t>(16, w->joints.size()) * sizeof(Joint) +
+                 std::max<size_t>(16, w->gens.size()) * sizeof(Gen) + std::max<size_t>(16, w->excl.size()) * 8;
+    return dev + n * kScratchPerBody + (size_t)table_size(w) * kScratchPerBucket +
+           w->joints.size() * kScratchPerJoint + kScratchPerWorld;
+}
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// A continuous-batching server over one context: requests (a world and a
+// number of steps) wait in FIFO order until the sum of admitted footprints
+// fits the HBM budget and a batch slot is free; every tick collects the
+// batch step launched by the previous tick, retires finished requests,
+// admits waiting ones, and launches one step of every admitted world as one
+// batch.  Worlds stay resident on the device from admission to completion.
+struct r2d_server {
+    struct Req {
+        r2d_world* w = nullptr;
+        uint32_t steps = 0, done = 0;
+        size_t bytes = 0;
+        int32_t state = R2D_REQ_QUEUED;
+        int32_t status = R2D_OK;
+        double t_submit = 0, t_admit = 0, t_done = 0;
+        double max_step_ms = 0;
+    };
+    r2d_ctx* ctx = nullptr;
+    r2d_serve_options opt{};
+    std::vector<Req> reqs;                  // ticket t is reqs[t - 1]
+    std::deque<uint64_t> queue;             // waiting for admission
+    std::vector<uint64_t> running;          // admitted, in order of admission
+    size_t resident = 0, peak_resident = 0;
+    bool inflight = false;
+    Batch batch;                            // the launched batch step
+    std::vector<uint64_t> batch_tickets;
+    double t_launch = 0;
+    uint64_t rejected = 0, completed = 0, batch_steps = 0, scene_steps = 0, body_steps = 0;
+    double gpu_ms = 0, wall_ms = 0;
+    std::vector<float> step_ms;             // device time of each batch step
+    std::vector<uint32_t> step_worlds;      // scenes in it
+    uint32_t last_completed = 0;
+};
+
+namespace {
+
+void retire(r2d_server* sv, uint64_t t, int32_t state, int32_t status)
+{
+    r2d_server::Req& q = sv->reqs[t - 1];
+    q.state = state;
+    q.status = status;
+    q.t_done = now_ms();
+    q.w->in_service = false;
+    sv->resident -= q.bytes;
+    sv->completed += 1;
+    sv->last_completed += 1;
+}
+
+// wait for the launched batch step and account for it
+int server_collect(r2d_server* sv)
+{
+    if (!sv->inflight) return R2D_OK;
+    sv->inflight = false;
+    r2d_ctx* c = sv->ctx;
+    int rc = batch_wait(c, sv->batch);
+    double wall = now_ms() - sv->t_launch;
+    float gms = batch_gpu_ms(c);
+    const size_t nw = sv->batch_tickets.size();
+    // per-world failures arrive as world statuses; anything else fails the batch
+    const bool per_world = rc == R2D_OK || sv->batch.world_status.size() == nw;
+    for (size_t i = 0; i < nw; ++i) {
+        uint64_t t = sv->batch_tickets[i];
+        r2d_server::Req& q = sv->reqs[t - 1];
+        if (!per_world) { retire(sv, t, R2D_REQ_FAILED, rc); continue; }
+        int32_t ws = sv->batch.world_status[i];
+        if (ws != 0) {
+            retire(sv, t, R2D_REQ_FAILED, ws == 4 ? R2D_E_NONFINITE : ws == 5 ? R2D_E_DEVICE : R2D_E_RANGE);
+            continue;
+        }
+        q.done += 1;
+        q.max_step_ms = std::max(q.max_step_ms, (double)gms);
+        sv->body_steps += (uint64_t)q.w->n;
+        if (q.done >= q.steps) retire(sv, t, R2D_REQ_DONE, R2D_OK);
+    }
+    std::vector<uint64_t> still;
+    for (uint64_t t : sv->running)
+        if (sv->reqs[t - 1].state == R2D_REQ_RUNNING) still.push_back(t);
+    sv->running.swap(still);
+    sv->batch_steps += 1;
+    sv->scene_steps += nw;
+    sv->gpu_ms += gms;
+    sv->wall_ms += wall;
+    sv->step_ms.push_back(gms);
+    sv->step_worlds.push_back((uint32_t)nw);
+    return per_world ? R2D_OK : rc;
+}
+
+uint32_t server_admit(r2d_server* sv)
+{
+    uint32_t n = 0;
+    while (!sv->queue.empty() && sv->running.size() < sv->opt.max_worlds) {
+        uint64_t t = sv->queue.front();
+        r2d_server::Req& q = sv->reqs[t - 1];
+        if (sv->resident + q.bytes > sv->opt.hbm_budget_bytes) break;   // FIFO: no overtaking
+        sv->queue.pop_front();
+        q.state = R2D_REQ_RUNNING;
+        q.t_admit = now_ms();
+        sv->resident += q.bytes;
+        sv->peak_resident = std::max(sv->peak_resident, sv->resident);
+        sv->running.push_back(t);
+        n += 1;
+    }
+    return n;
+}
+
+int server_launch(r2d_server* sv)
+{
+    if (sv->running.empty()) return R2D_OK;
+    std::vector<r2d_world*> ws;
+    ws.reserve(sv->running.size());
+    for (uint64_t t : sv->running) ws.push_back(sv->reqs[t - 1].w);
+    r2d_ctx* c = sv->ctx;
+    int rc = batch_prepare(c, ws.data(), ws.size(), sv->opt.dt, sv->opt.sub_steps, sv->opt.collision_iters, 1,
+                           sv->batch);
+    if (!rc) rc = batch_launch(c, sv->batch);
+    if (rc) {
+        for (uint64_t t : sv->running) retire(sv, t, R2D_REQ_FAILED, rc);
+        sv->running.clear();
+        return rc;
+    }
+    sv->batch_tickets = sv->running;
+    sv->t_launch = now_ms();
+    sv->inflight = true;
+    return R2D_OK;
+}
+
+// percentile of the batch-step times, each weighted by its scene count
+double weighted_pct(const std::vector<float>& v, const std::vector<uint32_t>& wts, double p)
+{
+    if (v.empty()) return 0.0;
+    std::vector<std::pair<float, uint32_t>> a(v.size());
+    uint64_t tot = 0;
+    for (size_t i = 0; i < v.size(); ++i) { a[i] = {v[i], wts[i]}; tot += wts[i]; }
+    std::sort(a.begin(), a.end());
+    double target = p * (double)tot;
+    uint64_t acc = 0;
+    for (auto& e : a) {
+        acc += e.second;
+        if ((double)acc >= target) return e.first;
+    }
+    return a.back().first;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -587,6 +833,11 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     if (!c) return R2D_E_OOM;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return R2D_E_DEVICE; }
+    for (int i = 0; i < 2; ++i) HIPCHK(hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&c->ev_begin));
+    HIPCHK(hipEventCreate(&c->ev_end));
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
     for (int i = 0; i < KT_N; ++i) HIPCHK(hipEventCreate(&c->timer.ev[i]));
     set_solve_smem_limit();
@@ -605,6 +856,11 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     if (c->hcounters) (void)hipHostFree(c->hcounters);
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     for (int i = 0; i < KT_N; ++i) (void)hipEventDestroy(c->timer.ev[i]);
+    (void)hipEventDestroy(c->fork);
+    for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c->join[i]);
+    (void)hipEventDestroy(c->ev_begin);
+    (void)hipEventDestroy(c->ev_end);
+    for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(c->aux[i]);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return R2D_OK;
@@ -632,6 +888,7 @@ R2D_EXPORT int r2d_world_create(r2d_ctx* c, float cell, size_t mult, r2d_world**
 R2D_EXPORT int r2d_world_destroy(r2d_world* w)
 {
     if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     (void)hipStreamSynchronize(w->ctx->stream);
     if (w->dblock) (void)hipFree(w->dblock);
     if (w->djoints) (void)hipFree(w->djoints);
@@ -750,6 +1007,7 @@ R2D_EXPORT int r2d_remove_body(r2d_world* w, uint32_t id)
 R2D_EXPORT int r2d_make_gravity(r2d_world* w, float g)
 {
     if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     HGen h{};
     h.type = 0; h.gx = 0.0f; h.gy = -g;
     w->gens.push_back(h);
@@ -760,6 +1018,7 @@ R2D_EXPORT int r2d_make_gravity(r2d_world* w, float g)
 R2D_EXPORT int r2d_make_spring(r2d_world* w, uint32_t id, float k, float c, float ax, float ay)
 {
     if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     if (w->slot_of(id) < 0) return R2D_E_INVALID_ID;
     HGen h{};
     h.type = 1; h.id = id; h.k = k; h.c = c; h.ax = ax; h.ay = ay;
@@ -773,6 +1032,7 @@ static const r2d_constraint_params kDefaultParams = {INFINITY, -INFINITY, 10.0f}
 static int add_joint(r2d_world* w, const r2d_constraint_params* p, HJoint h, size_t* handle)
 {
     if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     if (!p) p = &kDefaultParams;
     h.pmax = p->power_max; h.pmin = p->power_min; h.beta = p->beta;
     w->joints.push_back(h);
@@ -817,6 +1077,7 @@ R2D_EXPORT int r2d_make_motor_joint(r2d_world* w, const r2d_constraint_params* p
 R2D_EXPORT int r2d_exclude_collision_pair(r2d_world* w, uint32_t a, uint32_t b)
 {
     if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     uint64_t k = a < b ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
     auto it = std::lower_bound(w->excl.begin(), w->excl.end(), k);
     if (it == w->excl.end() || *it != k) w->excl.insert(it, k);
@@ -1035,5 +1296,159 @@ R2D_EXPORT int r2d_ctx_last_counts(r2d_ctx* c, uint64_t* out)
 {
     if (!c || !out) return R2D_E_INVALID_ARG;
     for (int i = 0; i < 4; ++i) out[i] = c->last_counts[i];
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_make_bodies(r2d_world* w, size_t n, const r2d_body_options* bo, const int32_t* kind,
+                               const float* dim_a, const float* dim_b, uint32_t* ids_out)
+{
+    if (!w || (n && (!bo || !kind || !dim_a || !dim_b))) return R2D_E_INVALID_ARG;
+    for (size_t i = 0; i < n; ++i)
+        if (kind[i] != 0 && kind[i] != 1) return R2D_E_INVALID_ARG;
+    int rc = begin_edit(w);
+    if (rc) return rc;
+    grow_host(w, (size_t)w->n + n);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t id = 0;
+        rc = make_body_common(w, &bo[i], kind[i] == 1, dim_a[i], kind[i] == 1 ? dim_b[i] : 0.f, &id);
+        if (rc) return rc;
+        if (ids_out) ids_out[i] = id;
+    }
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_world_footprint(r2d_world* w, size_t* bytes)
+{
+    if (!w || !bytes) return R2D_E_INVALID_ARG;
+    *bytes = world_footprint(w);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_serve_create(r2d_ctx* c, const r2d_serve_options* opt, r2d_server** out)
+{
+    if (!c || !opt || !out || opt->max_worlds == 0 || opt->hbm_budget_bytes == 0) return R2D_E_INVALID_ARG;
+    if (c->server) return R2D_E_BUSY;
+    r2d_server* sv = new (std::nothrow) r2d_server();
+    if (!sv) return R2D_E_OOM;
+    sv->ctx = c;
+    sv->opt = *opt;
+    c->server = sv;
+    *out = sv;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_serve_destroy(r2d_server* sv)
+{
+    if (!sv) return R2D_E_INVALID_ARG;
+    int rc = server_collect(sv);
+    for (auto& q : sv->reqs)
+        if (q.w) q.w->in_service = false;
+    sv->ctx->server = nullptr;
+    delete sv;
+    return rc;
+}
+
+R2D_EXPORT int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, uint64_t* ticket)
+{
+    if (!sv || !w || !ticket || w->ctx != sv->ctx) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
+    // the reference raises InvalidRigidBodyId from the first constraint solve
+    if (sv->opt.sub_steps > 0 && sv->opt.collision_iters > 0)
+        for (const HJoint& h : w->joints)
+            if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
+    size_t bytes = world_footprint(w);
+    if (bytes > sv->opt.hbm_budget_bytes) { sv->rejected += 1; return R2D_E_CAPACITY; }
+    r2d_server::Req q;
+    q.w = w;
+    q.steps = steps;
+    q.bytes = bytes;
+    q.t_submit = now_ms();
+    sv->reqs.push_back(q);
+    *ticket = sv->reqs.size();
+    w->in_service = true;
+    if (steps == 0) {
+        sv->resident += bytes;   // retire() gives it back
+        retire(sv, *ticket, R2D_REQ_DONE, R2D_OK);
+        return R2D_OK;
+    }
+    sv->queue.push_back(*ticket);
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_serve_tick(r2d_server* sv, r2d_serve_tick_info* info)
+{
+    if (!sv) return R2D_E_INVALID_ARG;
+    sv->last_completed = 0;
+    double gpu = 0, wall = 0;
+    const bool had = sv->inflight;
+    const double t_launch = sv->t_launch;
+    int rc = server_collect(sv);
+    if (had && !sv->step_ms.empty()) { gpu = sv->step_ms.back(); wall = now_ms() - t_launch; }
+    uint32_t admitted = server_admit(sv);
+    int rc2 = server_launch(sv);
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->admitted = admitted;
+        info->completed = sv->last_completed;
+        info->queued = (uint32_t)sv->queue.size();
+        info->running = (uint32_t)sv->running.size();
+        info->launched_worlds = sv->inflight ? (uint32_t)sv->batch_tickets.size() : 0u;
+        info->launched_bodies = sv->inflight ? sv->batch.N : 0u;
+        info->resident_bytes = sv->resident;
+        info->step_gpu_ms = gpu;
+        info->step_wall_ms = wall;
+    }
+    return rc ? rc : rc2;
+}
+
+R2D_EXPORT int r2d_serve_drain(r2d_server* sv)
+{
+    if (!sv) return R2D_E_INVALID_ARG;
+    int first = R2D_OK;
+    while (sv->inflight || !sv->running.empty() || !sv->queue.empty()) {
+        int rc = r2d_serve_tick(sv, nullptr);
+        if (rc && first == R2D_OK) first = rc;
+    }
+    return first;
+}
+
+R2D_EXPORT int r2d_serve_poll(r2d_server* sv, uint64_t ticket, r2d_request_status* out)
+{
+    if (!sv || !out || ticket == 0 || ticket > sv->reqs.size()) return R2D_E_INVALID_ARG;
+    const r2d_server::Req& q = sv->reqs[ticket - 1];
+    std::memset(out, 0, sizeof *out);
+    out->state = q.state;
+    out->status = q.status;
+    out->steps_done = q.done;
+    out->steps_requested = q.steps;
+    out->footprint_bytes = q.bytes;
+    double now = now_ms();
+    out->wait_ms = (q.state == R2D_REQ_QUEUED ? now : q.t_admit) - q.t_submit;
+    out->total_ms = (q.state >= R2D_REQ_DONE ? q.t_done : now) - q.t_submit;
+    out->max_step_ms = q.max_step_ms;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_serve_stats(r2d_server* sv, r2d_serve_summary* out)
+{
+    if (!sv || !out) return R2D_E_INVALID_ARG;
+    std::memset(out, 0, sizeof *out);
+    out->submitted = sv->reqs.size();
+    out->completed = sv->completed;
+    out->rejected = sv->rejected;
+    out->queued = sv->queue.size();
+    out->running = sv->running.size();
+    out->batch_steps = sv->batch_steps;
+    out->scene_steps = sv->scene_steps;
+    out->body_steps = sv->body_steps;
+    out->gpu_ms = sv->gpu_ms;
+    out->wall_ms = sv->wall_ms;
+    out->p50_step_ms = weighted_pct(sv->step_ms, sv->step_worlds, 0.50);
+    out->p99_step_ms = weighted_pct(sv->step_ms, sv->step_worlds, 0.99);
+    std::vector<uint32_t> ones(sv->step_ms.size(), 1u);
+    out->p50_batch_ms = weighted_pct(sv->step_ms, ones, 0.50);
+    out->p99_batch_ms = weighted_pct(sv->step_ms, ones, 0.99);
+    out->resident_bytes = sv->resident;
+    out->peak_resident_bytes = sv->peak_resident;
     return R2D_OK;
 }

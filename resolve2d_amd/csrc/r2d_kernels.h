@@ -21,14 +21,38 @@ size_t solve_lds_bytes(int nbodies);
 // the four phases of one step, enqueued in this order on one stream
 enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
-// solve-kernel variants (workgroup size x waves per SIMD; table in r2d_kernels.hip)
-constexpr int SOLVE_VARIANTS = 7;
+// solve-kernel variants (register-slot kernel shape + record-kernel workgroup
+// size; table in r2d_kernels.hip)
+constexpr int SOLVE_VARIANTS = 8;
 unsigned solve_variant_threads(int v);
 int solve_variant_slots(int v);
 
-// enqueue one phase of a batched step on stream st
-void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_t sub_steps, uint32_t iters,
-                  uint32_t max_bodies, bool lds, int solve_variant);
+// worlds of at most this many bodies may take the register-slot kernel
+constexpr uint32_t SLOT_NMAX = 1024;
+
+// how one batch's solve phase is split over the solve kernels
+struct SolvePlan {
+    int variant = 0;
+    uint32_t slot_cap = 0;    // items a world may have to take the slot kernel (0: no slot kernel)
+    uint32_t slot_nmax = 0;   // bodies a world may have to take the slot kernel
+    uint32_t lds_nmax = 0;    // bodies a world may have to keep its body state in LDS
+    uint32_t slot_maxn = 0;   // largest world each launch may take: sizes its LDS
+    uint32_t lds_maxn = 0;
+    uint32_t any_slot = 0, any_lds = 0, any_big = 0;
+};
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw);
+
+// the context's main stream plus two side streams for the solve kernels
+struct Streams {
+    hipStream_t main;
+    hipStream_t aux[2];
+    hipEvent_t fork;
+    hipEvent_t join[2];
+};
+
+// enqueue one phase of a batched step (phases in PH_* order on ss.main)
+void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint32_t sub_steps, uint32_t iters,
+                  const SolvePlan& sp);
 
 void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_const, uint32_t cap,
               uint32_t* total_dev, uint32_t* partials, hipStream_t st);

@@ -24,6 +24,8 @@
 //                   dataflow that reproduces the reference's sequential order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "r2d_device.h"
 #include "r2d_kernels.h"
 
@@ -1157,9 +1159,21 @@ __device__ __forceinline__ void rec_writeback(const Rec& r, const StepDev& S)
 }
 
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
-constexpr size_t SLOT_LDS_MAX = 150 * 1024;
 // LDS of the register-slot kernel: 13 words of body state per body
 __host__ __device__ inline size_t slot_lds_bytes(int nbodies) { return (size_t)nbodies * 13u * 4u; }
+
+// Which solve kernel steps a world: 0 the register-slot kernel (at most
+// slot_nmax bodies and slot_cap items), 1 the record kernel with LDS body
+// state (at most lds_nmax bodies), 2 the record kernel with body state in
+// global memory.  Every kernel evaluates the same rule, so each world is
+// stepped by exactly one of them.
+__device__ __forceinline__ int solve_route(uint32_t n, uint32_t nitems, uint32_t slot_cap, uint32_t slot_nmax,
+                                           uint32_t lds_nmax)
+{
+    if (slot_cap && n <= slot_nmax && nitems <= slot_cap) return 0;
+    if (n <= lds_nmax) return 1;
+    return 2;
+}
 
 // One workgroup steps one world: S sub-steps of forces, integration,
 // pre-step and K solver iterations (lib.zig:199-250).  Solver items run as a
@@ -1170,16 +1184,16 @@ __host__ __device__ inline size_t slot_lds_bytes(int nbodies) { return (size_t)n
 // so the earliest unfinished item always has a worker.
 template <bool LDS, int MAXT, int WAVES = 1>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
-void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap)
+void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap, uint32_t slot_nmax,
+             uint32_t lds_nmax)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = blockIdx.x;
     if (S.w_status[w] != 0) return;
-    // worlds whose items fit the register-slot kernel were stepped by it
-    if (S.w_item_base[w + 1] - S.w_item_base[w] <= slot_cap && S.worlds[w].nbodies < 0xffff &&
-        slot_lds_bytes(S.worlds[w].nbodies) <= SLOT_LDS_MAX)
+    if (solve_route((uint32_t)S.worlds[w].nbodies, S.w_item_base[w + 1] - S.w_item_base[w], slot_cap, slot_nmax,
+                    lds_nmax) != (LDS ? 1 : 2))
         return;
     if (threadIdx.x == 0) s_abort = 0u;
     const WorldDev& D = S.worlds[w];
@@ -1539,7 +1553,7 @@ __device__ __forceinline__ void slot_solve_joint(const Slot& z, const SlotState&
 
 template <int NSLOT, int MAXT, int WAVES>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
-void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
+void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
@@ -1550,7 +1564,7 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters)
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const WorldDev& D = S.worlds[w];
     const int n = D.nbodies;
-    if (nitems > (uint32_t)NSLOT * MAXT || n >= 0xffff || slot_lds_bytes(n) > SLOT_LDS_MAX) return;
+    if (solve_route((uint32_t)n, nitems, (uint32_t)NSLOT * MAXT, slot_nmax, 0u) != 0) return;
     if (threadIdx.x == 0) s_abort = 0u;
     const uint32_t base = D.body_base;
     const uint32_t nj = D.njoints;
@@ -1770,13 +1784,13 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs, unsigned cap = 4096)
 
 size_t solve_lds_bytes(int nbodies) { return (size_t)nbodies * 7u * 4u; }
 
-typedef void (*SlotFn)(StepDev, float, uint32_t, uint32_t);
-typedef void (*RecFn)(StepDev, float, uint32_t, uint32_t, uint32_t);
+typedef void (*SlotFn)(StepDev, float, uint32_t, uint32_t, uint32_t);
+typedef void (*RecFn)(StepDev, float, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t);
 struct SolveVariant {
     SlotFn slot_fn;      // register-slot kernel (nullptr: none)
     unsigned slot_threads;
     int slots;
-    RecFn rec_fn;        // record kernel for the worlds the slot kernel leaves
+    RecFn rec_fn;        // record kernel (LDS body state) for the worlds the slot kernel leaves
     unsigned threads;
 };
 // solve variants: the register-slot kernel's slots x workgroup size, and the
@@ -1789,16 +1803,35 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256},
     {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256},
     {nullptr, 0, 0, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 1024>, 1024},
 };
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
 int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
 
-void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_t sub_steps, uint32_t iters,
-                  uint32_t max_bodies, bool lds, int solve_variant)
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw)
+{
+    SolvePlan p;
+    p.variant = (variant >= 0 && variant < SOLVE_VARIANTS) ? variant : 0;
+    const SolveVariant& v = kSolveVariants[p.variant];
+    p.slot_cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
+    p.slot_nmax = v.slot_fn ? SLOT_NMAX : 0u;
+    p.lds_nmax = (uint32_t)((160u * 1024u - 1024u) / solve_lds_bytes(1));
+    for (size_t i = 0; i < nw; ++i) {
+        uint32_t n = nbodies[i];
+        if (p.slot_cap && n <= p.slot_nmax) { p.any_slot = 1; p.slot_maxn = std::max(p.slot_maxn, n); }
+        if (n <= p.lds_nmax) { p.any_lds = 1; p.lds_maxn = std::max(p.lds_maxn, n); }
+        else p.any_big = 1;
+    }
+    return p;
+}
+
+void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint32_t sub_steps, uint32_t iters,
+                  const SolvePlan& sp)
 {
     const unsigned BS = 256;
     const unsigned GRID = 1024;   // grid-stride kernels over device-sized ranges
+    hipStream_t st = ss.main;
     switch (phase) {
     case PH_BROAD:
         hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, S.counters);
@@ -1837,19 +1870,31 @@ void launch_phase(const StepDev& S, hipStream_t st, int phase, float dt, uint32_
         hipLaunchKernelGGL(k_list_rank, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
         break;
     default: {
-        const SolveVariant& v = kSolveVariants[solve_variant < SOLVE_VARIANTS ? solve_variant : 0];
-        size_t sm = solve_lds_bytes((int)max_bodies);
-        uint32_t cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
-        bool slot_ok = v.slot_fn && slot_lds_bytes((int)max_bodies) <= SLOT_LDS_MAX;
-        if (!slot_ok) cap = 0u;
-        if (lds && slot_ok)
-            hipLaunchKernelGGL(v.slot_fn, dim3(S.nworlds), dim3(v.slot_threads), slot_lds_bytes((int)max_bodies), st,
-                               S, dt, sub_steps, iters);
-        if (lds) {
-            hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), sm, st, S, dt, sub_steps, iters, cap);
-        } else {
-            hipLaunchKernelGGL((k_solve<false, 1024>), dim3(S.nworlds), dim3(1024), 0, st, S, dt, sub_steps, iters,
-                               0u);
+        // the three solve kernels take disjoint sets of worlds (solve_route);
+        // the record kernels run on side streams beside the slot kernel
+        const SolveVariant& v = kSolveVariants[sp.variant];
+        const bool run_slot = v.slot_fn && sp.any_slot, run_lds = sp.any_lds != 0, run_big = sp.any_big != 0;
+        if (run_lds || run_big) {
+            (void)hipEventRecord(ss.fork, st);
+            if (run_big) (void)hipStreamWaitEvent(ss.aux[1], ss.fork, 0);
+            if (run_lds) (void)hipStreamWaitEvent(ss.aux[0], ss.fork, 0);
+        }
+        if (run_big)
+            hipLaunchKernelGGL((k_solve<false, 1024>), dim3(S.nworlds), dim3(1024), 0, ss.aux[1], S, dt, sub_steps,
+                               iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
+        if (run_lds)
+            hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), solve_lds_bytes((int)sp.lds_maxn),
+                               ss.aux[0], S, dt, sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
+        if (run_slot)
+            hipLaunchKernelGGL(v.slot_fn, dim3(S.nworlds), dim3(v.slot_threads), slot_lds_bytes((int)sp.slot_maxn),
+                               st, S, dt, sub_steps, iters, sp.slot_nmax);
+        if (run_lds) {
+            (void)hipEventRecord(ss.join[0], ss.aux[0]);
+            (void)hipStreamWaitEvent(st, ss.join[0], 0);
+        }
+        if (run_big) {
+            (void)hipEventRecord(ss.join[1], ss.aux[1]);
+            (void)hipStreamWaitEvent(st, ss.join[1], 0);
         }
         hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
         break;

@@ -105,17 +105,42 @@ class GpuWorld:
         return self.lib.r2d_process(self.handle, dt, sub_steps, iters)
 
     # --- state export
+    def make_bodies(self, kind, px, py, vx, vy, angle, omega, mu, mass_kind, mass_value, dim_a, dim_b):
+        """n bodies in one call (r2d_make_bodies); every argument an array of n."""
+        n = len(kind)
+        opts = (N.BodyOptions * max(n, 1))()
+        rec = np.zeros(n, dtype=[("pos_x", "<f4"), ("pos_y", "<f4"), ("vel_x", "<f4"), ("vel_y", "<f4"),
+                                 ("angle", "<f4"), ("omega", "<f4"), ("mu", "<f4"), ("mass_kind", "<i4"),
+                                 ("mass_value", "<f4")])
+        for name, v in (("pos_x", px), ("pos_y", py), ("vel_x", vx), ("vel_y", vy), ("angle", angle),
+                        ("omega", omega), ("mu", mu), ("mass_kind", mass_kind), ("mass_value", mass_value)):
+            rec[name] = v
+        ctypes.memmove(opts, rec.ctypes.data, rec.nbytes)
+        k = np.ascontiguousarray(kind, dtype=np.int32)
+        a = np.ascontiguousarray(dim_a, dtype=np.float32)
+        b = np.ascontiguousarray(dim_b, dtype=np.float32)
+        ids = np.zeros(max(n, 1), dtype=np.uint32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        N.check(self.lib.r2d_make_bodies(self.handle, n, opts, k.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         a.ctypes.data_as(fp), b.ctypes.data_as(fp),
+                                         ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))), "r2d_make_bodies")
+        return ids[:n]
+
+    def footprint(self) -> int:
+        b = ctypes.c_size_t()
+        N.check(self.lib.r2d_world_footprint(self.handle, ctypes.byref(b)), "r2d_world_footprint")
+        return b.value
+
     def bodies(self) -> np.ndarray:
         n = ctypes.c_size_t()
         N.check(self.lib.r2d_num_bodies(self.handle, ctypes.byref(n)), "r2d_num_bodies")
         arr = (N.BodyState * max(n.value, 1))()
         N.check(self.lib.r2d_get_bodies(self.handle, arr, n.value, ctypes.byref(n)), "r2d_get_bodies")
+        words = ctypes.sizeof(N.BodyState) // 4
+        raw = np.frombuffer(arr, dtype=np.uint32, count=n.value * words).reshape(n.value, words)
         out = np.zeros((n.value, BODY_COLS))
-        for i in range(n.value):
-            s = arr[i]
-            out[i] = (s.id, s.type, s.is_static, s.pos_x, s.pos_y, s.mom_x, s.mom_y, s.force_x, s.force_y, s.mass,
-                      s.angle, s.ang_momentum, s.torque, s.inertia, s.mu, s.aabb_x, s.aabb_y, s.aabb_half_width,
-                      s.aabb_half_height)
+        out[:, 0:3] = raw[:, 0:3]
+        out[:, 3:] = raw[:, 4:20].view(np.float32)
         return out
 
     def manifolds(self) -> np.ndarray:

@@ -195,3 +195,68 @@ def car_demo():
     anchor = b.rect(-15.0, 9.0, 0.6, 0.4, mu=0.5)
     b.joint(2, anchor, tpos=(-15.0, 9.5), beta=5.0)
     return b.ops
+
+
+# ------------------------------------------------------------- served scenes
+def served_arrays(seed: int, n: int):
+    """A served scene of n mixed bodies with the geometry and materials of
+    `mixed` (discs r ~ U[0.2, 0.6], rectangles w, h ~ U[0.3, 1.0], angle ~
+    U[0, 2pi), density 1, friction 0.4, alternating disc / rectangle), drawn
+    vectorised so that scenes of thousands of bodies are cheap to generate.
+    The box scales with n at the density of the 4,096-body scene (100 x 200 m):
+    width = max(8, 100 sqrt(n / 4096)), height = 2 width.  Returns float32 /
+    int32 arrays in creation order (the ground is created first, as id 0)."""
+    rng = np.random.default_rng(seed)
+    width = max(8.0, 100.0 * math.sqrt(n / 4096.0))
+    height = 2.0 * width
+    x = rng.uniform(-width / 2, width / 2, n).astype(np.float32)
+    y = rng.uniform(1.0, 1.0 + height, n).astype(np.float32)
+    r = rng.uniform(0.2, 0.6, n).astype(np.float32)
+    w = rng.uniform(0.3, 1.0, n).astype(np.float32)
+    h = rng.uniform(0.3, 1.0, n).astype(np.float32)
+    ang = rng.uniform(0.0, 2 * math.pi, n).astype(np.float32)
+    kind = (np.arange(n) % 2).astype(np.int32)            # 0 disc, 1 rectangle
+    rect = kind == 1
+    return dict(n=n, width=np.float32(width), kind=kind, px=x, py=y,
+                angle=np.where(rect, ang, np.float32(0.0)).astype(np.float32),
+                dim_a=np.where(rect, w, r).astype(np.float32), dim_b=np.where(rect, h, np.float32(0.0)).astype(np.float32))
+
+
+def served_ops(seed: int, n: int):
+    """The scene of `served_arrays` as a list of scene operations (for the
+    oracle and small tests)."""
+    a = served_arrays(seed, n)
+    b = _Builder()
+    b.gravity(9.81)
+    ground = b.rect(0.0, -0.5, float(a["width"]) + 20.0, 1.0, mu=0.4)
+    b.static(ground)
+    for i in range(n):
+        if a["kind"][i] == 0:
+            b.disc(a["px"][i], a["py"][i], a["dim_a"][i], mu=0.4)
+        else:
+            b.rect(a["px"][i], a["py"][i], a["dim_a"][i], a["dim_b"][i], mu=0.4, angle=a["angle"][i])
+    return b.ops
+
+
+def build_served(world, seed: int, n: int):
+    """Build the scene of `served_arrays` into an engine world with one bulk
+    call (GpuWorld.make_bodies); the result equals applying `served_ops`."""
+    a = served_arrays(seed, n)
+    world.make_gravity(f32(9.81))
+    g = world.make_rect(0.0, -0.5, 0.0, 0.0, 0.0, 0.0, f32(0.4), 0, 1.0, f32(float(a["width"]) + 20.0), 1.0)
+    world.set_static(g, True)
+    z = np.zeros(n, dtype=np.float32)
+    world.make_bodies(a["kind"], a["px"], a["py"], z, z, a["angle"], z, np.full(n, f32(0.4), dtype=np.float32),
+                      np.zeros(n, dtype=np.int32), np.ones(n, dtype=np.float32), a["dim_a"], a["dim_b"])
+    return world
+
+
+def poisson_stream(seed: int, rate: float, count: int, n_min: int = 16, n_max: int = 8192):
+    """Arrival times (s, from 0) of a Poisson stream of `count` scenes at
+    `rate` scenes/s, with sizes log-uniform in [n_min, n_max] and one scene
+    seed each: a list of (t, n, scene_seed)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    n = np.exp(rng.uniform(math.log(n_min), math.log(n_max), count)).round().astype(int)
+    seeds = rng.integers(0, 2 ** 31, count)
+    return [(float(t[i]), int(n[i]), int(seeds[i])) for i in range(count)]

@@ -1,0 +1,100 @@
+"""Continuous-batching server (Python mirror of r2d_serve_* in include/r2d.h).
+
+The reference steps one world per `Solver.process` call (src/core/lib.zig:189);
+serving has no counterpart there.  The server is native (r2d_api.hip): it
+admits requests in arrival order while their footprints fit an HBM budget,
+keeps admitted worlds resident on the device, and steps every running world
+together, one batched launch per tick.  `run_stream` drives it with timed
+arrivals the way a request loop would.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as N
+from .scenes import DT, ITERS, SUB_STEPS
+
+
+class Server:
+    def __init__(self, ctx, hbm_budget_bytes: int, max_worlds: int, dt: float = DT, sub_steps: int = SUB_STEPS,
+                 collision_iters: int = ITERS):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        o = N.ServeOptions()
+        o.hbm_budget_bytes, o.max_worlds, o.dt = hbm_budget_bytes, max_worlds, dt
+        o.sub_steps, o.collision_iters = sub_steps, collision_iters
+        h = ctypes.c_void_p()
+        N.check(self.lib.r2d_serve_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)), "r2d_serve_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            N.check(self.lib.r2d_serve_destroy(self.handle), "r2d_serve_destroy")
+            self.handle = None
+
+    def submit(self, world, steps: int) -> int:
+        t = ctypes.c_uint64()
+        N.check(self.lib.r2d_serve_submit(self.handle, world.handle, steps, ctypes.byref(t)), "r2d_serve_submit")
+        return t.value
+
+    def tick(self) -> dict:
+        info = N.ServeTickInfo()
+        N.check(self.lib.r2d_serve_tick(self.handle, ctypes.byref(info)), "r2d_serve_tick")
+        return {f: getattr(info, f) for f, _ in N.ServeTickInfo._fields_}
+
+    def drain(self):
+        N.check(self.lib.r2d_serve_drain(self.handle), "r2d_serve_drain")
+
+    def poll(self, ticket: int) -> dict:
+        st = N.RequestStatus()
+        N.check(self.lib.r2d_serve_poll(self.handle, ticket, ctypes.byref(st)), "r2d_serve_poll")
+        return {f: getattr(st, f) for f, _ in N.RequestStatus._fields_}
+
+    def stats(self) -> dict:
+        s = N.ServeSummary()
+        N.check(self.lib.r2d_serve_stats(self.handle, ctypes.byref(s)), "r2d_serve_stats")
+        return {f: getattr(s, f) for f, _ in N.ServeSummary._fields_}
+
+
+def run_stream(server: Server, arrivals, build, steps: int, time_limit: float = None, on_done=None):
+    """Serve a timed stream.  `arrivals` is a list of (t_seconds, payload);
+    `build(payload)` returns a new world for it (built on the host while the
+    device steps the running batch); `on_done(payload, world, status)` is
+    called as requests finish (the caller may read and free the world).
+    Arrivals after `time_limit` seconds are not submitted.  Returns a dict of
+    wall time, request latencies and the server's statistics."""
+    t0 = time.perf_counter()
+    pending = list(arrivals)
+    pending.reverse()
+    live = {}
+    lat = []
+    submitted = 0
+    while pending or live:
+        now = time.perf_counter() - t0
+        while pending and pending[-1][0] <= now:
+            t_arr, payload = pending.pop()
+            if time_limit is not None and t_arr > time_limit:
+                pending.clear()
+                break
+            w = build(payload)
+            live[server.submit(w, steps)] = (payload, w)
+            submitted += 1
+        info = server.tick()
+        if info["completed"]:
+            for t in [t for t in live]:
+                st = server.poll(t)
+                if st["state"] >= N.REQ_DONE:
+                    payload, w = live.pop(t)
+                    lat.append(st["total_ms"])
+                    if on_done:
+                        on_done(payload, w, st)
+        if not live and pending:
+            # idle until the next arrival
+            time.sleep(max(0.0, min(pending[-1][0] - (time.perf_counter() - t0), 0.05)))
+    wall = time.perf_counter() - t0
+    out = dict(wall_s=wall, submitted=submitted, stats=server.stats())
+    if lat:
+        out["request_p50_ms"] = float(np.percentile(lat, 50))
+        out["request_p99_ms"] = float(np.percentile(lat, 99))
+    return out
