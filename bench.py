@@ -1,18 +1,25 @@
 """Benchmark of the resolve2d step on one MI355X.
 
-Workload (default): 1024 independent worlds, body counts uniform in [50, 500]
-(mixed discs / rectangles), each with 0-20 distance joints and 0-10 anchored
-springs, stepped together (dt = 1/60, 4 sub-steps, 4 solver iterations).
-A step is one Solver.process over every world of the batch.  Inputs are
-resident on the device before timing starts.
+Workload (the headline value): 1024 independent worlds, body counts uniform
+in [50, 500] (mixed discs / rectangles), each with 0-20 distance joints and
+0-10 anchored springs, stepped together (dt = 1/60, 4 sub-steps, 4 solver
+iterations).  A step is one Solver.process over every world of the batch;
+inputs are resident on the device before timing starts.
 
-Printed JSON line: body-steps/s of the batched step (value), p50/p99 step
-latency, the same scenes stepped one world per launch ("plain"), a roofline
-entry for the dominant kernel (timed with HIP events on the library's stream),
-and a CPU baseline (the oracle restatement on a bounded sample, one core).
+The printed JSON line also carries: the same scenes stepped one world per
+launch ("plain"); the 4,096-body single world in the reference order and in
+the coloured schedule; a served Poisson stream of scenes of 16-8192 bodies
+(continuous batching, p50/p99 step and request latency); accuracy (bit
+equality with the f32 oracle, error against an f64 run of the same method,
+maximum penetration, energy drift of the resting pyramid); a roofline entry
+for the dominant kernel (phase times from HIP events on the library's
+stream); and a CPU baseline (the oracle restatement on a bounded sample,
+worlds in parallel over host cores).
 """
 import argparse
 import json
+import math
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -23,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+G = 9.81
 
 
 def log(msg):
@@ -34,47 +42,204 @@ def checked(rc, what):
     _native.check(rc, what)
 
 
-def build_worlds(ctx, n_worlds, seed0):
-    from resolve2d_amd import scenes
-    from resolve2d_amd.engine import GpuWorld, apply_scene
-    worlds, specs = [], []
-    for i in range(n_worlds):
-        sc = scenes.batched_world(seed0 + i)
-        w = GpuWorld(ctx)
-        apply_scene(sc, w)
-        worlds.append(w)
-        specs.append(sc)
-    return worlds, specs
-
-
 def count_bodies(sc):
     return sum(1 for op in sc if op[0] in ("disc", "rect"))
 
 
-def cpu_baseline(specs, steps, budget_s):
-    """Oracle (f32 restatement of the reference step), one core, on worlds spread
-    over the batch, each over the full horizon, until the time budget is spent."""
+# ----------------------------------------------------------------- CPU baseline
+def _cpu_worker(args):
+    """One host core: the oracle (f32) on its share of the sampled worlds, each
+    over the full horizon, until its time budget is spent."""
+    specs, steps, budget = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import apply_scene
     body_steps, t_total, used = 0, 0.0, 0
-    order = list(range(0, len(specs), 37)) + [i for i in range(len(specs)) if i % 37]
-    for idx in order:
-        sc = specs[idx]
+    for sc in specs:
         o = pyoracle.OracleWorld("f32")
         apply_scene(sc, o)
-        n = count_bodies(sc)
         t0 = time.perf_counter()
         for _ in range(steps):
             o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
         t_total += time.perf_counter() - t0
-        body_steps += n * steps
+        body_steps += count_bodies(sc) * steps
         used += 1
         o.close()
-        if t_total > budget_s:
+        if t_total > budget:
             break
-    return body_steps / t_total, used, t_total
+    return body_steps, t_total, used
+
+
+def cpu_baseline(specs, steps, budget_s, cores):
+    """Worlds in parallel over `cores` host processes (started before the GPU is
+    touched); each process steps whole worlds spread over the batch."""
+    order = list(range(0, len(specs), 7)) + [i for i in range(len(specs)) if i % 7]
+    shares = [[specs[order[j]] for j in range(k, len(order), cores)] for k in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(sh, steps, budget_s) for sh in shares])
+    wall = time.perf_counter() - t0
+    body_steps = sum(r[0] for r in res)
+    busy = max(r[1] for r in res)
+    return body_steps / busy, sum(r[2] for r in res), busy, wall
+
+
+# ------------------------------------------------------------------- accuracy
+def energy(b):
+    """Kinetic + gravitational potential energy of the dynamic bodies of a
+    body array (engine layout: static col 2, pos 3-4, momentum 5-6, mass 9,
+    angular momentum 11, inertia 13)."""
+    dyn = b[:, 2] == 0
+    m, I = b[dyn, 9], b[dyn, 13]
+    p2 = b[dyn, 5] ** 2 + b[dyn, 6] ** 2
+    return float(np.sum(0.5 * p2 / m + 0.5 * b[dyn, 11] ** 2 / I + m * G * b[dyn, 4]))
+
+
+def state_error(gpu, ref):
+    """Max position / velocity error of gpu against ref (engine body arrays),
+    absolute and relative to the largest magnitude in ref."""
+    dyn = ref[:, 2] == 0
+    dp = np.abs(gpu[dyn, 3:5] - ref[dyn, 3:5]).max(initial=0.0)
+    v_g = gpu[dyn, 5:7] / gpu[dyn, 9:10]
+    v_r = ref[dyn, 5:7] / ref[dyn, 9:10]
+    dv = np.abs(v_g - v_r).max(initial=0.0)
+    return dict(max_abs_pos_m=float(dp), max_abs_vel_ms=float(dv),
+                rel_pos=float(dp / max(1.0, np.abs(ref[dyn, 3:5]).max(initial=0.0))),
+                rel_vel=float(dv / max(1e-6, np.abs(v_r).max(initial=0.0))))
+
+
+def max_penetration(man):
+    """Deepest contact (m) among the valid points of a manifold array."""
+    best = 0.0
+    for p in range(2):
+        base = 7 + 13 * p
+        valid = man[:, base] != 0
+        if valid.any():
+            best = max(best, float(-man[valid, base + 9].min()))
+    return best
+
+
+def accuracy(ctx, specs, sample, steps, snaps):
+    """The sampled worlds stepped on the GPU (batched together) against the
+    oracle in f32 (must be bit-equal) and in f64 (error of the method in f32)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    gw = [GpuWorld(ctx) for _ in sample]
+    o32 = [pyoracle.OracleWorld("f32") for _ in sample]
+    o64 = [pyoracle.OracleWorld("f64") for _ in sample]
+    for i, idx in enumerate(sample):
+        for e in (gw[i], o32[i], o64[i]):
+            apply_scene(specs[idx], e)
+    res = {}
+    bit_equal = True
+    for s in range(1, steps + 1):
+        ctx.process_batch_handles([w.handle for w in gw], scenes.DT, scenes.SUB_STEPS, scenes.ITERS, 1)
+        for i in range(len(sample)):
+            o32[i].process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+            o64[i].process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+        if s in snaps:
+            errs = []
+            for i in range(len(sample)):
+                g = gw[i].bodies()
+                a32 = o32[i].bodies()
+                bit_equal &= bool((g.astype(np.float32).view(np.uint32) ==
+                                   a32.astype(np.float32).view(np.uint32)).all())
+                errs.append(state_error(g, o64[i].bodies()))
+            res[f"step_{s}"] = {k: max(e[k] for e in errs) for k in errs[0]}
+    pen = max(max_penetration(w.manifolds()) for w in gw)
+    for e in gw + o32 + o64:
+        e.close()
+    return res, bit_equal, pen
+
+
+def pyramid_energy(ctx, steps):
+    """Energy drift of the resting 20-row pyramid (configuration 1) over the
+    horizon: GPU f32 against the f64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    g = GpuWorld(ctx)
+    o = pyoracle.OracleWorld("f64")
+    apply_scene(scenes.pyramid(20), g)
+    apply_scene(scenes.pyramid(20), o)
+    e0 = energy(g.bodies())
+    worst = 0.0
+    for s in range(1, steps + 1):
+        checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "pyramid step")
+        o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+        if s % 20 == 0 or s == steps:
+            worst = max(worst, abs(energy(g.bodies()) - e0) / abs(e0))
+    b, r = g.bodies(), o.bodies()
+    out = dict(steps=steps, drift=(energy(b) - e0) / abs(e0), max_abs_drift=worst,
+               drift_f64=(energy(r) - e0) / abs(e0), max_penetration_m=max_penetration(g.manifolds()),
+               vs_f64=state_error(b, r))
+    g.close()
+    o.close()
+    return out
+
+
+# --------------------------------------------------------------------- serving
+def served_arrivals(capacity, seconds, load, seed, steps):
+    """A Poisson stream of scenes (16-8192 bodies, log-uniform) offering `load`
+    x `capacity` body-steps/s for `seconds`."""
+    from resolve2d_amd import scenes
+    mean_n = (8192 - 16) / math.log(8192 / 16)
+    rate = load * capacity / (mean_n * steps)
+    arr = scenes.poisson_stream(seed, rate, max(1, int(rate * seconds)))
+    return rate, [(t, (n, s)) for t, n, s in arr]
+
+
+def serve_stream(ctx, rate, arrivals, steps, load):
+    """Serve a stream with continuous batching; returns its figures."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld
+    from resolve2d_amd.serve import Server, run_stream
+
+    def build(payload):
+        n, s = payload
+        return scenes.build_served(GpuWorld(ctx), s, n)
+
+    def done(payload, w, st):
+        w.close()
+
+    sv = Server(ctx, hbm_budget_bytes=64 << 30, max_worlds=4096)
+    try:
+        r = run_stream(sv, arrivals, build, steps, on_done=done)
+    finally:
+        sv.close()
+    st = r["stats"]
+    return {"value": st["body_steps"] / r["wall_s"], "unit": "body-steps/s", "offered_load": load,
+            "rate_scenes_per_s": rate, "scenes": st["completed"], "steps_per_scene": steps, "wall_s": r["wall_s"],
+            "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"],
+            "p50_batch_ms": st["p50_batch_ms"], "p99_batch_ms": st["p99_batch_ms"],
+            "p50_request_ms": r.get("request_p50_ms"), "p99_request_ms": r.get("request_p99_ms"),
+            "mean_scenes_per_batch": st["scene_steps"] / max(1, st["batch_steps"]), "lanes": st["lanes"],
+            "peak_resident_bytes": st["peak_resident_bytes"]}
+
+
+def plain_served(ctx, arrivals, steps, count):
+    """The same kind of scenes stepped one world per launch over the full
+    horizon: `count` scenes spread over the stream's sizes."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld
+    pick = sorted(arrivals, key=lambda a: a[1][0])
+    pick = [pick[int(i * (len(pick) - 1) / max(1, count - 1))] for i in range(count)]
+    bs, t = 0, 0.0
+    for _, (n, s) in pick:
+        w = scenes.build_served(GpuWorld(ctx), s, n)
+        ctx.sync()
+        a = time.perf_counter()
+        for _ in range(steps):
+            checked(w.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "plain served step")
+        ctx.sync()
+        t += time.perf_counter() - a
+        bs += (n + 1) * steps
+        w.close()
+    return bs / t, [a[1][0] for a in pick]
 
 
 def main():
@@ -84,12 +249,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--worlds", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1000)
-    ap.add_argument("--plain-worlds", type=int, default=32)
-    ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--plain-worlds", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=0, help="plain steps per world (0: the full horizon)")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="host processes for the CPU baseline (0: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--single-steps", type=int, default=300)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--serve-sat-seconds", type=float, default=4.0,
+                    help="also serve an overloaded stream (3x capacity) for this long (0: skip)")
+    ap.add_argument("--serve-plain-scenes", type=int, default=6)
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--tuned-hash", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
     args = ap.parse_args()
@@ -99,22 +272,42 @@ def main():
         sys.exit(2)
 
     from resolve2d_amd import scenes
+    t0 = time.perf_counter()
+    specs = [scenes.batched_world(args.seed + i) for i in range(args.worlds)]
+    n_bodies = sum(count_bodies(sc) for sc in specs)
+    log(f"generated {args.worlds} worlds, {n_bodies} bodies in {time.perf_counter() - t0:.1f} s")
+
+    # CPU baseline first: its worker processes start before anything touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        cores = args.cpu_cores or max(1, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline on {cores} processes")
+        rate, used, busy, wall = cpu_baseline(specs, args.steps, args.cpu_budget, cores)
+        cpu = {"value": rate, "unit": "body-steps/s", "cores": cores, "kind": "port",
+               "sample": f"oracle f32 restatement (the reference's algorithm in C), {used} worlds spread over "
+                         f"the batch, each x {args.steps} steps, worlds in parallel over {cores} processes "
+                         f"({busy:.1f} s per process)"}
+
+    from resolve2d_amd.engine import GpuWorld, apply_scene
     from resolve2d_amd.solver import Context
     ctx = Context(0)
     ctx.set_solve_variant(args.variant)
     t0 = time.perf_counter()
-    worlds, specs = build_worlds(ctx, args.worlds, args.seed)
+    worlds = []
+    for sc in specs:
+        w = GpuWorld(ctx)
+        apply_scene(sc, w)
+        worlds.append(w)
     handles = [w.handle for w in worlds]
-    n_bodies = sum(count_bodies(sc) for sc in specs)
     setup_s = time.perf_counter() - t0
     dt, S, K = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
-
-    log(f"built {args.worlds} worlds, {n_bodies} bodies in {setup_s:.1f} s")
+    log(f"built {args.worlds} device worlds in {setup_s:.1f} s")
     for _ in range(args.warmup):
         ctx.process_batch_handles(handles, dt, S, K, 1)
     ctx.sync()
 
-    # timed region: exactly `steps` batched steps; per-step HIP events on the library stream
+    # timed region: exactly `steps` batched steps (one call each, so each call's
+    # wall time is a step latency); phase events recorded on the library stream
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     lat = []
@@ -129,25 +322,26 @@ def main():
     phase_ms, phase_steps = ctx.phase_times()
     counts = ctx.last_counts()
     ctx.set_profiling(False)
-
     value = n_bodies * args.steps / elapsed
     lat_ms = np.array(lat) * 1e3
-    log(f"batched: {args.steps} steps in {elapsed:.3f} s")
+    log(f"batched: {args.steps} steps in {elapsed:.3f} s ({value / 1e6:.1f} M body-steps/s)")
+    batch_pen = max(max_penetration(w.manifolds()) for w in worlds[:: max(1, len(worlds) // 64)])
+    for w in worlds:
+        w.close()
 
-    # plain: the same scenes, one world per launch
-    from resolve2d_amd.engine import GpuWorld, apply_scene
+    # plain: the same scenes, one world per launch, over the same horizon
+    plain_steps = args.plain_steps or args.steps
     plain_body_steps, plain_t = 0, 0.0
-    for sc in specs[:args.plain_worlds]:
+    for sc in specs[:: max(1, len(specs) // args.plain_worlds)][:args.plain_worlds]:
         w = GpuWorld(ctx)
         apply_scene(sc, w)
-        checked(w.process(dt, S, K), "plain step")
         ctx.sync()
         a = time.perf_counter()
-        for _ in range(args.plain_steps):
+        for _ in range(plain_steps):
             checked(w.process(dt, S, K), "plain step")
         ctx.sync()
         plain_t += time.perf_counter() - a
-        plain_body_steps += count_bodies(sc) * args.plain_steps
+        plain_body_steps += count_bodies(sc) * plain_steps
         w.close()
     plain_value = plain_body_steps / plain_t
     log("plain done")
@@ -163,7 +357,7 @@ def main():
     solve_s = per_step["solve"] / 1e3
     achieved = solve_bytes / solve_s / 1e9 if solve_s > 0 else 0.0
 
-    # configuration 2: one 4,096-body world, full 300-step horizon, one world per launch,
+    # configuration 2: one 4,096-body world, full horizon, one world per launch,
     # reference order vs the coloured schedule
     single = {}
     if not args.no_single:
@@ -189,13 +383,35 @@ def main():
                              "p99_step_ms": float(np.percentile(np.array(lat1) * 1e3, 99))}
             w.close()
 
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline")
-        rate, used, tcpu = cpu_baseline(specs, args.steps, args.cpu_budget)
-        cpu = {"value": rate, "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"oracle f32 restatement, {used} worlds spread over the batch x {args.steps} steps "
-                         f"({tcpu:.1f} s)"}
+    acc = None
+    if not args.no_accuracy:
+        log("accuracy")
+        sample = list(range(0, args.worlds, max(1, args.worlds // 4)))[:4]
+        horizon = args.steps
+        errs, bit_eq, pen = accuracy(ctx, specs, sample, horizon, {1, 60, horizon})
+        acc = {"worlds": sample, "bit_equal_to_f32_oracle": bit_eq, "tolerance_vs_reference_f32": 1e-5,
+               "vs_f64_oracle": errs, "max_penetration_m": pen, "batch_max_penetration_m": batch_pen,
+               "pyramid_600": pyramid_energy(ctx, 600)}
+
+    serve = None
+    if not args.no_serve:
+        log("serving stream")
+        rate, arrivals = served_arrivals(value, args.serve_seconds, args.serve_load, args.seed, args.steps)
+        serve = serve_stream(ctx, rate, arrivals, args.steps, args.serve_load)
+        log(f"served {serve['scenes']} scenes, {serve['value'] / 1e6:.1f} M body-steps/s")
+        if args.serve_sat_seconds > 0:
+            rate2, arr2 = served_arrivals(value, args.serve_sat_seconds, 3.0, args.seed + 1, args.steps)
+            sat = serve_stream(ctx, rate2, arr2, args.steps, 3.0)
+            serve["saturated"] = {k: sat[k] for k in ("value", "offered_load", "scenes", "wall_s", "p50_step_ms",
+                                                      "p99_step_ms", "p50_request_ms", "p99_request_ms")}
+            log(f"saturated: {sat['value'] / 1e6:.1f} M body-steps/s")
+        pv, sizes = plain_served(ctx, arrivals, args.steps, args.serve_plain_scenes)
+        serve["plain"] = {"value": pv, "unit": "body-steps/s",
+                          "sample": f"{len(sizes)} served scenes of {sizes} bodies, {args.steps} steps each, one "
+                                    f"world per launch"}
+        serve["served_vs_plain"] = serve["value"] / pv
+        if "saturated" in serve:
+            serve["saturated_vs_plain"] = serve["saturated"]["value"] / pv
 
     out = {
         "metric": "body-steps/sec batched vs plain on 1 MI355X; p50/p99 step latency",
@@ -217,9 +433,12 @@ def main():
         "p50_step_ms": float(np.percentile(lat_ms, 50)),
         "p99_step_ms": float(np.percentile(lat_ms, 99)),
         "plain": {"value": plain_value, "unit": "body-steps/s",
-                  "sample": f"first {args.plain_worlds} worlds, {args.plain_steps} steps each, one world per launch"},
+                  "sample": f"{args.plain_worlds} worlds spread over the batch, {plain_steps} steps each, "
+                            f"one world per launch"},
         "batched_vs_plain": value / plain_value,
+        "served": serve,
         "single_world_4096": single,
+        "accuracy": acc,
         "phase_ms_per_step": per_step,
         "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "dominant_phase": dom},

@@ -237,15 +237,19 @@ int r2d_make_bodies(r2d_world* w, size_t n, const r2d_body_options* bo, const in
 
 /* ------------------------------------------------------------- serving */
 /* No counterpart in the reference (it steps one world per call).  A server
- * owns one context's batched step: requests -- a built world and a number of
- * steps -- are admitted in arrival order while the sum of their footprints
- * (r2d_world_footprint) fits hbm_budget_bytes and fewer than max_worlds
- * run; each r2d_serve_tick collects the batch step launched by the previous
- * tick, retires finished requests, admits waiting ones and launches one step
- * of every running world as one batch (continuous batching).  A world's
- * state stays on the device from admission to completion; while a request
- * holds it, edits of the world return R2D_E_BUSY (reads are allowed).  Each
- * world's result is bit-identical to stepping it alone. */
+ * steps the worlds of a context's callers with continuous batching:
+ * requests -- a built world and a number of steps -- are admitted while the
+ * sum of their footprints (r2d_world_footprint) fits hbm_budget_bytes and
+ * fewer than max_worlds run.  Admitted worlds are stepped in lanes by size
+ * (split_bodies), each lane with its own stream and step scratch, so small
+ * worlds are not held to the step time of the largest; within a lane,
+ * requests are admitted in arrival order.  Each r2d_serve_tick collects the
+ * lanes whose batch step has finished, retires finished requests, admits
+ * waiting ones and launches one step of every running world of each idle
+ * lane as one batch.  A world's state stays on the device from admission to
+ * completion; while a request holds it, edits of the world return
+ * R2D_E_BUSY (reads are allowed).  Each world's result is bit-identical to
+ * stepping it alone. */
 typedef struct r2d_server r2d_server;
 
 enum { R2D_REQ_QUEUED = 0, R2D_REQ_RUNNING = 1, R2D_REQ_DONE = 2, R2D_REQ_FAILED = 3 };
@@ -256,6 +260,9 @@ typedef struct {
     float dt;                   /* Solver.process arguments of every step */
     uint32_t sub_steps;
     uint32_t collision_iters;
+    uint32_t split_bodies[2];   /* lanes by world size: n <= split[0], n <= split[1], larger;
+                                   {0, 0} = one lane.  Each lane has its own stream and scratch
+                                   and steps at its own pace. */
 } r2d_serve_options;
 
 typedef struct {
@@ -266,8 +273,8 @@ typedef struct {
     uint32_t launched_worlds;   /* worlds in the batch step this tick launched */
     uint64_t launched_bodies;
     size_t resident_bytes;
-    double step_gpu_ms;         /* device time of the collected batch step */
-    double step_wall_ms;        /* its launch-to-collection wall time */
+    double step_gpu_ms;         /* device time of the slowest batch step it collected */
+    double step_wall_ms;        /* launch-to-collection wall time of that step */
 } r2d_serve_tick_info;
 
 typedef struct {
@@ -276,6 +283,7 @@ typedef struct {
     uint32_t steps_done;
     uint32_t steps_requested;
     size_t footprint_bytes;
+    uint32_t lane;
     double wait_ms;             /* submit -> admission (so far, while queued) */
     double total_ms;            /* submit -> completion (so far, while not done) */
     double max_step_ms;         /* slowest device step it took part in */
@@ -291,6 +299,7 @@ typedef struct {
     double p50_step_ms, p99_step_ms;    /* device step time seen by each world-step */
     double p50_batch_ms, p99_batch_ms;  /* device time per batch step */
     size_t resident_bytes, peak_resident_bytes;
+    uint32_t lanes;
 } r2d_serve_summary;
 
 /* Bytes a world is charged at admission: its device state plus its share of
@@ -302,8 +311,9 @@ int r2d_serve_destroy(r2d_server* sv);
 /* Queues a request; R2D_E_CAPACITY if the world can never fit the budget,
  * R2D_E_INVALID_ID if a constraint names a removed body. */
 int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, uint64_t* ticket);
-/* Does not wait for the step it launches: host work between ticks overlaps
- * the device.  info may be NULL. */
+/* Waits until some lane's step has finished (if any is in flight), then
+ * admits and launches; it does not wait for the steps it launches, so host
+ * work between ticks overlaps the device.  info may be NULL. */
 int r2d_serve_tick(r2d_server* sv, r2d_serve_tick_info* info);
 /* Ticks until every request has finished. */
 int r2d_serve_drain(r2d_server* sv);

@@ -69,7 +69,8 @@ class ManifoldState(ctypes.Structure):
 
 class ServeOptions(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_worlds", ctypes.c_uint32), ("dt", ctypes.c_float),
-                ("sub_steps", ctypes.c_uint32), ("collision_iters", ctypes.c_uint32)]
+                ("sub_steps", ctypes.c_uint32), ("collision_iters", ctypes.c_uint32),
+                ("split_bodies", ctypes.c_uint32 * 2)]
 
 
 class ServeTickInfo(ctypes.Structure):
@@ -81,7 +82,7 @@ class ServeTickInfo(ctypes.Structure):
 
 class RequestStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("steps_done", ctypes.c_uint32),
-                ("steps_requested", ctypes.c_uint32), ("footprint_bytes", ctypes.c_size_t),
+                ("steps_requested", ctypes.c_uint32), ("footprint_bytes", ctypes.c_size_t), ("lane", ctypes.c_uint32),
                 ("wait_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("max_step_ms", ctypes.c_double)]
 
 
@@ -90,7 +91,8 @@ class ServeSummary(ctypes.Structure):
                                                "batch_steps", "scene_steps", "body_steps")] + \
                [(n, ctypes.c_double) for n in ("gpu_ms", "wall_ms", "p50_step_ms", "p99_step_ms", "p50_batch_ms",
                                                "p99_batch_ms")] + \
-               [("resident_bytes", ctypes.c_size_t), ("peak_resident_bytes", ctypes.c_size_t)]
+               [("resident_bytes", ctypes.c_size_t), ("peak_resident_bytes", ctypes.c_size_t),
+                ("lanes", ctypes.c_uint32)]
 
 
 P = ctypes.c_void_p

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/r2d.h"
@@ -26,6 +28,7 @@ using namespace r2d;
 #define R2D_EXPORT extern "C" __attribute__((visibility("default")))
 
 static thread_local char g_last_error[512] = "";
+static std::atomic<uint64_t> g_epoch{0};
 
 static void note_error(const char* what, hipError_t e, int line)
 {
@@ -125,12 +128,12 @@ struct r2d_ctx {
     size_t hstatus_cap = 0;
     // profiling
     bool profiling = false;
-    KernelTimer timer{};
+    std::vector<hipEvent_t> ev_pool;              // phase events of timed steps not yet summed
+    size_t ev_used = 0;
     double phase_ms[4] = {0, 0, 0, 0};
     uint64_t phase_steps = 0;
     uint64_t epoch = 0;
     uint64_t last_counts[4] = {0, 0, 0, 0};
-    std::vector<r2d_world*> last_batch;
     // captured step: replayed while the batch layout and launch settings stay the same
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;          // whole step
@@ -139,6 +142,7 @@ struct r2d_ctx {
     uint64_t graph_uses = 0;                      // launches with the current key
     std::vector<WorldDev> uploaded_descs;
     int solve_variant = -1;                       // -1: choose by world size
+    uint32_t lds_nmax = 0;                        // 0: LDS body state up to the LDS capacity
 
     Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}}; }
 };
@@ -478,16 +482,32 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
                             (uint32_t)b.J + est_m);
     if (rc) return rc;
     // solve variant: context override (autotuner), environment, then the
-    // default: the register-slot kernel for small worlds beside a record
-    // kernel sized for the largest world of the batch
-    uint32_t maxn = sizes.empty() ? 0u : *std::max_element(sizes.begin(), sizes.end());
-    int variant = maxn <= 600 ? 1 : 7;
+    // default (register-slot kernel for small worlds, 256-thread record
+    // kernels for the others)
+    int variant = 1;
     if (c->solve_variant >= 0) variant = c->solve_variant;
     if (const char* e = std::getenv("R2D_SOLVE_VARIANT")) {
         int v = std::atoi(e);
         if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
-    b.plan = plan_solve(variant, sizes.data(), nw);
+    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax);
+    return R2D_OK;
+}
+
+// sum the phase times of the timed steps recorded so far
+int timer_flush(r2d_ctx* c)
+{
+    if (!c->ev_used) return R2D_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k + KT_N <= c->ev_used; k += KT_N) {
+        for (int p = 0; p < 4; ++p) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[k + p], c->ev_pool[k + p + 1]));
+            c->phase_ms[p] += ms;
+        }
+        c->phase_steps += 1;
+    }
+    c->ev_used = 0;
     return R2D_OK;
 }
 
@@ -505,7 +525,7 @@ int batch_launch(r2d_ctx* c, Batch& b)
     const Streams ss = c->streams();
     const uint32_t sub = (uint32_t)b.sub, iters = (uint32_t)b.iters;
     auto eager_phase = [&](int ph) { launch_phase(S, ss, ph, b.dt, sub, iters, b.plan); };
-    KernelTimer* tm = c->profiling ? &c->timer : nullptr;
+    const bool tm = c->profiling;
     hipGraphExec_t* want = nullptr;   // graphs this launch replays
     if (c->use_graph) {
         // key: everything the captured launches bake in (the world descriptors
@@ -538,20 +558,28 @@ int batch_launch(r2d_ctx* c, Batch& b)
             else for (int ph = 0; ph < PH_N; ++ph) eager_phase(ph);
             continue;
         }
-        // timed step: events on the stream between the phases
+        // timed step: events on the stream between the phases, summed later
+        // (timer_flush), so timing adds no synchronisation
+        if (c->ev_used + KT_N > c->ev_pool.size()) {
+            if (c->ev_pool.size() >= 4000) {
+                int rc = timer_flush(c);
+                if (rc) return rc;
+            } else {
+                for (int i = 0; i < 64 * KT_N; ++i) {
+                    hipEvent_t e;
+                    HIPCHK(hipEventCreate(&e));
+                    c->ev_pool.push_back(e);
+                }
+            }
+        }
+        hipEvent_t* ev = c->ev_pool.data() + c->ev_used;
         for (int ph = 0; ph < PH_N; ++ph) {
-            tm->mark(ph, c->stream);
+            HIPCHK(hipEventRecord(ev[ph], c->stream));
             if (want) HIPCHK(hipGraphLaunch(want[ph], c->stream));
             else eager_phase(ph);
         }
-        tm->mark(KT_END, c->stream);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        for (int p = 0; p < 4; ++p) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, c->timer.ev[p], c->timer.ev[p + 1]));
-            c->phase_ms[p] += ms;
-        }
-        c->phase_steps += 1;
+        HIPCHK(hipEventRecord(ev[KT_END], c->stream));
+        c->ev_used += KT_N;
     }
     HIPCHK(hipMemcpyAsync(c->hcounters, S.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(c->hstatus, S.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
@@ -594,8 +622,7 @@ int batch_wait(r2d_ctx* c, Batch& b)
         int rc = batch_launch(c, b);
         if (rc) return rc;
     }
-    c->epoch += 1;
-    c->last_batch = b.worlds;
+    c->epoch = g_epoch.fetch_add(1) + 1;   // unique over contexts: a world's epoch names one batch
     for (size_t i = 0; i < nw; ++i) {
         b.worlds[i]->host_valid = false;
         b.worlds[i]->epoch = c->epoch;
@@ -618,11 +645,6 @@ int step_batch(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t
     if (nw == 0 || steps == 0) return R2D_OK;
     for (size_t i = 0; i < nw; ++i)
         if (worlds[i] && worlds[i]->in_service) return R2D_E_BUSY;
-    // the step scratch is the context's: finish the server's launched batch first
-    if (c->server) {
-        int rc = server_collect(c->server);
-        if (rc) return rc;
-    }
     Batch b;
     int rc = batch_prepare(c, worlds, nw, dt, sub_steps, iters, steps, b);
     if (rc) return rc;
@@ -677,15 +699,20 @@ double now_ms()
 
 }  // namespace
 
-// A continuous-batching server over one context: requests (a world and a
-// number of steps) wait in FIFO order until the sum of admitted footprints
-// fits the HBM budget and a batch slot is free; every tick collects the
-// batch step launched by the previous tick, retires finished requests,
-// admits waiting ones, and launches one step of every admitted world as one
-// batch.  Worlds stay resident on the device from admission to completion.
+// A continuous-batching server: requests (a world and a number of steps) are
+// admitted while the sum of admitted footprints fits the HBM budget and a
+// batch slot is free.  Admitted worlds are stepped in lanes by size class,
+// each lane with its own stream and step scratch (an internal context), so a
+// lane of small worlds is not held to the step time of the largest world:
+// every tick collects the lanes whose batch step has finished, retires
+// finished requests, admits waiting ones (arrival order within a lane), and
+// launches one step of every running world of each idle lane as one batch.
+// Worlds stay resident on the device from admission to completion.
 struct r2d_server {
     struct Req {
-        r2d_world* w = nullptr;
+        r2d_world* w = nullptr;   // held while queued or running
+        uint32_t n_bodies = 0;
+        uint32_t lane = 0;
         uint32_t steps = 0, done = 0;
         size_t bytes = 0;
         int32_t state = R2D_REQ_QUEUED;
@@ -693,54 +720,71 @@ struct r2d_server {
         double t_submit = 0, t_admit = 0, t_done = 0;
         double max_step_ms = 0;
     };
-    r2d_ctx* ctx = nullptr;
+    struct Lane {
+        r2d_ctx* ctx = nullptr;               // own stream and step scratch
+        std::deque<uint64_t> queue;           // waiting for admission, in arrival order
+        std::vector<uint64_t> running;        // admitted, in order of admission
+        bool inflight = false;
+        Batch batch;                          // the launched batch step
+        std::vector<uint64_t> batch_tickets;
+        double t_launch = 0;
+    };
+    r2d_ctx* home = nullptr;
     r2d_serve_options opt{};
-    std::vector<Req> reqs;                  // ticket t is reqs[t - 1]
-    std::deque<uint64_t> queue;             // waiting for admission
-    std::vector<uint64_t> running;          // admitted, in order of admission
+    uint32_t nlanes = 1;
+    Lane lanes[3];
+    std::vector<Req> reqs;                    // ticket t is reqs[t - 1]
     size_t resident = 0, peak_resident = 0;
-    bool inflight = false;
-    Batch batch;                            // the launched batch step
-    std::vector<uint64_t> batch_tickets;
-    double t_launch = 0;
+    size_t nrunning = 0;
     uint64_t rejected = 0, completed = 0, batch_steps = 0, scene_steps = 0, body_steps = 0;
     double gpu_ms = 0, wall_ms = 0;
-    std::vector<float> step_ms;             // device time of each batch step
-    std::vector<uint32_t> step_worlds;      // scenes in it
+    std::vector<float> step_ms;               // device time of each batch step
+    std::vector<uint32_t> step_worlds;        // scenes in it
     uint32_t last_completed = 0;
+    double last_gpu_ms = 0, last_wall_ms = 0;
 };
 
 namespace {
 
+uint32_t lane_of(const r2d_server* sv, uint32_t n)
+{
+    for (uint32_t l = 0; l + 1 < sv->nlanes; ++l)
+        if (n <= sv->opt.split_bodies[l]) return l;
+    return sv->nlanes - 1;
+}
+
 void retire(r2d_server* sv, uint64_t t, int32_t state, int32_t status)
 {
     r2d_server::Req& q = sv->reqs[t - 1];
+    if (q.state == R2D_REQ_RUNNING) sv->nrunning -= 1;
     q.state = state;
     q.status = status;
     q.t_done = now_ms();
+    q.n_bodies = (uint32_t)q.w->n;
     q.w->in_service = false;
+    q.w->ctx = sv->home;   // back to the caller's context
+    q.w = nullptr;         // released: the caller may destroy the world
     sv->resident -= q.bytes;
     sv->completed += 1;
     sv->last_completed += 1;
 }
 
-// wait for the launched batch step and account for it
-int server_collect(r2d_server* sv)
+// wait for a lane's launched batch step and account for it
+int lane_collect(r2d_server* sv, r2d_server::Lane& L)
 {
-    if (!sv->inflight) return R2D_OK;
-    sv->inflight = false;
-    r2d_ctx* c = sv->ctx;
-    int rc = batch_wait(c, sv->batch);
-    double wall = now_ms() - sv->t_launch;
-    float gms = batch_gpu_ms(c);
-    const size_t nw = sv->batch_tickets.size();
+    if (!L.inflight) return R2D_OK;
+    L.inflight = false;
+    int rc = batch_wait(L.ctx, L.batch);
+    double wall = now_ms() - L.t_launch;
+    float gms = batch_gpu_ms(L.ctx);
+    const size_t nw = L.batch_tickets.size();
     // per-world failures arrive as world statuses; anything else fails the batch
-    const bool per_world = rc == R2D_OK || sv->batch.world_status.size() == nw;
+    const bool per_world = rc == R2D_OK || L.batch.world_status.size() == nw;
     for (size_t i = 0; i < nw; ++i) {
-        uint64_t t = sv->batch_tickets[i];
+        uint64_t t = L.batch_tickets[i];
         r2d_server::Req& q = sv->reqs[t - 1];
         if (!per_world) { retire(sv, t, R2D_REQ_FAILED, rc); continue; }
-        int32_t ws = sv->batch.world_status[i];
+        int32_t ws = L.batch.world_status[i];
         if (ws != 0) {
             retire(sv, t, R2D_REQ_FAILED, ws == 4 ? R2D_E_NONFINITE : ws == 5 ? R2D_E_DEVICE : R2D_E_RANGE);
             continue;
@@ -751,54 +795,71 @@ int server_collect(r2d_server* sv)
         if (q.done >= q.steps) retire(sv, t, R2D_REQ_DONE, R2D_OK);
     }
     std::vector<uint64_t> still;
-    for (uint64_t t : sv->running)
+    for (uint64_t t : L.running)
         if (sv->reqs[t - 1].state == R2D_REQ_RUNNING) still.push_back(t);
-    sv->running.swap(still);
+    L.running.swap(still);
     sv->batch_steps += 1;
     sv->scene_steps += nw;
     sv->gpu_ms += gms;
     sv->wall_ms += wall;
     sv->step_ms.push_back(gms);
     sv->step_worlds.push_back((uint32_t)nw);
+    sv->last_gpu_ms = std::max(sv->last_gpu_ms, (double)gms);
+    sv->last_wall_ms = std::max(sv->last_wall_ms, wall);
     return per_world ? R2D_OK : rc;
+}
+
+int server_collect(r2d_server* sv)
+{
+    int first = R2D_OK;
+    for (uint32_t l = 0; l < sv->nlanes; ++l) {
+        int rc = lane_collect(sv, sv->lanes[l]);
+        if (rc && first == R2D_OK) first = rc;
+    }
+    return first;
 }
 
 uint32_t server_admit(r2d_server* sv)
 {
     uint32_t n = 0;
-    while (!sv->queue.empty() && sv->running.size() < sv->opt.max_worlds) {
-        uint64_t t = sv->queue.front();
-        r2d_server::Req& q = sv->reqs[t - 1];
-        if (sv->resident + q.bytes > sv->opt.hbm_budget_bytes) break;   // FIFO: no overtaking
-        sv->queue.pop_front();
-        q.state = R2D_REQ_RUNNING;
-        q.t_admit = now_ms();
-        sv->resident += q.bytes;
-        sv->peak_resident = std::max(sv->peak_resident, sv->resident);
-        sv->running.push_back(t);
-        n += 1;
+    for (uint32_t l = 0; l < sv->nlanes; ++l) {
+        r2d_server::Lane& L = sv->lanes[l];
+        while (!L.queue.empty() && sv->nrunning < sv->opt.max_worlds) {
+            uint64_t t = L.queue.front();
+            r2d_server::Req& q = sv->reqs[t - 1];
+            if (sv->resident + q.bytes > sv->opt.hbm_budget_bytes) break;   // no overtaking within a lane
+            L.queue.pop_front();
+            q.state = R2D_REQ_RUNNING;
+            q.t_admit = now_ms();
+            q.w->ctx = L.ctx;     // stepped (and read back) on the lane's stream until released
+            sv->resident += q.bytes;
+            sv->peak_resident = std::max(sv->peak_resident, sv->resident);
+            sv->nrunning += 1;
+            L.running.push_back(t);
+            n += 1;
+        }
     }
     return n;
 }
 
-int server_launch(r2d_server* sv)
+int lane_launch(r2d_server* sv, r2d_server::Lane& L)
 {
-    if (sv->running.empty()) return R2D_OK;
+    if (L.running.empty() || L.inflight) return R2D_OK;
     std::vector<r2d_world*> ws;
-    ws.reserve(sv->running.size());
-    for (uint64_t t : sv->running) ws.push_back(sv->reqs[t - 1].w);
-    r2d_ctx* c = sv->ctx;
-    int rc = batch_prepare(c, ws.data(), ws.size(), sv->opt.dt, sv->opt.sub_steps, sv->opt.collision_iters, 1,
-                           sv->batch);
-    if (!rc) rc = batch_launch(c, sv->batch);
+    ws.reserve(L.running.size());
+    for (uint64_t t : L.running) ws.push_back(sv->reqs[t - 1].w);
+    L.ctx->solve_variant = sv->home->solve_variant;
+    int rc = batch_prepare(L.ctx, ws.data(), ws.size(), sv->opt.dt, sv->opt.sub_steps, sv->opt.collision_iters, 1,
+                           L.batch);
+    if (!rc) rc = batch_launch(L.ctx, L.batch);
     if (rc) {
-        for (uint64_t t : sv->running) retire(sv, t, R2D_REQ_FAILED, rc);
-        sv->running.clear();
+        for (uint64_t t : L.running) retire(sv, t, R2D_REQ_FAILED, rc);
+        L.running.clear();
         return rc;
     }
-    sv->batch_tickets = sv->running;
-    sv->t_launch = now_ms();
-    sv->inflight = true;
+    L.batch_tickets = L.running;
+    L.t_launch = now_ms();
+    L.inflight = true;
     return R2D_OK;
 }
 
@@ -839,9 +900,9 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     HIPCHK(hipEventCreate(&c->ev_begin));
     HIPCHK(hipEventCreate(&c->ev_end));
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
-    for (int i = 0; i < KT_N; ++i) HIPCHK(hipEventCreate(&c->timer.ev[i]));
     set_solve_smem_limit();
     if (const char* e = std::getenv("R2D_NO_GRAPH")) c->use_graph = !(e[0] == '1');
+    if (const char* e = std::getenv("R2D_LDS_NMAX")) c->lds_nmax = (uint32_t)std::atoi(e);
     *out = c;
     return R2D_OK;
 }
@@ -855,7 +916,7 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     if (c->dworlds) (void)hipFree(c->dworlds);
     if (c->hcounters) (void)hipHostFree(c->hcounters);
     if (c->hstatus) (void)hipHostFree(c->hstatus);
-    for (int i = 0; i < KT_N; ++i) (void)hipEventDestroy(c->timer.ev[i]);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c->fork);
     for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c->join[i]);
     (void)hipEventDestroy(c->ev_begin);
@@ -1154,6 +1215,11 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
 {
     if (!w) return R2D_E_INVALID_ARG;
     r2d_ctx* c = w->ctx;
+    // a launched server step is rewriting the step scratch: finish it first
+    if (c->server) {
+        int rc = server_collect(c->server);
+        if (rc) return rc;
+    }
     if (w->epoch == 0 || w->epoch != c->epoch) {
         if (n) *n = 0;
         return w->epoch == 0 ? R2D_OK : R2D_E_RANGE;
@@ -1231,6 +1297,8 @@ R2D_EXPORT int r2d_ctx_set_profiling(r2d_ctx* c, int on)
 R2D_EXPORT int r2d_ctx_phase_times(r2d_ctx* c, double* out_ms, uint64_t* steps)
 {
     if (!c) return R2D_E_INVALID_ARG;
+    int rc = timer_flush(c);
+    if (rc) return rc;
     if (out_ms) for (int i = 0; i < 4; ++i) out_ms[i] = c->phase_ms[i];
     if (steps) *steps = c->phase_steps;
     return R2D_OK;
@@ -1239,6 +1307,8 @@ R2D_EXPORT int r2d_ctx_phase_times(r2d_ctx* c, double* out_ms, uint64_t* steps)
 R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
 {
     if (!c) return R2D_E_INVALID_ARG;
+    int rc = timer_flush(c);
+    if (rc) return rc;
     for (int i = 0; i < 4; ++i) c->phase_ms[i] = 0;
     c->phase_steps = 0;
     return R2D_OK;
@@ -1330,8 +1400,20 @@ R2D_EXPORT int r2d_serve_create(r2d_ctx* c, const r2d_serve_options* opt, r2d_se
     if (c->server) return R2D_E_BUSY;
     r2d_server* sv = new (std::nothrow) r2d_server();
     if (!sv) return R2D_E_OOM;
-    sv->ctx = c;
+    sv->home = c;
     sv->opt = *opt;
+    // lanes: one, plus one per increasing split
+    sv->nlanes = 1;
+    for (int l = 0; l < 2 && opt->split_bodies[l] > (l ? opt->split_bodies[l - 1] : 0u); ++l) sv->nlanes += 1;
+    for (uint32_t l = 0; l < sv->nlanes; ++l) {
+        int rc = r2d_ctx_create(c->device, &sv->lanes[l].ctx);
+        if (rc) {
+            for (uint32_t k = 0; k < l; ++k) r2d_ctx_destroy(sv->lanes[k].ctx);
+            delete sv;
+            return rc;
+        }
+        sv->lanes[l].ctx->server = sv;
+    }
     c->server = sv;
     *out = sv;
     return R2D_OK;
@@ -1341,16 +1423,24 @@ R2D_EXPORT int r2d_serve_destroy(r2d_server* sv)
 {
     if (!sv) return R2D_E_INVALID_ARG;
     int rc = server_collect(sv);
-    for (auto& q : sv->reqs)
-        if (q.w) q.w->in_service = false;
-    sv->ctx->server = nullptr;
+    for (auto& q : sv->reqs)   // unfinished requests release their worlds
+        if (q.w) {
+            q.w->in_service = false;
+            q.w->ctx = sv->home;
+            q.w->epoch = 0;
+        }
+    for (uint32_t l = 0; l < sv->nlanes; ++l) {
+        sv->lanes[l].ctx->server = nullptr;
+        r2d_ctx_destroy(sv->lanes[l].ctx);
+    }
+    sv->home->server = nullptr;
     delete sv;
     return rc;
 }
 
 R2D_EXPORT int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, uint64_t* ticket)
 {
-    if (!sv || !w || !ticket || w->ctx != sv->ctx) return R2D_E_INVALID_ARG;
+    if (!sv || !w || !ticket || w->ctx != sv->home) return R2D_E_INVALID_ARG;
     if (w->in_service) return R2D_E_BUSY;
     // the reference raises InvalidRigidBodyId from the first constraint solve
     if (sv->opt.sub_steps > 0 && sv->opt.collision_iters > 0)
@@ -1362,6 +1452,7 @@ R2D_EXPORT int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, ui
     q.w = w;
     q.steps = steps;
     q.bytes = bytes;
+    q.lane = lane_of(sv, (uint32_t)w->n);
     q.t_submit = now_ms();
     sv->reqs.push_back(q);
     *ticket = sv->reqs.size();
@@ -1371,7 +1462,7 @@ R2D_EXPORT int r2d_serve_submit(r2d_server* sv, r2d_world* w, uint32_t steps, ui
         retire(sv, *ticket, R2D_REQ_DONE, R2D_OK);
         return R2D_OK;
     }
-    sv->queue.push_back(*ticket);
+    sv->lanes[q.lane].queue.push_back(*ticket);
     return R2D_OK;
 }
 
@@ -1379,33 +1470,59 @@ R2D_EXPORT int r2d_serve_tick(r2d_server* sv, r2d_serve_tick_info* info)
 {
     if (!sv) return R2D_E_INVALID_ARG;
     sv->last_completed = 0;
-    double gpu = 0, wall = 0;
-    const bool had = sv->inflight;
-    const double t_launch = sv->t_launch;
-    int rc = server_collect(sv);
-    if (had && !sv->step_ms.empty()) { gpu = sv->step_ms.back(); wall = now_ms() - t_launch; }
+    sv->last_gpu_ms = sv->last_wall_ms = 0;
+    int first = R2D_OK;
+    // collect the lanes whose step has finished; with none finished, wait for the first
+    bool any = false, waiting = false;
+    for (;;) {
+        for (uint32_t l = 0; l < sv->nlanes; ++l) {
+            r2d_server::Lane& L = sv->lanes[l];
+            if (!L.inflight) continue;
+            waiting = true;
+            if (hipEventQuery(L.ctx->ev_end) == hipErrorNotReady) continue;
+            int rc = lane_collect(sv, L);
+            if (rc && first == R2D_OK) first = rc;
+            any = true;
+        }
+        if (any || !waiting) break;
+        std::this_thread::yield();
+    }
     uint32_t admitted = server_admit(sv);
-    int rc2 = server_launch(sv);
+    uint32_t launched = 0;
+    uint64_t bodies = 0;
+    for (uint32_t l = 0; l < sv->nlanes; ++l) {
+        r2d_server::Lane& L = sv->lanes[l];
+        if (L.inflight) continue;
+        int rc = lane_launch(sv, L);
+        if (rc && first == R2D_OK) first = rc;
+        if (L.inflight) { launched += (uint32_t)L.batch_tickets.size(); bodies += L.batch.N; }
+    }
     if (info) {
         std::memset(info, 0, sizeof *info);
         info->admitted = admitted;
         info->completed = sv->last_completed;
-        info->queued = (uint32_t)sv->queue.size();
-        info->running = (uint32_t)sv->running.size();
-        info->launched_worlds = sv->inflight ? (uint32_t)sv->batch_tickets.size() : 0u;
-        info->launched_bodies = sv->inflight ? sv->batch.N : 0u;
+        size_t queued = 0;
+        for (uint32_t l = 0; l < sv->nlanes; ++l) queued += sv->lanes[l].queue.size();
+        info->queued = (uint32_t)queued;
+        info->running = (uint32_t)sv->nrunning;
+        info->launched_worlds = launched;
+        info->launched_bodies = bodies;
         info->resident_bytes = sv->resident;
-        info->step_gpu_ms = gpu;
-        info->step_wall_ms = wall;
+        info->step_gpu_ms = sv->last_gpu_ms;
+        info->step_wall_ms = sv->last_wall_ms;
     }
-    return rc ? rc : rc2;
+    return first;
 }
 
 R2D_EXPORT int r2d_serve_drain(r2d_server* sv)
 {
     if (!sv) return R2D_E_INVALID_ARG;
     int first = R2D_OK;
-    while (sv->inflight || !sv->running.empty() || !sv->queue.empty()) {
+    for (;;) {
+        bool busy = sv->nrunning > 0;
+        for (uint32_t l = 0; l < sv->nlanes; ++l)
+            busy = busy || sv->lanes[l].inflight || !sv->lanes[l].queue.empty();
+        if (!busy) break;
         int rc = r2d_serve_tick(sv, nullptr);
         if (rc && first == R2D_OK) first = rc;
     }
@@ -1422,6 +1539,7 @@ R2D_EXPORT int r2d_serve_poll(r2d_server* sv, uint64_t ticket, r2d_request_statu
     out->steps_done = q.done;
     out->steps_requested = q.steps;
     out->footprint_bytes = q.bytes;
+    out->lane = q.lane;
     double now = now_ms();
     out->wait_ms = (q.state == R2D_REQ_QUEUED ? now : q.t_admit) - q.t_submit;
     out->total_ms = (q.state >= R2D_REQ_DONE ? q.t_done : now) - q.t_submit;
@@ -1436,8 +1554,10 @@ R2D_EXPORT int r2d_serve_stats(r2d_server* sv, r2d_serve_summary* out)
     out->submitted = sv->reqs.size();
     out->completed = sv->completed;
     out->rejected = sv->rejected;
-    out->queued = sv->queue.size();
-    out->running = sv->running.size();
+    size_t queued = 0;
+    for (uint32_t l = 0; l < sv->nlanes; ++l) queued += sv->lanes[l].queue.size();
+    out->queued = queued;
+    out->running = sv->nrunning;
     out->batch_steps = sv->batch_steps;
     out->scene_steps = sv->scene_steps;
     out->body_steps = sv->body_steps;
@@ -1450,5 +1570,6 @@ R2D_EXPORT int r2d_serve_stats(r2d_server* sv, r2d_serve_summary* out)
     out->p99_batch_ms = weighted_pct(sv->step_ms, ones, 0.99);
     out->resident_bytes = sv->resident;
     out->peak_resident_bytes = sv->peak_resident;
+    out->lanes = sv->nlanes;
     return R2D_OK;
 }

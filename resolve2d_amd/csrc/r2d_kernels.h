@@ -10,10 +10,6 @@ namespace r2d {
 // phase markers recorded by the optional per-step timer
 enum { KT_BROAD_BEGIN = 0, KT_NARROW_BEGIN, KT_SCHED_BEGIN, KT_SOLVE_BEGIN, KT_END, KT_N };
 
-struct KernelTimer {
-    hipEvent_t ev[KT_N];
-    void mark(int id, hipStream_t st) { (void)hipEventRecord(ev[id], st); }
-};
 
 // bytes of LDS the solve kernel needs for a world of n bodies
 size_t solve_lds_bytes(int nbodies);
@@ -40,7 +36,9 @@ struct SolvePlan {
     uint32_t lds_maxn = 0;
     uint32_t any_slot = 0, any_lds = 0, any_big = 0;
 };
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw);
+// lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
+// many as LDS holds); larger worlds use the global-memory record kernel
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit);
 
 // the context's main stream plus two side streams for the solve kernels
 struct Streams {

@@ -1792,24 +1792,29 @@ struct SolveVariant {
     int slots;
     RecFn rec_fn;        // record kernel (LDS body state) for the worlds the slot kernel leaves
     unsigned threads;
+    RecFn big_fn;        // record kernel with body state in global memory (worlds beyond LDS)
+    unsigned big_threads;
 };
 // solve variants: the register-slot kernel's slots x workgroup size, and the
 // record kernel's workgroup size for worlds with more items than slots
+// (fewer waves per world keep a dependency chain's wave from sharing its
+// SIMD with the world's other waves: 256 threads beat 1,024 by about 3x on a
+// 4,096-body world)
 static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
-    {nullptr, 0, 0, k_solve<true, 1024>, 1024},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256},
-    {nullptr, 0, 0, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 1024>, 1024},
+    {nullptr, 0, 0, k_solve<true, 1024>, 1024, k_solve<false, 1024>, 1024},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {nullptr, 0, 0, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 128, 2>, 128, k_solve<false, 128, 2>, 128},
 };
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
 int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
 
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw)
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit)
 {
     SolvePlan p;
     p.variant = (variant >= 0 && variant < SOLVE_VARIANTS) ? variant : 0;
@@ -1817,6 +1822,7 @@ SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw)
     p.slot_cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
     p.slot_nmax = v.slot_fn ? SLOT_NMAX : 0u;
     p.lds_nmax = (uint32_t)((160u * 1024u - 1024u) / solve_lds_bytes(1));
+    if (lds_limit) p.lds_nmax = std::min(p.lds_nmax, lds_limit);
     for (size_t i = 0; i < nw; ++i) {
         uint32_t n = nbodies[i];
         if (p.slot_cap && n <= p.slot_nmax) { p.any_slot = 1; p.slot_maxn = std::max(p.slot_maxn, n); }
@@ -1880,8 +1886,8 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             if (run_lds) (void)hipStreamWaitEvent(ss.aux[0], ss.fork, 0);
         }
         if (run_big)
-            hipLaunchKernelGGL((k_solve<false, 1024>), dim3(S.nworlds), dim3(1024), 0, ss.aux[1], S, dt, sub_steps,
-                               iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
+            hipLaunchKernelGGL(v.big_fn, dim3(S.nworlds), dim3(v.big_threads), 0, ss.aux[1], S, dt, sub_steps, iters,
+                               sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
         if (run_lds)
             hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), solve_lds_bytes((int)sp.lds_maxn),
                                ss.aux[0], S, dt, sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
@@ -1909,6 +1915,8 @@ void set_solve_smem_limit()
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)kSolveVariants[v].rec_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)kSolveVariants[v].big_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
     }
 }

@@ -148,18 +148,18 @@ class GpuWorld:
         N.check(self.lib.r2d_get_manifolds(self.handle, None, 0, ctypes.byref(cnt)), "r2d_get_manifolds")
         arr = (N.ManifoldState * max(cnt.value, 1))()
         N.check(self.lib.r2d_get_manifolds(self.handle, arr, cnt.value, ctypes.byref(cnt)), "r2d_get_manifolds")
-        out = np.zeros((cnt.value, MANIFOLD_COLS))
-        for i in range(cnt.value):
-            m = arr[i]
-            row = [m.ref_id, m.inc_id, m.normal_x, m.normal_y, m.tangent_x, m.tangent_y, m.friction]
-            for p in range(2):
-                q = m.points[p]
-                if q.valid:
-                    row += [1, q.acc_pn, q.acc_pt, q.ref_r_x, q.ref_r_y, q.inc_r_x, q.inc_r_y, q.pos_x, q.pos_y,
-                            q.depth, q.original_depth, q.mass_n, q.mass_t]
-                else:
-                    row += [0] * 13
-            out[i] = row
+        m = cnt.value
+        words = ctypes.sizeof(N.ManifoldState) // 4            # 2 ids, 5 floats, 2 x (valid + 12 floats)
+        raw = np.frombuffer(arr, dtype=np.uint32, count=m * words).reshape(m, words)
+        out = np.zeros((m, MANIFOLD_COLS))
+        out[:, 0:2] = raw[:, 0:2]
+        out[:, 2:7] = raw[:, 2:7].view(np.float32)
+        for p in range(2):
+            base = 7 + 13 * p
+            valid = raw[:, base] != 0
+            out[:, base] = valid
+            vals = raw[:, base + 1:base + 13].view(np.float32)
+            out[:, base + 1:base + 13] = np.where(valid[:, None], vals, 0.0)
         return out
 
 

@@ -2,10 +2,10 @@
 
 The reference steps one world per `Solver.process` call (src/core/lib.zig:189);
 serving has no counterpart there.  The server is native (r2d_api.hip): it
-admits requests in arrival order while their footprints fit an HBM budget,
-keeps admitted worlds resident on the device, and steps every running world
-together, one batched launch per tick.  `run_stream` drives it with timed
-arrivals the way a request loop would.
+admits requests while their footprints fit an HBM budget, keeps admitted
+worlds resident on the device, and steps them in lanes by world size -- each
+lane one batched launch per step, at its own pace, on its own stream.
+`run_stream` drives it with timed arrivals the way a request loop would.
 """
 import ctypes
 import time
@@ -18,12 +18,15 @@ from .scenes import DT, ITERS, SUB_STEPS
 
 class Server:
     def __init__(self, ctx, hbm_budget_bytes: int, max_worlds: int, dt: float = DT, sub_steps: int = SUB_STEPS,
-                 collision_iters: int = ITERS):
+                 collision_iters: int = ITERS, split_bodies=(1024, 4096)):
+        """split_bodies: lanes by world size (n <= split[0], n <= split[1],
+        larger); (0, 0) steps every world in one lane."""
         self.ctx = ctx
         self.lib = ctx.lib
         o = N.ServeOptions()
         o.hbm_budget_bytes, o.max_worlds, o.dt = hbm_budget_bytes, max_worlds, dt
         o.sub_steps, o.collision_iters = sub_steps, collision_iters
+        o.split_bodies[0], o.split_bodies[1] = split_bodies
         h = ctypes.c_void_p()
         N.check(self.lib.r2d_serve_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)), "r2d_serve_create")
         self.handle = h

@@ -80,9 +80,13 @@ def test_served_results_equal_alone(gpu_ctx):
         assert st["scene_steps"] == STEPS * len(mix)
         assert st["body_steps"] == STEPS * sum(w.bodies().shape[0] for _, w in mix)
         assert st["p99_step_ms"] >= st["p50_step_ms"] > 0
+        assert st["lanes"] == 3
+        lanes = set()
         for t in tickets:
             p = sv.poll(t)
             assert p["state"] == 2 and p["status"] == 0 and p["steps_done"] == STEPS, p
+            lanes.add(p["lane"])
+        assert lanes == {0, 1, 2}
     finally:
         sv.close()
     for i, (sp, w) in enumerate(mix):
@@ -142,3 +146,22 @@ def test_global_memory_route_matches_oracle(gpu_ctx, oracle):
     assert bits_equal(a, b), first_diff(a, b)
     assert bits_equal(g.manifolds(), o.manifolds())
     g.close()
+
+
+def test_finished_worlds_may_be_destroyed_before_the_server(gpu_ctx):
+    """A finished request releases its world: the caller may free it while the
+    server lives on, and closing the server touches only unfinished ones."""
+    from resolve2d_amd.serve import Server
+    sv = Server(gpu_ctx, 1 << 30, max_worlds=8)
+    ws = [_served_world(gpu_ctx, 60 + i, 30 + 10 * i) for i in range(4)]
+    t = [sv.submit(w, 2) for w in ws[:3]]
+    sv.drain()
+    for w in ws[:3]:
+        w.close()
+    late = sv.submit(ws[3], 50)
+    sv.tick()
+    sv.close()                      # releases the unfinished request's world
+    assert sv.handle is None
+    assert ws[3].process(1.0 / 60.0, 4, 4) == 0
+    assert all(x > 0 for x in t + [late])
+    ws[3].close()
