@@ -182,6 +182,46 @@ def pyramid_energy(ctx, steps):
     return out
 
 
+def large_world(ctx, steps, snaps, with_f64):
+    """The large constrained world (65,536 bodies, ropes, motors, walls): GPU
+    step time over the horizon and the error against an f64 run of the same
+    method at the snapshot steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    ops = scenes.large_constrained()
+    g = GpuWorld(ctx)
+    apply_scene(ops, g)
+    n = g.bodies().shape[0]
+    ctx.sync()
+    shots, lat, t = {}, [], 0.0
+    for s in range(1, steps + 1):
+        a = time.perf_counter()
+        checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "large-world step")
+        lat.append(time.perf_counter() - a)
+        if s in snaps:
+            shots[s] = g.bodies()
+    man = g.manifolds()
+    g.close()
+    out = {"bodies": n, "steps": steps, "ms_per_step": float(np.mean(lat) * 1e3),
+           "value": n * steps / float(np.sum(lat)), "unit": "body-steps/s",
+           "p50_step_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
+           "p99_step_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
+           "max_penetration_m": max_penetration(man)}
+    if with_f64:
+        o = pyoracle.OracleWorld("f64")
+        apply_scene(ops, o)
+        errs = {}
+        for s in range(1, steps + 1):
+            o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+            if s in snaps:
+                errs[f"step_{s}"] = state_error(shots[s], o.bodies())
+        o.close()
+        out["vs_f64_oracle"] = errs
+    return out
+
+
 # --------------------------------------------------------------------- serving
 def served_arrivals(capacity, seconds, load, seed, steps):
     """A Poisson stream of scenes (16-8192 bodies, log-uniform) offering `load`
@@ -263,6 +303,8 @@ def main():
                     help="also serve an overloaded stream (3x capacity) for this long (0: skip)")
     ap.add_argument("--serve-plain-scenes", type=int, default=6)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--tuned-hash", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
     args = ap.parse_args()
@@ -304,6 +346,14 @@ def main():
     log(f"built {args.worlds} device worlds in {setup_s:.1f} s")
     for _ in range(args.warmup):
         ctx.process_batch_handles(handles, dt, S, K, 1)
+    tuned, tune_steps = None, 0
+    if args.variant < 0 and not args.no_tune:
+        # launch shape picked on live warm-up steps (every shape gives the same bits)
+        from resolve2d_amd import autotune
+        cands = autotune.candidates(lds_limits=(0,))
+        tuned = autotune.tune(ctx, handles, dt, S, K, rounds=2, cands=cands, cache_path=None)
+        tune_steps = 0 if tuned["cached"] else 2 * len(cands)
+        log(f"autotune: variant {tuned['variant']} ({tuned['solve_ms']:.3f} ms solve)")
     ctx.sync()
 
     # timed region: exactly `steps` batched steps (one call each, so each call's
@@ -383,6 +433,11 @@ def main():
                              "p99_step_ms": float(np.percentile(np.array(lat1) * 1e3, 99))}
             w.close()
 
+    large = None
+    if not args.no_large:
+        log("large constrained world")
+        large = large_world(ctx, 120, {1, 10, 120}, not args.no_accuracy)
+
     acc = None
     if not args.no_accuracy:
         log("accuracy")
@@ -419,7 +474,7 @@ def main():
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": args.warmup + tune_steps,
         "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -429,7 +484,9 @@ def main():
         "config": {"workload": "batched worlds: 1024 worlds x U[50,500] mixed discs/rects, 0-20 distance joints, "
                                "0-10 springs per world; dt=1/60, 4 sub-steps, 4 iterations",
                    "worlds": args.worlds, "bodies": n_bodies, "manifolds_last_step": counts["manifolds"],
-                   "schedule": "reference order (bit-exact)", "parallelism": "1 GPU"},
+                   "schedule": "reference order (bit-exact)", "parallelism": "1 GPU",
+                   "solve_launch": ({"autotuned": True, "variant": tuned["variant"], "candidates_ms": tuned["all_ms"]}
+                                    if tuned else {"autotuned": False, "variant": args.variant})},
         "p50_step_ms": float(np.percentile(lat_ms, 50)),
         "p99_step_ms": float(np.percentile(lat_ms, 99)),
         "plain": {"value": plain_value, "unit": "body-steps/s",
@@ -438,6 +495,7 @@ def main():
         "batched_vs_plain": value / plain_value,
         "served": serve,
         "single_world_4096": single,
+        "large_world_65536": large,
         "accuracy": acc,
         "phase_ms_per_step": per_step,
         "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -207,11 +207,16 @@ int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
 /* Accumulated milliseconds and step count per phase: out_ms[4], *steps. */
 int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
-/* Launch configuration of the solve kernel: -1 (default) picks by world
- * size; 0..7 select the register-slot kernel's slots x workgroup size and the
+/* Launch configuration of the solve kernel: -1 (default) is variant 1;
+ * 0..7 select the register-slot kernel's slots x workgroup size and the
  * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner
  * (resolve2d_amd/autotune.py) picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);
+/* Largest world (bodies) whose solver body state is kept in LDS; larger
+ * worlds use the solve kernel with body state in global memory.  0 (default)
+ * = as many as LDS holds.  Results are identical either way; this is a
+ * launch-shape knob for the autotuner. */
+int r2d_ctx_set_lds_limit(r2d_ctx* ctx, uint32_t max_bodies);
 /* Totals of the last step of the context: entries, raw candidates,
  * manifolds, solver items (out[4]). */
 int r2d_ctx_last_counts(r2d_ctx* ctx, uint64_t* out);

@@ -902,7 +902,6 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
     set_solve_smem_limit();
     if (const char* e = std::getenv("R2D_NO_GRAPH")) c->use_graph = !(e[0] == '1');
-    if (const char* e = std::getenv("R2D_LDS_NMAX")) c->lds_nmax = (uint32_t)std::atoi(e);
     *out = c;
     return R2D_OK;
 }
@@ -1359,6 +1358,13 @@ R2D_EXPORT int r2d_ctx_set_solve_variant(r2d_ctx* c, int variant)
 {
     if (!c || variant < -1 || variant >= SOLVE_VARIANTS) return R2D_E_INVALID_ARG;
     c->solve_variant = variant;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_set_lds_limit(r2d_ctx* c, uint32_t max_bodies)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    c->lds_nmax = max_bodies;
     return R2D_OK;
 }
 

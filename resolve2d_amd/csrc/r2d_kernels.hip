@@ -471,47 +471,47 @@ __global__ void k_narrow(StepDev S)
         uint32_t a = S.cand_a[q], b = S.cand_b[q], k = S.cand_k[q];
         uint32_t flag = 0, info = 0;
         float nx = 0.f, ny = 0.f;
+        // the pair filters of updateManifolds (lib.zig:273-282) depend on the
+        // pair only, so they run first: the first-occurrence test below costs
+        // (cells of a before k) x (cells of b), large for wall-sized bodies
+        uint32_t w = S.bworld[a];
+        const WorldDev& D = S.worlds[w];
+        uint32_t sa = a - D.body_base, sb = b - D.body_base;
+        uint32_t fa = D.flags[sa], fb = D.flags[sb];
+        bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
+        if (ok && D.nexcl) ok = !excluded(D, D.id[sa], D.id[sb]);
+        if (ok) ok = aabb_hit(D, sa, sb);
         // b first appears in a's query list at cell k unless an earlier cell of a
         // hashes to a bucket b also occupies (buckets are deduplicated, so
         // within one bucket b appears once)
-        bool first = true;
-        {
+        if (ok) {
             const uint32_t* ba = S.ent_bkt + S.cell_off[a];
             const uint32_t* bb = S.ent_bkt + S.cell_off[b];
             uint32_t nb = S.ncell[b];
-            for (uint32_t j = 0; j < nb && first; ++j) {
+            for (uint32_t j = 0; j < nb && ok; ++j) {
                 uint32_t h = bb[j];
                 for (uint32_t kk = 0; kk < k; ++kk)
-                    if (ba[kk] == h) { first = false; break; }
+                    if (ba[kk] == h) { ok = false; break; }
             }
         }
-        if (first) {
-            uint32_t w = S.bworld[a];
-            const WorldDev& D = S.worlds[w];
-            uint32_t sa = a - D.body_base, sb = b - D.body_base;
-            uint32_t fa = D.flags[sa], fb = D.flags[sb];
-            bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
-            if (ok && D.nexcl) ok = !excluded(D, D.id[sa], D.id[sb]);
-            if (ok) ok = aabb_hit(D, sa, sb);
-            if (ok) {
-                Geo ga = load_geo(D, sa, S.cc[a], S.cs[a]);
-                Geo gb = load_geo(D, sb, S.cc[b], S.cs[b]);
-                // performNarrowSAT: lower id is the first reference body
-                bool a_first = ga.id < gb.id;
-                const Geo& o1 = a_first ? ga : gb;
-                const Geo& o2 = a_first ? gb : ga;
-                Sat sat;
-                const float undef = -3.0316488252093987e-13f;   // Zig Debug fill of `undefined` (0xAA)
-                sat.normal = mk(undef, undef);
-                sat.pen = __builtin_inff();
-                sat.nid = 0; sat.ref = 0; sat.set = false;
-                if (overlap_sat(sat, o1, o2, 0) && overlap_sat(sat, o2, o1, 1)) {
-                    flag = 1;
-                    // ref body expressed relative to (a, b)
-                    bool ref_is_a = (sat.ref == 0) == a_first;
-                    info = (uint32_t)sat.nid | (ref_is_a ? 4u : 0u);
-                    nx = sat.normal.x; ny = sat.normal.y;
-                }
+        if (ok) {
+            Geo ga = load_geo(D, sa, S.cc[a], S.cs[a]);
+            Geo gb = load_geo(D, sb, S.cc[b], S.cs[b]);
+            // performNarrowSAT: lower id is the first reference body
+            bool a_first = ga.id < gb.id;
+            const Geo& o1 = a_first ? ga : gb;
+            const Geo& o2 = a_first ? gb : ga;
+            Sat sat;
+            const float undef = -3.0316488252093987e-13f;   // Zig Debug fill of `undefined` (0xAA)
+            sat.normal = mk(undef, undef);
+            sat.pen = __builtin_inff();
+            sat.nid = 0; sat.ref = 0; sat.set = false;
+            if (overlap_sat(sat, o1, o2, 0) && overlap_sat(sat, o2, o1, 1)) {
+                flag = 1;
+                // ref body expressed relative to (a, b)
+                bool ref_is_a = (sat.ref == 0) == a_first;
+                info = (uint32_t)sat.nid | (ref_is_a ? 4u : 0u);
+                nx = sat.normal.x; ny = sat.normal.y;
             }
         }
         S.cand_flag[q] = flag;

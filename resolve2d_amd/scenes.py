@@ -260,3 +260,55 @@ def poisson_stream(seed: int, rate: float, count: int, n_min: int = 16, n_max: i
     n = np.exp(rng.uniform(math.log(n_min), math.log(n_max), count)).round().astype(int)
     seeds = rng.integers(0, 2 ** 31, count)
     return [(float(t[i]), int(n[i]), int(seeds[i])) for i in range(count)]
+
+
+def large_constrained(seed: int = 4, n_bodies: int = 65536, ropes: int = 100, links: int = 20, motors: int = 64,
+                      size: float = 600.0):
+    """The large constrained world: a size x size m domain walled by four
+    static boxes (floor, ceiling, left, right; 10 m thick), gravity 9.81, and
+    n_bodies dynamic bodies in total:
+      - `ropes` ropes of `links` distance joints each: links + 1 discs of
+        radius 0.2 spaced 0.8 m apart horizontally, rest length = initial
+        distance (default parameters);
+      - `motors` wheels (discs of radius 1, density 1) driven by motor
+        joints, target angular velocity ~ U[-5, 5] rad/s, power cap +-50
+        (the reference's motor adds beta (w* - w) dt to the angular momentum,
+        which overshoots unless beta dt / I < 1: wheels of inertia pi / 2
+        keep it at 0.03);
+      - the remaining bodies mixed as in `mixed` (discs r ~ U[0.2, 0.6],
+        rectangles w, h ~ U[0.3, 1.0], angle ~ U[0, 2pi), density 1,
+        friction 0.4) at uniform positions inside the walls.
+    Body ids exceed the reference's u16 range (65,540 bodies)."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.gravity(9.81)
+    h = size / 2.0
+    t = 10.0
+    walls = [(0.0, -t / 2, size + 2 * t, t), (0.0, size + t / 2, size + 2 * t, t),
+             (-h - t / 2, size / 2, t, size), (h + t / 2, size / 2, t, size)]
+    for (x, y, w, hh) in walls:
+        b.static(b.rect(x, y, w, hh, mu=0.4))
+    inner = h - 5.0
+    for r in range(ropes):
+        x0 = rng.uniform(-inner, inner - 0.8 * links)
+        y0 = rng.uniform(5.0, size - 5.0)
+        ids = [b.disc(x0 + 0.8 * k, y0, 0.2, mu=0.4) for k in range(links + 1)]
+        for k in range(links):
+            b.joint(0, ids[k], ids[k + 1], target=f32(0.8 * (k + 1) + x0) - f32(0.8 * k + x0))
+    omega = rng.uniform(-5.0, 5.0, motors)
+    for k in range(motors):
+        wheel = b.disc(rng.uniform(-inner, inner), rng.uniform(5.0, size - 5.0), 1.0, mu=0.4)
+        b.joint(3, wheel, target=omega[k], pmax=50.0, pmin=-50.0)
+    rest = n_bodies - ropes * (links + 1) - motors
+    x = rng.uniform(-inner, inner, rest)
+    y = rng.uniform(5.0, size - 5.0, rest)
+    rad = rng.uniform(0.2, 0.6, rest)
+    w = rng.uniform(0.3, 1.0, rest)
+    hgt = rng.uniform(0.3, 1.0, rest)
+    ang = rng.uniform(0.0, 2 * math.pi, rest)
+    for i in range(rest):
+        if i % 2 == 0:
+            b.disc(x[i], y[i], rad[i], mu=0.4)
+        else:
+            b.rect(x[i], y[i], w[i], hgt[i], mu=0.4, angle=ang[i])
+    return b.ops

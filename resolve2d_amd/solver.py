@@ -33,6 +33,7 @@ class Context:
         h = ctypes.c_void_p()
         N.check(self.lib.r2d_ctx_create(device, ctypes.byref(h)), "r2d_ctx_create")
         self.handle = h
+        self.profiling = False
 
     def close(self):
         if self.handle:
@@ -56,8 +57,12 @@ class Context:
     def set_solve_variant(self, variant: int):
         N.check(self.lib.r2d_ctx_set_solve_variant(self.handle, variant), "r2d_ctx_set_solve_variant")
 
+    def set_lds_limit(self, max_bodies: int):
+        N.check(self.lib.r2d_ctx_set_lds_limit(self.handle, max_bodies), "r2d_ctx_set_lds_limit")
+
     def set_profiling(self, on: bool):
         N.check(self.lib.r2d_ctx_set_profiling(self.handle, 1 if on else 0), "r2d_ctx_set_profiling")
+        self.profiling = bool(on)
 
     def phase_times(self):
         ms = (ctypes.c_double * 4)()

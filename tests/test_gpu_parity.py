@@ -116,3 +116,33 @@ def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
         assert bits_equal(g.bodies(), o.bodies()), (variant, i, first_diff(g.bodies(), o.bodies()))
         assert bits_equal(g.manifolds(), o.manifolds()), (variant, i)
         g.close()
+
+
+def test_autotune_picks_and_caches(gpu_ctx, oracle, tmp_path):
+    """The tuner steps the batch with each candidate (results unchanged: the
+    batch still equals the oracle) and reuses its choice for the same shape."""
+    import os
+    from resolve2d_amd import autotune, scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    specs = [scenes.batched_world(70 + i) for i in range(3)]
+    gw = [GpuWorld(gpu_ctx) for _ in specs]
+    ow = [oracle.OracleWorld("f32") for _ in specs]
+    for sc, g, o in zip(specs, gw, ow):
+        apply_scene(sc, g)
+        apply_scene(sc, o)
+    path = os.path.join(str(tmp_path), "tune.json")
+    cands = [(1, 0), (6, 0), (7, 1024)]
+    try:
+        r = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, rounds=1, cands=cands, cache_path=path)
+        assert (r["variant"], r["lds_limit"]) in cands and not r["cached"]
+        r2 = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, cands=cands, cache_path=path)
+        assert r2["cached"] and r2["variant"] == r["variant"]
+    finally:
+        gpu_ctx.set_solve_variant(-1)
+        gpu_ctx.set_lds_limit(0)
+    for o in ow:
+        for _ in range(len(cands)):
+            assert o.process(scenes.DT, 4, 4) == 0
+    for g, o in zip(gw, ow):
+        assert bits_equal(g.bodies(), o.bodies())
+        g.close()

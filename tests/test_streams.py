@@ -52,3 +52,20 @@ def test_poisson_stream_rate_and_sizes():
     assert n.min() >= 16 and n.max() <= 8192
     # log-uniform: the median is the geometric mean of the bounds
     assert abs(math.log(np.median(n)) - 0.5 * (math.log(16) + math.log(8192))) < 0.15
+
+
+def test_large_constrained_scene_shape():
+    from collections import Counter
+    ops = scenes.large_constrained()
+    c = Counter(o[0] for o in ops)
+    assert c["disc"] + c["rect"] == 65540 and c["static"] == 4 and c["gravity"] == 1
+    joints = [o for o in ops if o[0] == "joint"]
+    dist = [j for j in joints if j[1] == 0]
+    mot = [j for j in joints if j[1] == 3]
+    assert len(dist) == 2000 and len(mot) == 64
+    assert all(j[2] == np.float32(50.0) and j[3] == np.float32(-50.0) for j in mot)
+    assert all(-5.0 <= j[11] <= 5.0 for j in mot)
+    # ropes: 100 chains of 20 consecutive links
+    pairs = sorted((j[5], j[6]) for j in dist)
+    assert all(b == a + 1 for a, b in pairs)
+    assert len({a for a, _ in pairs} - {b for _, b in pairs}) == 100
