@@ -954,7 +954,6 @@ static void F(color_items)(F(world) *w, int32_t *perm)
     }
     uint64_t *mask = (uint64_t *)calloc(2 * (size_t)N + 2, sizeof(uint64_t));
     uint32_t *col = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(I + 1));
-    uint32_t next_unique = 128;
     for (int32_t it = 0; it < I; ++it) {
         int32_t r[2] = { -1, -1 };
         if (it < C) {
@@ -976,7 +975,7 @@ static void F(color_items)(F(world) *w, int32_t *perm)
         uint32_t cc;
         if (~m0) cc = (uint32_t)__builtin_ctzll(~m0);
         else if (~m1) cc = 64u + (uint32_t)__builtin_ctzll(~m1);
-        else cc = next_unique++;
+        else cc = 128u + (uint32_t)it;   /* a colour of its own, increasing with the item */
         col[it] = cc;
         for (int k = 0; k < 2; ++k) {
             if (r[k] < 0) continue;

@@ -364,7 +364,6 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.it_color, I * 4);
     add((void**)&S.it_perm, I * 4);
     add((void**)&S.recs, I * 128);
-    add((void**)&S.chist, 130 * (W + 1) * 4);
     add((void**)&S.cmask, 2 * N * 8);
     add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
     add((void**)&S.counters, 64 * 4);
@@ -490,7 +489,9 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         int v = std::atoi(e);
         if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
-    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax);
+    bool colored = false;
+    for (size_t i = 0; i < nw; ++i) colored = colored || worlds[i]->sched != 0;
+    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored);
     return R2D_OK;
 }
 

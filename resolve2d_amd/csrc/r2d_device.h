@@ -107,7 +107,6 @@ struct StepDev {
     uint32_t* lists;                       // [2*cap_i]
     uint32_t* it_color;                    // [cap_i] colour (0 in reference order)
     uint32_t* it_perm;                     // [cap_i] world-local items in execution order
-    uint32_t* chist;                       // [130*W] colour histogram per world
     uint64_t* cmask;                       // [2*N] colours used per body (coloured schedule)
     uint4* recs;                           // [8*cap_i] solver item records (128 B each)
     // global-memory solver state (worlds too large for LDS)

@@ -35,10 +35,11 @@ struct SolvePlan {
     uint32_t slot_maxn = 0;   // largest world each launch may take: sizes its LDS
     uint32_t lds_maxn = 0;
     uint32_t any_slot = 0, any_lds = 0, any_big = 0;
+    uint32_t any_colored = 0;  // some world uses the coloured schedule
 };
 // lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
 // many as LDS holds); larger worlds use the global-memory record kernel
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit);
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored);
 
 // the context's main stream plus two side streams for the solve kernels
 struct Streams {

@@ -739,71 +739,131 @@ __global__ void k_list_fill(StepDev S)
     }
 }
 
-// Coloured schedule: greedy colouring of each world's items in item order;
-// an item takes the smallest colour unused by earlier items sharing one of
-// its ordering bodies.  Worlds in reference order keep colour 0 everywhere.
-__global__ void k_color(StepDev S)
+__device__ __forceinline__ uint32_t ld_acq(uint32_t* p)
 {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void st_rel(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
+
+// Coloured schedule (a world with r2d_set_schedule(w, 1)): greedy colouring
+// of the world's items in item order -- an item takes the smallest colour
+// unused by earlier items sharing one of its ordering bodies; past 127
+// colours an item takes 128 + its index, a colour of its own -- then the
+// execution order (colour, item).  One workgroup per world.  The colouring
+// runs as a dataflow over per-body counters in item order (an item waits
+// until every earlier item on its bodies has taken its colour), the pattern
+// of the solver; the order is a stable block-wide counting sort.  Worlds in
+// the reference order get colour 0 and the identity order.
+__global__ __launch_bounds__(256) void k_color(StepDev S)
+{
+    __shared__ uint32_t s_scan[256];
+    __shared__ uint32_t s_maxc, s_abort, s_base;
     if (S.counters[C_OVERFLOW]) return;
-    for (uint32_t w = grid_tid(); w < S.nworlds; w += grid_stride()) {
-        const WorldDev& D = S.worlds[w];
-        uint32_t i0 = S.w_item_base[w], i1 = S.w_item_base[w + 1];
-        if (D.sched == 0) {
-            for (uint32_t it = i0; it < i1; ++it) { S.it_color[it] = 0u; S.it_perm[it] = it - i0; }
-            continue;
-        }
-        for (int b = 0; b < D.nbodies; ++b) { S.cmask[2 * (D.body_base + b)] = 0; S.cmask[2 * (D.body_base + b) + 1] = 0; }
-        uint32_t overflow_color = 128u;
-        for (uint32_t it = i0; it < i1; ++it) {
-            int32_t r0 = S.it_r0[it], r1 = S.it_r1[it];
+    const uint32_t w = blockIdx.x;
+    const WorldDev& D = S.worlds[w];
+    const uint32_t i0 = S.w_item_base[w], nit = S.w_item_base[w + 1] - i0;
+    const uint32_t T = blockDim.x, t = threadIdx.x;
+    if (D.sched == 0) {
+        for (uint32_t p = t; p < nit; p += T) { S.it_color[i0 + p] = 0u; S.it_perm[i0 + p] = p; }
+        return;
+    }
+    const uint32_t base = D.body_base;
+    for (int b = t; b < D.nbodies; b += T) {
+        S.cmask[2 * (base + b)] = 0ull;
+        S.cmask[2 * (base + b) + 1] = 0ull;
+        S.g_cnt[base + b] = 0u;
+    }
+    if (t == 0) { s_maxc = 0u; s_abort = 0u; s_base = 0u; }
+    __syncthreads();
+    uint32_t spins = 0;
+    for (uint32_t p = t; p < nit;) {
+        if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        const uint32_t it = i0 + p;
+        const int32_t r0 = S.it_r0[it], r1 = S.it_r1[it];
+        bool ready = r0 < 0 || ld_acq(&S.g_cnt[r0]) == S.it_k0[it];
+        if (ready && r1 >= 0) ready = ld_acq(&S.g_cnt[r1]) == S.it_k1[it];
+        if (ready) {
             uint64_t m0 = 0, m1 = 0;
             if (r0 >= 0) { m0 |= S.cmask[2 * r0]; m1 |= S.cmask[2 * r0 + 1]; }
             if (r1 >= 0) { m0 |= S.cmask[2 * r1]; m1 |= S.cmask[2 * r1 + 1]; }
             uint32_t c;
             if (~m0) c = (uint32_t)__builtin_ctzll(~m0);
             else if (~m1) c = 64u + (uint32_t)__builtin_ctzll(~m1);
-            else c = overflow_color++;   // a colour of its own is always independent
+            else c = 128u + p;
             S.it_color[it] = c;
-            if (c < 64) {
-                uint64_t bit = 1ull << c;
-                if (r0 >= 0) S.cmask[2 * r0] |= bit;
-                if (r1 >= 0) S.cmask[2 * r1] |= bit;
-            } else if (c < 128) {
-                uint64_t bit = 1ull << (c - 64);
-                if (r0 >= 0) S.cmask[2 * r0 + 1] |= bit;
-                if (r1 >= 0) S.cmask[2 * r1 + 1] |= bit;
+            if (c < 128) {
+                const uint32_t hi = c >> 6;
+                const uint64_t bit = 1ull << (c & 63u);
+                if (r0 >= 0) S.cmask[2 * r0 + hi] |= bit;
+                if (r1 >= 0) S.cmask[2 * r1 + hi] |= bit;
             }
+            if (r0 >= 0) st_rel(&S.g_cnt[r0], S.it_k0[it] + 1u);
+            if (r1 >= 0) st_rel(&S.g_cnt[r1], S.it_k1[it] + 1u);
+            atomicMax(&s_maxc, umin(c, 128u));
+            p += T;
+            spins = 0;
+        } else {
+            // every item's predecessors come first, so some item is always ready
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                S.w_status[w] = 5;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
-        // execution order = (colour, item): a stable counting sort; colours past
-        // 127 are unique and increase with the item index, so they share the
-        // last bin in item order.  Workers walk this order, the same order the
-        // per-body ranks use, so the earliest unfinished item always has a worker.
-        uint32_t* h = S.chist + 130 * w;
-        for (int c = 0; c < 130; ++c) h[c] = 0u;
-        for (uint32_t it = i0; it < i1; ++it) h[umin(S.it_color[it], 128u) + 1] += 1u;
-        for (int c = 1; c < 130; ++c) h[c] += h[c - 1];
-        for (uint32_t it = i0; it < i1; ++it) {
-            uint32_t b = umin(S.it_color[it], 128u);
-            S.it_perm[i0 + h[b]] = it - i0;
-            h[b] += 1u;
+    }
+    __syncthreads();
+    if (s_abort) return;
+    // execution order: stable counting sort by min(colour, 128); each thread
+    // owns a contiguous chunk of items, bins are placed one after another
+    const uint32_t chunk = (nit + T - 1) / T;
+    const uint32_t lo = umin(t * chunk, nit), hi = umin(lo + chunk, nit);
+    const uint32_t maxc = s_maxc;
+    for (uint32_t bin = 0; bin <= maxc; ++bin) {
+        uint32_t cnt = 0;
+        for (uint32_t p = lo; p < hi; ++p) cnt += umin(S.it_color[i0 + p], 128u) == bin;
+        s_scan[t] = cnt;
+        __syncthreads();
+        for (uint32_t d = 1; d < T; d <<= 1) {
+            uint32_t v = t >= d ? s_scan[t - d] : 0u;
+            __syncthreads();
+            s_scan[t] += v;
+            __syncthreads();
         }
+        uint32_t off = s_base + s_scan[t] - cnt;
+        for (uint32_t p = lo; p < hi; ++p)
+            if (umin(S.it_color[i0 + p], 128u) == bin) S.it_perm[i0 + off++] = p;
+        __syncthreads();
+        if (t == T - 1) s_base += s_scan[T - 1];
+        __syncthreads();
     }
 }
 
-// rank of each item among the items sharing one of its resources: in the
-// reference's sequential order (constraints, then manifolds in discovery
-// order), or colour by colour for the coloured schedule
+// rank of each item among the items sharing one of its resources: in item
+// order (the reference's sequential order: constraints, then manifolds in
+// discovery order); with BY_COLOR, for bodies of coloured worlds, in
+// (colour, item) order
+template <bool BY_COLOR>
 __global__ void k_list_rank(StepDev S)
 {
     if (S.counters[C_OVERFLOW]) return;
     for (uint32_t x = grid_tid(); x < S.nb; x += grid_stride()) {
+        if (BY_COLOR && S.worlds[S.bworld[x]].sched == 0) continue;
         uint32_t o = S.deg_off[x], n = S.deg_off[x + 1] - o;
         uint32_t* l = S.lists + o;
+        auto key = [&](uint32_t v) -> uint64_t {
+            return BY_COLOR ? (((uint64_t)S.it_color[v >> 1] << 32) | v) : (uint64_t)v;
+        };
         for (uint32_t i = 1; i < n; ++i) {
             uint32_t v = l[i];
-            uint64_t kv = ((uint64_t)S.it_color[v >> 1] << 32) | v;
+            uint64_t kv = key(v);
             uint32_t j = i;
-            while (j > 0 && (((uint64_t)S.it_color[l[j - 1] >> 1] << 32) | l[j - 1]) > kv) { l[j] = l[j - 1]; --j; }
+            while (j > 0 && key(l[j - 1]) > kv) { l[j] = l[j - 1]; --j; }
             l[j] = v;
         }
         for (uint32_t p = 0; p < n; ++p) {
@@ -820,14 +880,6 @@ struct SolverState {
     uint32_t *cnt, *deg;
 };
 
-__device__ __forceinline__ uint32_t ld_acq(uint32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void st_rel(uint32_t* p, uint32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 template <class ST>
 __device__ __noinline__ void solve_joint(const WorldDev& D, const ST st, const Joint& J, float dt)
@@ -1158,7 +1210,6 @@ __device__ __forceinline__ void rec_writeback(const Rec& r, const StepDev& S)
     }
 }
 
-constexpr uint32_t SPIN_LIMIT = 1u << 22;
 // LDS of the register-slot kernel: 13 words of body state per body
 __host__ __device__ inline size_t slot_lds_bytes(int nbodies) { return (size_t)nbodies * 13u * 4u; }
 
@@ -1814,9 +1865,10 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
 int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
 
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit)
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored)
 {
     SolvePlan p;
+    p.any_colored = any_colored ? 1u : 0u;
     p.variant = (variant >= 0 && variant < SOLVE_VARIANTS) ? variant : 0;
     const SolveVariant& v = kSolveVariants[p.variant];
     p.slot_cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
@@ -1871,9 +1923,11 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
         hipLaunchKernelGGL(k_items_joints, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
         hipLaunchKernelGGL(k_items_mans, dim3(GRID), dim3(BS), 0, st, S);
         scan_u32(S.deg, S.deg_off, nullptr, S.nb, S.nb, nullptr, S.partials, st);
-        hipLaunchKernelGGL(k_color, dim3(blocks_for(S.nworlds, 64)), dim3(64), 0, st, S);
         hipLaunchKernelGGL(k_list_fill, dim3(GRID), dim3(BS), 0, st, S);
-        hipLaunchKernelGGL(k_list_rank, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_list_rank<false>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        hipLaunchKernelGGL(k_color, dim3(S.nworlds), dim3(256), 0, st, S);
+        if (sp.any_colored)
+            hipLaunchKernelGGL(k_list_rank<true>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
         break;
     default: {
         // the three solve kernels take disjoint sets of worlds (solve_route);
