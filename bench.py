@@ -305,6 +305,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the timed region")
     ap.add_argument("--tuned-hash", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
     args = ap.parse_args()
@@ -344,8 +345,9 @@ def main():
     setup_s = time.perf_counter() - t0
     dt, S, K = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
     log(f"built {args.worlds} device worlds in {setup_s:.1f} s")
+    batch = ctx.prepare_batch(handles)
     for _ in range(args.warmup):
-        ctx.process_batch_handles(handles, dt, S, K, 1)
+        batch.process(dt, S, K, 1)
     tuned = None
     if args.variant < 0 and not args.no_tune:
         # launch shape picked on a copy of the batch advanced to mid-horizon,
@@ -366,25 +368,28 @@ def main():
         log(f"autotune: variant {tuned['variant']} ({tuned['solve_ms']:.3f} ms solve at step {args.steps // 2})")
     ctx.sync()
 
-    # timed region: exactly `steps` batched steps (one call each, so each call's
-    # wall time is a step latency); phase events recorded on the library stream
+    # timed region: exactly `steps` batched steps in one call (the hipGraph of a
+    # step replayed back to back); HIP events between the phases of every step
+    # on the library's stream give the per-step device latency
     ctx.set_profiling(True)
     ctx.reset_phase_times()
-    lat = []
     ctx.sync()
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        a = time.perf_counter()
-        ctx.process_batch_handles(handles, dt, S, K, 1)
-        lat.append(time.perf_counter() - a)
+    batch.process(dt, S, K, args.steps)
     ctx.sync()
     elapsed = time.perf_counter() - t_start
     phase_ms, phase_steps = ctx.phase_times()
+    lat_ms = np.array(ctx.step_times())
     counts = ctx.last_counts()
     ctx.set_profiling(False)
     value = n_bodies * args.steps / elapsed
-    lat_ms = np.array(lat) * 1e3
     log(f"batched: {args.steps} steps in {elapsed:.3f} s ({value / 1e6:.1f} M body-steps/s)")
+    # interactive use: one step per call, host work and synchronisation included
+    call_lat = []
+    for _ in range(args.call_steps):
+        a = time.perf_counter()
+        batch.process(dt, S, K, 1)
+        call_lat.append((time.perf_counter() - a) * 1e3)
     batch_pen = max(max_penetration(w.manifolds()) for w in worlds[:: max(1, len(worlds) // 64)])
     for w in worlds:
         w.close()
@@ -500,6 +505,11 @@ def main():
                                     if tuned else {"autotuned": False, "variant": args.variant})},
         "p50_step_ms": float(np.percentile(lat_ms, 50)),
         "p99_step_ms": float(np.percentile(lat_ms, 99)),
+        "step_latency": "device time per step from HIP events (the timed region's 240 steps)",
+        "per_call": {"p50_ms": float(np.percentile(call_lat, 50)) if call_lat else None,
+                     "p99_ms": float(np.percentile(call_lat, 99)) if call_lat else None,
+                     "calls": len(call_lat), "note": "one step per r2d_process_batch call after the timed "
+                                                     "region, host preparation and synchronisation included"},
         "plain": {"value": plain_value, "unit": "body-steps/s",
                   "sample": f"{args.plain_worlds} worlds spread over the batch, {plain_steps} steps each, "
                             f"one world per launch"},

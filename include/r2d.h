@@ -206,6 +206,9 @@ int r2d_set_schedule(r2d_world* w, int mode);
 int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
 /* Accumulated milliseconds and step count per phase: out_ms[4], *steps. */
 int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
+/* Device time (ms) of each timed step since the last reset, in order: up to
+ * cap values into out (may be NULL), the count into *n. */
+int r2d_ctx_step_times(r2d_ctx* ctx, float* out, size_t cap, size_t* n);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
 /* Launch configuration of the solve kernel: -1 (default) is variant 1;
  * 0..7 select the register-slot kernel's slots x workgroup size and the

@@ -135,6 +135,7 @@ SIGNATURES = {
     "r2d_ctx_set_profiling": [P, I32],
     "r2d_ctx_phase_times": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
     "r2d_ctx_reset_phase_times": [P],
+    "r2d_ctx_step_times": [P, ctypes.POINTER(ctypes.c_float), SZ, SZP],
     "r2d_ctx_last_counts": [P, ctypes.POINTER(ctypes.c_uint64)],
     "r2d_debug_eval": [P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), SZ],
     "r2d_last_error": [ctypes.c_char_p, SZ],

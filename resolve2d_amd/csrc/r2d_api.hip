@@ -99,6 +99,9 @@ struct r2d_world {
     bool dev_valid = false;   // device body block is current
     bool topo_dirty = true;   // joints / generators / exclusions need upload
     bool in_service = false;  // held by a server until its request completes
+    uint64_t version = 1;     // bumped by every edit and device reallocation
+    uint64_t desc_version = 0, checked_version = 0;
+    WorldDev desc{};          // descriptor of `desc_version` (batch offsets aside)
     // manifolds of the last step this world took part in
     uint64_t epoch = 0;
     uint32_t batch_index = 0;
@@ -131,6 +134,7 @@ struct r2d_ctx {
     std::vector<hipEvent_t> ev_pool;              // phase events of timed steps not yet summed
     size_t ev_used = 0;
     double phase_ms[4] = {0, 0, 0, 0};
+    std::vector<float> step_ms;                   // device time of each timed step
     uint64_t phase_steps = 0;
     uint64_t epoch = 0;
     uint64_t last_counts[4] = {0, 0, 0, 0};
@@ -144,7 +148,9 @@ struct r2d_ctx {
     int solve_variant = -1;                       // -1: choose by world size
     uint32_t lds_nmax = 0;                        // 0: LDS body state up to the LDS capacity
 
-    Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}}; }
+    KernelTrace* trace = nullptr;                 // per-kernel timing (R2D_TRACE_KERNELS)
+
+    Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}, trace}; }
 };
 
 namespace {
@@ -163,6 +169,7 @@ int ensure_world_device(r2d_world* w)
         if (hipMalloc(&w->dblock, (size_t)F_NFIELDS * c * 4) != hipSuccess) return R2D_E_OOM;
         w->dcap = c;
         w->dev_valid = false;
+        w->version += 1;
     }
     return R2D_OK;
 }
@@ -207,6 +214,7 @@ int begin_edit(r2d_world* w)
     int rc = download_bodies(w);
     if (rc) return rc;
     w->dev_valid = false;
+    w->version += 1;
     return R2D_OK;
 }
 
@@ -283,6 +291,7 @@ int upload_topology(r2d_world* w)
         HIPCHK(hipMemcpyAsync(w->dexcl, w->excl.data(), w->excl.size() * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     w->topo_dirty = false;
+    w->version += 1;   // device topology pointers may have moved
     return R2D_OK;
 }
 
@@ -344,8 +353,8 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.ncell, (N + 1) * 4); add((void**)&S.cell_off, (N + 1) * 4);
     add((void**)&S.cc, N * 4); add((void**)&S.cs, N * 4); add((void**)&S.sflag, N * 4);
     add((void**)&S.bkt_cnt, (B + 1) * 4); add((void**)&S.bkt_off, (B + 1) * 4);
-    add((void**)&S.bkt_fill, (B + 1) * 4); add((void**)&S.bkt_len, (B + 1) * 4);
-    add((void**)&S.bkt_items, E * 4);
+    add((void**)&S.bkt_fill, (B + 1) * 4);
+    add((void**)&S.bkt_items, E * 4); add((void**)&S.bkt_sorted, E * 4);
     add((void**)&S.ent_body, E * 4); add((void**)&S.ent_bkt, E * 4);
     add((void**)&S.ent_cnt, (E + 1) * 4); add((void**)&S.ent_off, (E + 1) * 4);
     add((void**)&S.cand_a, R * 4); add((void**)&S.cand_b, R * 4); add((void**)&S.cand_k, R * 4);
@@ -356,7 +365,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.m_prev1, M * 4); add((void**)&S.m_prev2, M * 4); add((void**)&S.m_valid, M * 4);
     add((void**)&S.m_pt, (size_t)MP_NFIELDS * 2 * M * 4);
     add((void**)&S.w_man_begin, (W + 1) * 4); add((void**)&S.w_item_base, (W + 1) * 4);
-    add((void**)&S.w_status, (W + 1) * 4);
+    add((void**)&S.w_status, (W + 1) * 4); add((void**)&S.w_order, (W + 1) * 4);
     add((void**)&S.it_r0, I * 4); add((void**)&S.it_r1, I * 4);
     add((void**)&S.it_k0, I * 4); add((void**)&S.it_k1, I * 4);
     add((void**)&S.deg, (N + 1) * 4); add((void**)&S.deg_off, (N + 1) * 4); add((void**)&S.deg_fill, (N + 1) * 4);
@@ -449,9 +458,11 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
     for (size_t i = 0; i < nw; ++i) {
         r2d_world* w = worlds[i];
         if (!w || w->ctx != c) return R2D_E_INVALID_ARG;
-        if (solves)
+        if (solves && w->checked_version != w->version) {
             for (const HJoint& h : w->joints)
                 if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
+            w->checked_version = w->version;
+        }
     }
     b = Batch();
     b.worlds.assign(worlds, worlds + nw);
@@ -463,7 +474,11 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         int rc;
         if (!w->dev_valid && (rc = upload_bodies(w))) return rc;
         if ((rc = upload_topology(w))) return rc;
-        WorldDev d = make_desc(w);
+        if (w->desc_version != w->version) {
+            w->desc = make_desc(w);
+            w->desc_version = w->version;
+        }
+        WorldDev d = w->desc;
         d.body_base = (uint32_t)b.N;
         d.bucket_base = (uint32_t)b.B;
         d.joint_base = (uint32_t)b.J;
@@ -501,11 +516,14 @@ int timer_flush(r2d_ctx* c)
     if (!c->ev_used) return R2D_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
     for (size_t k = 0; k + KT_N <= c->ev_used; k += KT_N) {
+        float total = 0.f;
         for (int p = 0; p < 4; ++p) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[k + p], c->ev_pool[k + p + 1]));
             c->phase_ms[p] += ms;
+            total += ms;
         }
+        c->step_ms.push_back(total);
         c->phase_steps += 1;
     }
     c->ev_used = 0;
@@ -594,6 +612,7 @@ int batch_wait(r2d_ctx* c, Batch& b)
     const size_t nw = b.worlds.size();
     for (;;) {
         HIPCHK(hipEventSynchronize(c->ev_end));
+        if (c->trace && c->trace->marks.size() > 4096) c->trace->flush();
         uint32_t committed = c->hcounters[C_STEPS];
         b.done += committed;
         b.world_status.assign(c->hstatus, c->hstatus + nw);
@@ -676,11 +695,11 @@ void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
 // Admission estimate of one resident world: its device block and topology
 // plus its share of the step scratch at the initial capacity estimates
 // (per body: cell ranges, 4 cell entries, 8 raw candidates, 4 manifolds and
-// 4 solver items; per hash bucket 16 B; per joint one solver item).
-constexpr size_t kScratchPerBody = 88 + 4 * 20 + 8 * 32 + 4 * 128 + 4 * 160;
-constexpr size_t kScratchPerBucket = 16;
+// 4 solver items; per hash bucket 12 B; per joint one solver item).
+constexpr size_t kScratchPerBody = 88 + 4 * 24 + 8 * 32 + 4 * 128 + 4 * 160;
+constexpr size_t kScratchPerBucket = 12;
 constexpr size_t kScratchPerJoint = 160;
-constexpr size_t kScratchPerWorld = 12 + 130 * 4 + sizeof(WorldDev);
+constexpr size_t kScratchPerWorld = 16 + sizeof(WorldDev);
 
 size_t world_footprint(const r2d_world* w)
 {
@@ -903,6 +922,13 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
     set_solve_smem_limit();
     if (const char* e = std::getenv("R2D_NO_GRAPH")) c->use_graph = !(e[0] == '1');
+    if (const char* e = std::getenv("R2D_TRACE_KERNELS")) {
+        if (e[0] == '1') {
+            c->trace = new (std::nothrow) KernelTrace();
+            if (c->trace) c->trace->main = c->stream;
+            c->use_graph = false;   // an event after every launch: eager launches
+        }
+    }
     *out = c;
     return R2D_OK;
 }
@@ -911,6 +937,10 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
 {
     if (!c) return R2D_E_INVALID_ARG;
     (void)hipStreamSynchronize(c->stream);
+    if (c->trace) {
+        c->trace->report(stderr);
+        delete c->trace;
+    }
     destroy_graphs(c);
     if (c->arena) (void)hipFree(c->arena);
     if (c->dworlds) (void)hipFree(c->dworlds);
@@ -970,6 +1000,7 @@ R2D_EXPORT int r2d_world_clear(r2d_world* w)
     w->gens.clear();
     w->excl.clear();
     w->topo_dirty = true;
+    w->version += 1;
     w->epoch = 0;
     return R2D_OK;
 }
@@ -1061,6 +1092,7 @@ R2D_EXPORT int r2d_remove_body(r2d_world* w, uint32_t id)
     w->id2slot[id] = -1;
     w->n -= 1;
     w->topo_dirty = true;   // joint / spring slots move
+    w->version += 1;
     w->epoch = 0;
     return R2D_OK;
 }
@@ -1073,6 +1105,7 @@ R2D_EXPORT int r2d_make_gravity(r2d_world* w, float g)
     h.type = 0; h.gx = 0.0f; h.gy = -g;
     w->gens.push_back(h);
     w->topo_dirty = true;
+    w->version += 1;
     return R2D_OK;
 }
 
@@ -1085,6 +1118,7 @@ R2D_EXPORT int r2d_make_spring(r2d_world* w, uint32_t id, float k, float c, floa
     h.type = 1; h.id = id; h.k = k; h.c = c; h.ax = ax; h.ay = ay;
     w->gens.push_back(h);
     w->topo_dirty = true;
+    w->version += 1;
     return R2D_OK;
 }
 
@@ -1098,6 +1132,7 @@ static int add_joint(r2d_world* w, const r2d_constraint_params* p, HJoint h, siz
     h.pmax = p->power_max; h.pmin = p->power_min; h.beta = p->beta;
     w->joints.push_back(h);
     w->topo_dirty = true;
+    w->version += 1;
     if (handle) *handle = w->joints.size() - 1;
     return R2D_OK;
 }
@@ -1143,6 +1178,7 @@ R2D_EXPORT int r2d_exclude_collision_pair(r2d_world* w, uint32_t a, uint32_t b)
     auto it = std::lower_bound(w->excl.begin(), w->excl.end(), k);
     if (it == w->excl.end() || *it != k) w->excl.insert(it, k);
     w->topo_dirty = true;
+    w->version += 1;
     return R2D_OK;
 }
 
@@ -1274,16 +1310,20 @@ R2D_EXPORT int r2d_set_hash_params(r2d_world* w, int mode, float cell, size_t mu
 {
     if (!w || (mode != 0 && mode != 1)) return R2D_E_INVALID_ARG;
     if (mode == 1 && (!(cell > 0.0f) || mult == 0)) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     w->hash_mode = mode;
     w->hash_cell = cell;
     w->hash_mult = mult;
+    w->version += 1;
     return R2D_OK;
 }
 
 R2D_EXPORT int r2d_set_schedule(r2d_world* w, int mode)
 {
     if (!w || (mode != 0 && mode != 1)) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
     w->sched = mode;
+    w->version += 1;
     return R2D_OK;
 }
 
@@ -1304,6 +1344,16 @@ R2D_EXPORT int r2d_ctx_phase_times(r2d_ctx* c, double* out_ms, uint64_t* steps)
     return R2D_OK;
 }
 
+R2D_EXPORT int r2d_ctx_step_times(r2d_ctx* c, float* out, size_t cap, size_t* n)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    int rc = timer_flush(c);
+    if (rc) return rc;
+    if (n) *n = c->step_ms.size();
+    if (out) std::memcpy(out, c->step_ms.data(), std::min(cap, c->step_ms.size()) * sizeof(float));
+    return R2D_OK;
+}
+
 R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
 {
     if (!c) return R2D_E_INVALID_ARG;
@@ -1311,6 +1361,7 @@ R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
     if (rc) return rc;
     for (int i = 0; i < 4; ++i) c->phase_ms[i] = 0;
     c->phase_steps = 0;
+    c->step_ms.clear();
     return R2D_OK;
 }
 

@@ -83,8 +83,9 @@ struct StepDev {
     float *cc, *cs;          // cos / sin of the angle at step start
     uint32_t* sflag;         // scheduling flags
     // spatial hash buckets
-    uint32_t *bkt_cnt, *bkt_off, *bkt_fill, *bkt_len;  // [B+1]
-    uint32_t* bkt_items;     // [cap_e]
+    uint32_t *bkt_cnt, *bkt_off, *bkt_fill;  // [B+1]
+    uint32_t* bkt_items;     // [cap_e] bucket members as filled (any order)
+    uint32_t* bkt_sorted;    // [cap_e] bucket members in descending slot
     // entries -> raw candidates
     uint32_t *ent_body, *ent_bkt;  // [cap_e]
     uint32_t *ent_cnt, *ent_off;   // [cap_e+1]
@@ -100,6 +101,7 @@ struct StepDev {
     // per world
     uint32_t *w_man_begin, *w_item_base;   // [W+1]
     int32_t* w_status;
+    uint32_t* w_order;       // [W] worlds in the order the solve workgroups take them
     // solver items (joints then manifolds, per world)
     int32_t *it_r0, *it_r1;
     uint32_t *it_k0, *it_k1;

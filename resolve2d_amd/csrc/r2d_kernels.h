@@ -3,6 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
 #include "r2d_device.h"
 
 namespace r2d {
@@ -41,12 +46,31 @@ struct SolvePlan {
 // many as LDS holds); larger worlds use the global-memory record kernel
 SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored);
 
+// Optional per-kernel timing (a context created with R2D_TRACE_KERNELS set):
+// an event after every launch, summed per kernel name; the steps run eagerly.
+struct KernelTrace {
+    struct Mark {
+        const char* name;
+        hipEvent_t ev;
+        hipStream_t st;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, uint64_t>> totals;
+    hipStream_t main = nullptr;
+    void mark(const char* name, hipStream_t st);
+    void flush();                 // synchronise and add the recorded intervals
+    void report(FILE* f);
+    ~KernelTrace();
+};
+
 // the context's main stream plus two side streams for the solve kernels
 struct Streams {
     hipStream_t main;
     hipStream_t aux[2];
     hipEvent_t fork;
     hipEvent_t join[2];
+    KernelTrace* trace;
 };
 
 // enqueue one phase of a batched step (phases in PH_* order on ss.main)

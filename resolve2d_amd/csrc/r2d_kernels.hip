@@ -411,36 +411,43 @@ __global__ void k_bucket_fill(StepDev S)
 
 // Within a bucket the reference stores entries in reverse insertion order
 // (pre-decrement scatter, SpatialHash.zig:36-40, 59-68): descending body slot.
-// A body listed twice in one bucket cannot change which pair appears first.
-__global__ void k_bucket_sort(StepDev S)
+// Each entry's place is the number of bucket members with a higher slot plus
+// its rank among its own body's earlier entries in the same bucket (a body
+// listed twice in one bucket -- two of its cells hash alike -- sits in
+// adjacent places; readers skip the repeat, which cannot change which pair
+// appears first).
+__global__ void k_bucket_rank(StepDev S)
 {
     if (S.counters[C_OVERFLOW]) return;
-    for (uint32_t gb = grid_tid(); gb < S.nbuckets; gb += grid_stride()) {
+    uint32_t E = dev_count(&S.counters[C_ENTRIES], S.cap_e);
+    for (uint32_t e = grid_tid(); e < E; e += grid_stride()) {
+        uint32_t a = S.ent_body[e], gb = S.ent_bkt[e];
         uint32_t o = S.bkt_off[gb], n = S.bkt_cnt[gb];
-        uint32_t* it = S.bkt_items + o;
-        for (uint32_t i = 1; i < n; ++i) {
-            uint32_t v = it[i];
-            uint32_t j = i;
-            while (j > 0 && it[j - 1] < v) { it[j] = it[j - 1]; --j; }
-            it[j] = v;
+        const uint32_t* it = S.bkt_items + o;
+        uint32_t greater = 0, same = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            uint32_t v = it[j];
+            greater += v > a;
+            same += v == a;
         }
-        uint32_t u = 0;
-        for (uint32_t i = 0; i < n; ++i)
-            if (u == 0 || it[u - 1] != it[i]) it[u++] = it[i];
-        S.bkt_len[gb] = u;
+        uint32_t dup = 0;
+        if (same > 1)
+            for (uint32_t x = S.cell_off[a]; x < e; ++x) dup += S.ent_bkt[x] == gb;
+        S.bkt_sorted[o + greater + dup] = a;
     }
 }
 
-// raw candidates per entry: bucket members with a higher slot than the entry's body
+// raw candidates per entry: distinct bucket members with a higher slot than
+// the entry's body
 __global__ void k_ent_cnt(StepDev S)
 {
     if (S.counters[C_OVERFLOW]) return;
     uint32_t E = dev_count(&S.counters[C_ENTRIES], S.cap_e);
     for (uint32_t e = grid_tid(); e < E; e += grid_stride()) {
         uint32_t a = S.ent_body[e], gb = S.ent_bkt[e];
-        const uint32_t* it = S.bkt_items + S.bkt_off[gb];
-        uint32_t n = S.bkt_len[gb], c = 0;
-        while (c < n && it[c] > a) ++c;
+        const uint32_t* it = S.bkt_sorted + S.bkt_off[gb];
+        uint32_t n = S.bkt_cnt[gb], c = 0;
+        for (uint32_t j = 0; j < n && it[j] > a; ++j) c += (j == 0 || it[j] != it[j - 1]);
         S.ent_cnt[e] = c;
     }
 }
@@ -452,10 +459,14 @@ __global__ void k_cand_emit(StepDev S)
     if (S.counters[C_RAW] > S.cap_r) { if (grid_tid() == 0) atomicOr(&S.counters[C_OVERFLOW], 2u); return; }
     for (uint32_t e = grid_tid(); e < E; e += grid_stride()) {
         uint32_t a = S.ent_body[e], gb = S.ent_bkt[e];
-        const uint32_t* it = S.bkt_items + S.bkt_off[gb];
+        const uint32_t* it = S.bkt_sorted + S.bkt_off[gb];
         uint32_t o = S.ent_off[e], c = S.ent_cnt[e];
         uint32_t k = e - S.cell_off[a];
-        for (uint32_t j = 0; j < c; ++j) { S.cand_a[o + j] = a; S.cand_b[o + j] = it[j]; S.cand_k[o + j] = k; }
+        for (uint32_t j = 0, q = 0; q < c; ++j) {
+            if (j > 0 && it[j] == it[j - 1]) continue;
+            S.cand_a[o + q] = a; S.cand_b[o + q] = it[j]; S.cand_k[o + q] = k;
+            ++q;
+        }
     }
 }
 
@@ -662,6 +673,33 @@ __global__ void k_world_ranges(StepDev S)
         uint32_t jb = (w == S.nworlds) ? S.njoints : S.worlds[w].joint_base;
         S.w_item_base[w] = jb + mb;
         if (w == S.nworlds) S.counters[C_ITEMS] = jb + mb;
+    }
+}
+
+// Order in which the solve kernels' workgroups take worlds: most items first
+// (longest-processing-time first shortens the tail of the launch).  Worlds
+// are binned by log2 of their item count; any order gives the same results.
+__global__ __launch_bounds__(1024) void k_world_order(StepDev S)
+{
+    __shared__ uint32_t hist[33];
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t nw = S.nworlds;
+    if (threadIdx.x < 33) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+        uint32_t items = S.w_item_base[w + 1] - S.w_item_base[w];
+        atomicAdd(&hist[(uint32_t)__builtin_clz(items + 1u)], 1u);   // fewer leading zeros = more items
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int b = 0; b < 33; ++b) { uint32_t c = hist[b]; hist[b] = acc; acc += c; }
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+        uint32_t items = S.w_item_base[w + 1] - S.w_item_base[w];
+        uint32_t pos = atomicAdd(&hist[(uint32_t)__builtin_clz(items + 1u)], 1u);
+        S.w_order[pos] = w;
     }
 }
 
@@ -1241,7 +1279,7 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
-    const uint32_t w = blockIdx.x;
+    const uint32_t w = S.w_order[blockIdx.x];
     if (S.w_status[w] != 0) return;
     if (solve_route((uint32_t)S.worlds[w].nbodies, S.w_item_base[w + 1] - S.w_item_base[w], slot_cap, slot_nmax,
                     lds_nmax) != (LDS ? 1 : 2))
@@ -1464,6 +1502,8 @@ struct Slot {
     float p[2][7];           // manifold: r1x r1y r2x r2y depth mass_n mass_t per point
                              // joint: pmax pmin beta r1x r1y r2x r2y | target tx ty
     float acc[4];            // pn0 pt0 pn1 pt1
+    uint32_t degs;           // items at r0 | items at r1 << 16 (ordering bodies)
+    float bias[2];           // per point: Baumgarte bias of this sub-step, -1 when depth >= 0
 };
 constexpr uint32_t SLOT_USED = 0x80000000u;
 
@@ -1602,6 +1642,90 @@ __device__ __forceinline__ void slot_solve_joint(const Slot& z, const SlotState&
     }
 }
 
+// one body's solver state as read by a chain link
+struct BodyState {
+    float mx, my, L, im, ii;
+};
+
+__device__ __forceinline__ float lds_relaxed(const float* p)
+{
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
+__device__ __forceinline__ BodyState body_state(const SlotState& st, uint32_t i)
+{
+    return BodyState{lds_relaxed(&st.mx[i]), lds_relaxed(&st.my[i]), lds_relaxed(&st.L[i]), lds_relaxed(&st.im[i]),
+                     lds_relaxed(&st.ii[i])};
+}
+
+// CollisionManifold.calculateImpulses (collision.zig:135-218) on a slot, with
+// both bodies' state already read (the link's only reads of them) and the
+// Baumgarte bias of the sub-step precomputed ((0.02 max(0, -depth - 0.005)) / dt,
+// collision.zig:172; -1 marks depth >= 0)
+__device__ __forceinline__ void slot_contact(Slot& z, const BodyState& b1, const BodyState& b2, const SlotState& st)
+{
+    const uint32_t l1 = z.pair & 0xffffu, l2 = z.pair >> 16;
+    v2 vl1 = scale2(mk(b1.mx, b1.my), b1.im);
+    float om1 = b1.L * b1.ii;
+    v2 vl2 = scale2(mk(b2.mx, b2.my), b2.im);
+    float om2 = b2.L * b2.ii;
+    v2 n = mk(z.nx, z.ny);
+    v2 t = rot90cw(n);
+    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
+    float rot1 = 0.f, rot2 = 0.f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (!(z.flags & (RF_P0 << p))) continue;
+        if (z.bias[p] < 0.f) { z.acc[2 * p] = 0.f; z.acc[2 * p + 1] = 0.f; continue; }
+        v2 r1 = mk(z.p[p][0], z.p[p][1]), r2 = mk(z.p[p][2], z.p[p][3]);
+        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
+        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
+        v2 dv = sub2(v1, v2v);
+        float num = dot2(dv, n) + z.bias[p];
+        float pn = num * z.p[p][5];
+        if (pn < MIN_MANIFOLD_IMPULSE) continue;
+        num = dot2(dv, t);
+        float pt = num * z.p[p][6];
+        float acc_pn = z.acc[2 * p], acc_pt = z.acc[2 * p + 1];
+        float new_pn = maxn(0.0f, acc_pn + pn);
+        float applied_pn = new_pn - acc_pn;
+        acc_pn = new_pn;
+        float max_pt = z.fric * absf(acc_pn);
+        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
+        float applied_pt = new_pt - acc_pt;
+        z.acc[2 * p] = acc_pn;
+        z.acc[2 * p + 1] = new_pt;
+        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
+        if (!(z.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
+        if (!(z.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
+    }
+    if (z.flags & RF_W1) { st.mx[l1] = b1.mx + app1.x; st.my[l1] = b1.my + app1.y; st.L[l1] = b1.L + rot1; }
+    if (z.flags & RF_W2) { st.mx[l2] = b2.mx + app2.x; st.my[l2] = b2.my + app2.y; st.L[l2] = b2.L + rot2; }
+}
+
+// ordering bodies of a slot (-1: none) and the Baumgarte biases of a sub-step
+__device__ __forceinline__ void slot_ordering(const Slot& z, int32_t& r0, int32_t& r1)
+{
+    uint32_t a = z.pair & 0xffffu, b = z.pair >> 16;
+    if (z.flags & RF_JOINT) {
+        r0 = a == 0xffffu ? -1 : (int32_t)a;
+        r1 = b == 0xffffu ? -1 : (int32_t)b;
+    } else {
+        r0 = (z.flags & RF_W1) ? (int32_t)a : -1;
+        r1 = (z.flags & RF_W2) ? (int32_t)b : -1;
+    }
+}
+
+__device__ __forceinline__ void slot_biases(Slot& z, float dt)
+{
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float depth = z.p[p][4];
+        z.bias[p] = depth >= 0.f ? -1.0f : BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
+    }
+}
+
 template <int NSLOT, int MAXT, int WAVES>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
@@ -1609,7 +1733,7 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
-    const uint32_t w = blockIdx.x;
+    const uint32_t w = S.w_order[blockIdx.x];
     if (S.w_status[w] != 0) return;
     const uint32_t ibase = S.w_item_base[w];
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
@@ -1657,6 +1781,9 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
             rec_build(r, S, D, st, pos, ibase, nj, mbeg, base);
             slot_from_rec(sl[z], r);
             if (r.flags & RF_JOINT) slot_joint(sl[z], D.joints[r.idx], D);
+            int32_t q0, q1;
+            slot_ordering(sl[z], q0, q1);
+            sl[z].degs = (q0 >= 0 ? st.deg[q0] : 0u) | ((q1 >= 0 ? st.deg[q1] : 0u) << 16);
             nslot = (uint32_t)z + 1u;
         }
     }
@@ -1727,8 +1854,13 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
                 S.m_prev1[sl[z].idx] = D.ang[sl[z].pair & 0xffffu];
                 S.m_prev2[sl[z].idx] = D.ang[sl[z].pair >> 16];
             }
+            slot_biases(sl[z], sub_dt);
         }
-        // K iterations as an ordered dataflow over register-resident items
+        // K iterations as an ordered dataflow over register-resident items.  A
+        // link's counters and its bodies' state are read together: LDS serves
+        // one workgroup's operations in order, so state read after a counter
+        // that shows the item ready is the state its predecessor released
+        // (values read for an item that is not ready are discarded)
         {
             uint32_t k = nslot ? 0u : iters, cz = 0, spins = 0;
             bool have = false;
@@ -1736,54 +1868,57 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
             int32_t r0 = -1, r1 = -1;
             uint32_t t0 = 0, t1 = 0;
             while (k < iters) {
-                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 if (!have) {
 #pragma unroll
                     for (int z = 0; z < NSLOT; ++z)
                         if ((uint32_t)z == cz) cur = sl[z];
                     have = true;
-                    uint32_t a = cur.pair & 0xffffu, b = cur.pair >> 16;
-                    if (cur.flags & RF_JOINT) {
-                        r0 = a == 0xffffu ? -1 : (int32_t)a;
-                        r1 = b == 0xffffu ? -1 : (int32_t)b;
-                    } else {
-                        r0 = (cur.flags & RF_W1) ? (int32_t)a : -1;
-                        r1 = (cur.flags & RF_W2) ? (int32_t)b : -1;
-                    }
-                    if (r0 >= 0) t0 = k * st.deg[r0] + (cur.ranks & 0xffffu);
-                    if (r1 >= 0) t1 = k * st.deg[r1] + (cur.ranks >> 16);
+                    slot_ordering(cur, r0, r1);
+                    t0 = k * (cur.degs & 0xffffu) + (cur.ranks & 0xffffu);
+                    t1 = k * (cur.degs >> 16) + (cur.ranks >> 16);
                 }
-                bool ready = true;
-                if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
-                if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
+                uint32_t c0 = r0 >= 0 ? __hip_atomic_load(&st.cnt[r0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                      : t0;
+                uint32_t c1 = r1 >= 0 ? __hip_atomic_load(&st.cnt[r1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                      : t1;
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                const bool contact = !(cur.flags & RF_JOINT);
+                BodyState b1{}, b2{};
+                if (contact) {
+                    b1 = body_state(st, cur.pair & 0xffffu);
+                    b2 = body_state(st, cur.pair >> 16);
+                }
+                const bool ready = c0 == t0 && c1 == t1;
                 if (ready) {
-                    if (cur.flags & RF_JOINT) {
+                    if (!contact) {
                         slot_solve_joint(cur, st, sub_dt);
                     } else {
-                        Rec r;
-                        rec_from_slot(r, cur);
-                        rec_impulses(r, st, sub_dt);
-#pragma unroll
-                        for (int f = 0; f < 4; ++f) cur.acc[f] = r.acc[f];
+                        slot_contact(cur, b1, b2, st);
 #pragma unroll
                         for (int z = 0; z < NSLOT; ++z)
                             if ((uint32_t)z == cz)
 #pragma unroll
                                 for (int f = 0; f < 4; ++f) sl[z].acc[f] = cur.acc[f];
                     }
-                    if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
-                    if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                    // the state writes above reach LDS before the counters (in order)
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     have = false;
                     spins = 0;
                     if (++cz == nslot) { cz = 0; ++k; }
                 } else {
+                    // every item's predecessors precede it, so some item is always
+                    // ready; a stall this long means corrupt ordering data: stop
+                    if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                     if (++spins > SPIN_LIMIT) {
                         __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         S.w_status[w] = 5;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
                 }
+                // a wave with no lane ready yields its SIMD
+                if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
             }
         }
         __syncthreads();
@@ -1865,6 +2000,65 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
 int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
 
+void KernelTrace::mark(const char* name, hipStream_t st)
+{
+    if (marks.size() == pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        pool.push_back(e);
+    }
+    hipEvent_t e = pool[marks.size()];
+    if (hipEventRecord(e, st) != hipSuccess) return;
+    marks.push_back({name, e, st});
+}
+
+void KernelTrace::flush()
+{
+    if (marks.empty()) return;
+    (void)hipDeviceSynchronize();
+    // a main-stream kernel's interval starts at the previous main-stream mark;
+    // a side-stream kernel's at the last main-stream mark before it (the fork)
+    const Mark* prev_main = nullptr;
+    for (const Mark& m : marks) {
+        if (prev_main && m.name[0]) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, prev_main->ev, m.ev) == hipSuccess) {
+                auto& t = totals[m.name];
+                t.first += ms;
+                t.second += 1;
+            }
+        }
+        if (m.st == main) prev_main = &m;
+    }
+    marks.clear();
+}
+
+void KernelTrace::report(FILE* f)
+{
+    flush();
+    std::vector<std::pair<double, std::string>> v;
+    double all = 0;
+    for (auto& kv : totals) { v.push_back({kv.second.first, kv.first}); all += kv.second.first; }
+    std::sort(v.rbegin(), v.rend());
+    std::fprintf(f, "[r2d] kernel times (eager launches, one event after each):\n");
+    for (auto& e : v) {
+        const auto& t = totals[e.second];
+        std::fprintf(f, "[r2d] %-28s calls %8llu  total %10.3f ms  avg %9.2f us  %5.1f%%\n", e.second.c_str(),
+                     (unsigned long long)t.second, t.first, 1e3 * t.first / (double)std::max<uint64_t>(t.second, 1),
+                     100.0 * t.first / std::max(all, 1e-9));
+    }
+}
+
+KernelTrace::~KernelTrace()
+{
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+static inline void tmark(const Streams& ss, const char* name, hipStream_t st)
+{
+    if (ss.trace) ss.trace->mark(name, st);
+}
+
 SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored)
 {
     SolvePlan p;
@@ -1893,41 +2087,72 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
     switch (phase) {
     case PH_BROAD:
         hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, S.counters);
+        tmark(ss, "k_begin_step", st);
         hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nworlds + 1, BS)), dim3(BS), 0, st,
                            reinterpret_cast<uint32_t*>(S.w_status), S.nworlds + 1);
+        tmark(ss, "k_zero_u32", st);
         hipLaunchKernelGGL(k_fill_bworld, dim3(S.nworlds), dim3(256), 0, st, S);
+        tmark(ss, "k_fill_bworld", st);
         hipLaunchKernelGGL(k_prep, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_prep", st);
         scan_u32(S.ncell, S.cell_off, nullptr, S.nb, S.nb, &S.counters[C_ENTRIES], S.partials, st);
+        tmark(ss, "scan ncell", st);
         hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_cnt,
                            S.nbuckets + 1);
+        tmark(ss, "k_zero_u32", st);
         hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nbuckets + 1, BS)), dim3(BS), 0, st, S.bkt_fill,
                            S.nbuckets + 1);
+        tmark(ss, "k_zero_u32", st);
         hipLaunchKernelGGL(k_entries, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_entries", st);
         scan_u32(S.bkt_cnt, S.bkt_off, nullptr, S.nbuckets, S.nbuckets, nullptr, S.partials, st);
+        tmark(ss, "scan bkt_cnt", st);
         hipLaunchKernelGGL(k_bucket_fill, dim3(GRID), dim3(BS), 0, st, S);
-        hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(S.nbuckets, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_bucket_fill", st);
+        hipLaunchKernelGGL(k_bucket_rank, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_bucket_rank", st);
         hipLaunchKernelGGL(k_ent_cnt, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_ent_cnt", st);
         scan_u32(S.ent_cnt, S.ent_off, &S.counters[C_ENTRIES], 0, S.cap_e, &S.counters[C_RAW], S.partials, st);
+        tmark(ss, "scan ent_cnt", st);
         hipLaunchKernelGGL(k_cand_emit, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_cand_emit", st);
         break;
     case PH_NARROW:
         hipLaunchKernelGGL(k_narrow, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_narrow", st);
         scan_u32(S.cand_flag, S.cand_idx, &S.counters[C_RAW], 0, S.cap_r, &S.counters[C_MANIFOLDS], S.partials, st);
+        tmark(ss, "scan cand_flag", st);
         hipLaunchKernelGGL(k_man_build, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_man_build", st);
         break;
     case PH_SCHED:
         hipLaunchKernelGGL(k_world_ranges, dim3(blocks_for(S.nworlds + 1, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_world_ranges", st);
         hipLaunchKernelGGL(k_joint_mark, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_joint_mark", st);
         hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg, S.nb + 1);
+        tmark(ss, "k_zero_u32", st);
         hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(S.nb + 1, BS)), dim3(BS), 0, st, S.deg_fill, S.nb + 1);
+        tmark(ss, "k_zero_u32", st);
         hipLaunchKernelGGL(k_items_joints, dim3(blocks_for(S.njoints, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_items_joints", st);
         hipLaunchKernelGGL(k_items_mans, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_items_mans", st);
         scan_u32(S.deg, S.deg_off, nullptr, S.nb, S.nb, nullptr, S.partials, st);
+        tmark(ss, "scan deg", st);
         hipLaunchKernelGGL(k_list_fill, dim3(GRID), dim3(BS), 0, st, S);
+        tmark(ss, "k_list_fill", st);
         hipLaunchKernelGGL(k_list_rank<false>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+        tmark(ss, "k_list_rank<false>", st);
         hipLaunchKernelGGL(k_color, dim3(S.nworlds), dim3(256), 0, st, S);
-        if (sp.any_colored)
+        tmark(ss, "k_color", st);
+        hipLaunchKernelGGL(k_world_order, dim3(1), dim3(1024), 0, st, S);
+        tmark(ss, "k_world_order", st);
+        if (sp.any_colored) {
             hipLaunchKernelGGL(k_list_rank<true>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+            tmark(ss, "k_list_rank<true>", st);
+        }
         break;
     default: {
         // the three solve kernels take disjoint sets of worlds (solve_route);
@@ -1939,15 +2164,21 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             if (run_big) (void)hipStreamWaitEvent(ss.aux[1], ss.fork, 0);
             if (run_lds) (void)hipStreamWaitEvent(ss.aux[0], ss.fork, 0);
         }
-        if (run_big)
+        if (run_big) {
             hipLaunchKernelGGL(v.big_fn, dim3(S.nworlds), dim3(v.big_threads), 0, ss.aux[1], S, dt, sub_steps, iters,
                                sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
-        if (run_lds)
+            tmark(ss, "k_solve (global)", ss.aux[1]);
+        }
+        if (run_lds) {
             hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), solve_lds_bytes((int)sp.lds_maxn),
                                ss.aux[0], S, dt, sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
-        if (run_slot)
+            tmark(ss, "k_solve (record)", ss.aux[0]);
+        }
+        if (run_slot) {
             hipLaunchKernelGGL(v.slot_fn, dim3(S.nworlds), dim3(v.slot_threads), slot_lds_bytes((int)sp.slot_maxn),
                                st, S, dt, sub_steps, iters, sp.slot_nmax);
+            tmark(ss, "k_solve_slots", st);
+        }
         if (run_lds) {
             (void)hipEventRecord(ss.join[0], ss.aux[0]);
             (void)hipStreamWaitEvent(st, ss.join[0], 0);
@@ -1957,6 +2188,7 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             (void)hipStreamWaitEvent(st, ss.join[1], 0);
         }
         hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
+        tmark(ss, "k_commit_step", st);
         break;
     }
     }

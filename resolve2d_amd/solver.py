@@ -54,6 +54,10 @@ class Context:
         N.check(self.lib.r2d_process_batch(self.handle, arr, len(handles), dt, sub_steps, collision_iters, steps),
                 "r2d_process_batch")
 
+    def prepare_batch(self, handles) -> "PreparedBatch":
+        """A reusable handle array for stepping the same worlds many times."""
+        return PreparedBatch(self, handles)
+
     def set_solve_variant(self, variant: int):
         N.check(self.lib.r2d_ctx_set_solve_variant(self.handle, variant), "r2d_ctx_set_solve_variant")
 
@@ -70,6 +74,14 @@ class Context:
         N.check(self.lib.r2d_ctx_phase_times(self.handle, ms, ctypes.byref(steps)), "r2d_ctx_phase_times")
         return list(ms), steps.value
 
+    def step_times(self):
+        """Device time (ms) of every timed step since the last reset."""
+        n = ctypes.c_size_t()
+        N.check(self.lib.r2d_ctx_step_times(self.handle, None, 0, ctypes.byref(n)), "r2d_ctx_step_times")
+        out = (ctypes.c_float * max(n.value, 1))()
+        N.check(self.lib.r2d_ctx_step_times(self.handle, out, n.value, ctypes.byref(n)), "r2d_ctx_step_times")
+        return list(out)[:n.value]
+
     def reset_phase_times(self):
         N.check(self.lib.r2d_ctx_reset_phase_times(self.handle), "r2d_ctx_reset_phase_times")
 
@@ -77,6 +89,20 @@ class Context:
         out = (ctypes.c_uint64 * 4)()
         N.check(self.lib.r2d_ctx_last_counts(self.handle, out), "r2d_ctx_last_counts")
         return dict(entries=out[0], raw_candidates=out[1], manifolds=out[2], items=out[3])
+
+
+class PreparedBatch:
+    """Worlds stepped together by r2d_process_batch, with the C array of
+    their handles built once."""
+
+    def __init__(self, ctx: Context, handles):
+        self.ctx = ctx
+        self.n = len(handles)
+        self.arr = (ctypes.c_void_p * max(self.n, 1))(*handles)
+
+    def process(self, dt: float, sub_steps: int, collision_iters: int, steps: int = 1):
+        N.check(self.ctx.lib.r2d_process_batch(self.ctx.handle, self.arr, self.n, dt, sub_steps, collision_iters,
+                                               steps), "r2d_process_batch")
 
 
 _default_ctx: Optional[Context] = None
