@@ -357,7 +357,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.bkt_items, E * 4); add((void**)&S.bkt_sorted, E * 4);
     add((void**)&S.ent_body, E * 4); add((void**)&S.ent_bkt, E * 4);
     add((void**)&S.ent_cnt, (E + 1) * 4); add((void**)&S.ent_off, (E + 1) * 4);
-    add((void**)&S.cand_a, R * 4); add((void**)&S.cand_b, R * 4); add((void**)&S.cand_k, R * 4);
+    add((void**)&S.cand, R * 16);
     add((void**)&S.cand_flag, (R + 1) * 4); add((void**)&S.cand_idx, (R + 1) * 4);
     add((void**)&S.sat_nx, R * 4); add((void**)&S.sat_ny, R * 4); add((void**)&S.sat_info, R * 4);
     add((void**)&S.m_ref, M * 4); add((void**)&S.m_inc, M * 4);
@@ -696,7 +696,7 @@ void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
 // plus its share of the step scratch at the initial capacity estimates
 // (per body: cell ranges, 4 cell entries, 8 raw candidates, 4 manifolds and
 // 4 solver items; per hash bucket 12 B; per joint one solver item).
-constexpr size_t kScratchPerBody = 88 + 4 * 24 + 8 * 32 + 4 * 128 + 4 * 160;
+constexpr size_t kScratchPerBody = 88 + 4 * 24 + 8 * 36 + 4 * 128 + 4 * 160;
 constexpr size_t kScratchPerBucket = 12;
 constexpr size_t kScratchPerJoint = 160;
 constexpr size_t kScratchPerWorld = 16 + sizeof(WorldDev);

@@ -89,7 +89,7 @@ struct StepDev {
     // entries -> raw candidates
     uint32_t *ent_body, *ent_bkt;  // [cap_e]
     uint32_t *ent_cnt, *ent_off;   // [cap_e+1]
-    uint32_t *cand_a, *cand_b, *cand_k;   // [cap_r] lower body, partner, cell index in the lower body
+    uint4* cand;             // [cap_r] raw candidates: lower body, partner, cell index in the lower body, 0
     uint32_t *cand_flag, *cand_idx;      // [cap_r+1]
     float *sat_nx, *sat_ny;
     uint32_t* sat_info;      // normal id | ref-is-lower-slot << 2 | collides << 3

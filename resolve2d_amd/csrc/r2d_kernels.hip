@@ -223,6 +223,17 @@ struct Edge {
     v2 dir, a, b;
 };
 
+// a[i] for a run-time i with the array kept in registers (selects, not a
+// dynamically indexed private array, which would live in scratch memory)
+__device__ __forceinline__ v2 pick4(const v2 (&a)[4], int i)
+{
+    v2 r = a[0];
+    r = i == 1 ? a[1] : r;
+    r = i == 2 ? a[2] : r;
+    r = i == 3 ? a[3] : r;
+    return r;
+}
+
 // Disc.getNormal (Disc.zig:86-90); Rectangle.getNormal (Rectangle.zig:132-151)
 __device__ __forceinline__ Edge get_normal(const Geo& self, const Geo& other, int iter)
 {
@@ -234,7 +245,7 @@ __device__ __forceinline__ Edge get_normal(const Geo& self, const Geo& other, in
         return e;
     }
     int nx = (iter == 3) ? 0 : iter + 1;
-    v2 a1 = self.wv[iter], a2 = self.wv[nx];
+    v2 a1 = pick4(self.wv, iter), a2 = pick4(self.wv, nx);
     e.dir = rot90ccw(normalize2(sub2(a2, a1)));
     e.a = a1; e.b = a2;
     return e;
@@ -271,11 +282,13 @@ struct Sat {
 };
 
 // overlapSAT (collision.zig:228-289); `which` tags the reference body
-__device__ bool overlap_sat(Sat& ret, const Geo& ref, const Geo& inc, int which)
+__device__ __forceinline__ bool overlap_sat(Sat& ret, const Geo& ref, const Geo& inc, int which)
 {
     const float EPS = SAT_OVERLAP_THRESHOLD;
     int nn = ref.rect ? 4 : 1;
-    for (int it = 0; it < nn; ++it) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        if (it >= nn) break;
         Edge e = get_normal(ref, inc, it);
         v2 normal = e.dir;
         bool flipped = false;
@@ -464,7 +477,7 @@ __global__ void k_cand_emit(StepDev S)
         uint32_t k = e - S.cell_off[a];
         for (uint32_t j = 0, q = 0; q < c; ++j) {
             if (j > 0 && it[j] == it[j - 1]) continue;
-            S.cand_a[o + q] = a; S.cand_b[o + q] = it[j]; S.cand_k[o + q] = k;
+            S.cand[o + q] = make_uint4(a, it[j], k, 0u);   // one 16-byte store per candidate
             ++q;
         }
     }
@@ -479,7 +492,8 @@ __global__ void k_narrow(StepDev S)
     if (S.counters[C_OVERFLOW]) return;
     uint32_t R = dev_count(&S.counters[C_RAW], S.cap_r);
     for (uint32_t q = grid_tid(); q < R; q += grid_stride()) {
-        uint32_t a = S.cand_a[q], b = S.cand_b[q], k = S.cand_k[q];
+        const uint4 cq = S.cand[q];
+        const uint32_t a = cq.x, b = cq.y, k = cq.z;
         uint32_t flag = 0, info = 0;
         float nx = 0.f, ny = 0.f;
         // the pair filters of updateManifolds (lib.zig:273-282) depend on the
@@ -510,8 +524,8 @@ __global__ void k_narrow(StepDev S)
             Geo gb = load_geo(D, sb, S.cc[b], S.cs[b]);
             // performNarrowSAT: lower id is the first reference body
             bool a_first = ga.id < gb.id;
-            const Geo& o1 = a_first ? ga : gb;
-            const Geo& o2 = a_first ? gb : ga;
+            const Geo o1 = a_first ? ga : gb;   // by value: selecting references would put both in scratch
+            const Geo o2 = a_first ? gb : ga;
             Sat sat;
             const float undef = -3.0316488252093987e-13f;   // Zig Debug fill of `undefined` (0xAA)
             sat.normal = mk(undef, undef);
@@ -592,7 +606,8 @@ __global__ void k_man_build(StepDev S)
     for (uint32_t q = grid_tid(); q < R; q += grid_stride()) {
         if (!S.cand_flag[q]) continue;
         uint32_t m = S.cand_idx[q];
-        uint32_t a = S.cand_a[q], b = S.cand_b[q];
+        const uint4 cq = S.cand[q];
+        const uint32_t a = cq.x, b = cq.y;
         uint32_t info = S.sat_info[q];
         int nid = (int)(info & 3u);
         bool ref_is_a = (info & 4u) != 0;
