@@ -1045,13 +1045,17 @@ constexpr uint32_t NONE = 0xffffffffu;
 // words: 0 flags, 1 body a (manifold ref / joint r0), 2 body b (inc / r1),
 // 3 index (manifold in the step arrays / world-local joint), 4-5 ranks at the
 // ordering bodies, 6-8 normal + friction, 9-15 point 0 (r1x r1y r2x r2y depth
-// mass_n mass_t), 16-22 point 1, 24-27 impulses (pn0 pt0 pn1 pt1)
+// mass_n mass_t), 16-22 point 1, 24-27 impulses (pn0 pt0 pn1 pt1), 28 the
+// ordering bodies' item counts, 29-30 the points' Baumgarte biases
 struct Rec {
     uint32_t flags, a, b, idx, rk0, rk1;
     float nx, ny, fric;
     float p[2][7];
     float acc[4];
+    uint32_t degs;       // items at r0 | items at r1 << 16; DEGS_WIDE when either needs more than 16 bits
+    float bias[2];       // Baumgarte bias of the sub-step per point, -1 when depth >= 0
 };
+constexpr uint32_t DEGS_WIDE = 0xffffffffu;
 
 __device__ __forceinline__ void rec_load(Rec& r, const uint4* q)
 {
@@ -1068,6 +1072,9 @@ __device__ __forceinline__ void rec_load(Rec& r, const uint4* q)
     r.p[1][6] = __uint_as_float(v5.z);
     r.acc[0] = __uint_as_float(v6.x); r.acc[1] = __uint_as_float(v6.y);
     r.acc[2] = __uint_as_float(v6.z); r.acc[3] = __uint_as_float(v6.w);
+    uint4 v7 = q[7];
+    r.degs = v7.x;
+    r.bias[0] = __uint_as_float(v7.y); r.bias[1] = __uint_as_float(v7.z);
 }
 
 __device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
@@ -1083,6 +1090,7 @@ __device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
     q[5] = make_uint4(__float_as_uint(r.p[1][4]), __float_as_uint(r.p[1][5]), __float_as_uint(r.p[1][6]), 0u);
     q[6] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
                       __float_as_uint(r.acc[3]));
+    q[7] = make_uint4(r.degs, __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), 0u);
 }
 
 __device__ __forceinline__ int32_t rec_r0(const Rec& r)
@@ -1094,6 +1102,26 @@ __device__ __forceinline__ int32_t rec_r1(const Rec& r)
 {
     if (r.flags & RF_JOINT) return r.b == NONE ? -1 : (int32_t)r.b;
     return (r.flags & RF_W2) ? (int32_t)r.b : -1;
+}
+
+// the ordering bodies' item counts, packed when both fit 16 bits
+template <class ST>
+__device__ __forceinline__ uint32_t rec_degs(const Rec& r, const ST& st)
+{
+    int32_t q0 = rec_r0(r), q1 = rec_r1(r);
+    uint32_t d0 = q0 >= 0 ? st.deg[q0] : 0u, d1 = q1 >= 0 ? st.deg[q1] : 0u;
+    return (d0 > 0xffffu || d1 > 0xffffu) ? DEGS_WIDE : (d0 | (d1 << 16));
+}
+
+// Baumgarte bias of each point for a sub-step of length dt (collision.zig:172),
+// -1 for a point whose depth is >= 0 (calculateImpulses resets and skips it)
+__device__ __forceinline__ void rec_biases(Rec& r, float dt)
+{
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float depth = r.p[p][4];
+        r.bias[p] = depth >= 0.f ? -1.0f : BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
+    }
 }
 
 // CollisionManifold.preStep effective masses (collision.zig:117-132) for one point
@@ -1124,6 +1152,8 @@ __device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldD
     r.rk0 = S.it_k0[item];
     r.rk1 = S.it_k1[item];
     r.nx = r.ny = r.fric = 0.0f;
+    r.degs = 0u;
+    r.bias[0] = r.bias[1] = -1.0f;
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -1240,6 +1270,79 @@ __device__ __forceinline__ void rec_impulses(Rec& r, const ST& st, float dt)
     if (r.flags & RF_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
 }
 
+// one body's solver state as read by a chain link
+struct BodyState {
+    float mx, my, L, im, ii;
+};
+
+__device__ __forceinline__ float lds_relaxed(const float* p)
+{
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
+// a body's state read without ordering: only for LDS, where one workgroup's
+// operations are served in order (see the dataflow loops)
+template <class ST>
+__device__ __forceinline__ BodyState body_state(const ST& st, uint32_t i)
+{
+    return BodyState{lds_relaxed(&st.mx[i]), lds_relaxed(&st.my[i]), lds_relaxed(&st.L[i]), lds_relaxed(&st.im[i]),
+                     lds_relaxed(&st.ii[i])};
+}
+
+// the same with ordinary loads (after the item's counters were acquired)
+template <class ST>
+__device__ __forceinline__ BodyState body_state_ordered(const ST& st, uint32_t i)
+{
+    return BodyState{st.mx[i], st.my[i], st.L[i], st.im[i], st.ii[i]};
+}
+
+// CollisionManifold.calculateImpulses (collision.zig:135-218) on a slot or a
+// record, with both bodies' state already read (the link's only reads of
+// them) and the Baumgarte bias of the sub-step precomputed
+// ((0.02 max(0, -depth - 0.005)) / dt, collision.zig:172; -1 marks depth >= 0)
+template <class Item, class ST>
+__device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t l2, const BodyState& b1,
+                                                 const BodyState& b2, const ST& st)
+{
+    v2 vl1 = scale2(mk(b1.mx, b1.my), b1.im);
+    float om1 = b1.L * b1.ii;
+    v2 vl2 = scale2(mk(b2.mx, b2.my), b2.im);
+    float om2 = b2.L * b2.ii;
+    v2 n = mk(z.nx, z.ny);
+    v2 t = rot90cw(n);
+    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
+    float rot1 = 0.f, rot2 = 0.f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (!(z.flags & (RF_P0 << p))) continue;
+        if (z.bias[p] < 0.f) { z.acc[2 * p] = 0.f; z.acc[2 * p + 1] = 0.f; continue; }
+        v2 r1 = mk(z.p[p][0], z.p[p][1]), r2 = mk(z.p[p][2], z.p[p][3]);
+        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
+        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
+        v2 dv = sub2(v1, v2v);
+        float num = dot2(dv, n) + z.bias[p];
+        float pn = num * z.p[p][5];
+        if (pn < MIN_MANIFOLD_IMPULSE) continue;
+        num = dot2(dv, t);
+        float pt = num * z.p[p][6];
+        float acc_pn = z.acc[2 * p], acc_pt = z.acc[2 * p + 1];
+        float new_pn = maxn(0.0f, acc_pn + pn);
+        float applied_pn = new_pn - acc_pn;
+        acc_pn = new_pn;
+        float max_pt = z.fric * absf(acc_pn);
+        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
+        float applied_pt = new_pt - acc_pt;
+        z.acc[2 * p] = acc_pn;
+        z.acc[2 * p + 1] = new_pt;
+        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
+        if (!(z.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
+        if (!(z.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
+    }
+    if (z.flags & RF_W1) { st.mx[l1] = b1.mx + app1.x; st.my[l1] = b1.my + app1.y; st.L[l1] = b1.L + rot1; }
+    if (z.flags & RF_W2) { st.mx[l2] = b2.mx + app2.x; st.my[l2] = b2.my + app2.y; st.L[l2] = b2.L + rot2; }
+}
+
 // record -> the step's manifold arrays (what r2d_get_manifolds reports)
 __device__ __forceinline__ void rec_writeback(const Rec& r, const StepDev& S)
 {
@@ -1337,6 +1440,8 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
         Rec r;
         rec_build(r, S, D, st, pos, ibase, nj, mbeg, base);
+        r.degs = rec_degs(r, st);
+        if (!(r.flags & RF_JOINT)) rec_biases(r, sub_dt);
         rec_store(r, recs + (size_t)pos * 8);
     }
 
@@ -1399,6 +1504,7 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
                 Rec r;
                 rec_load(r, q);
                 rec_tgs(r, S, D, st);
+                rec_biases(r, sub_dt);
                 rec_store(r, q);
             } else if (s + 1 == sub_steps) {
                 S.m_prev1[h.w] = D.ang[h.y];
@@ -1406,7 +1512,12 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
             }
         }
         __syncthreads();
-        // K iterations of (constraints, manifolds) as one ordered dataflow
+        // K iterations of (constraints, manifolds) as one ordered dataflow.  With
+        // LDS body state a link's counters and its bodies' state are read
+        // together (LDS serves one workgroup's operations in order, so state read
+        // after a counter that shows the item ready is what its predecessor
+        // released; values read for an item that is not ready are discarded);
+        // with body state in global memory the state is read after an acquire.
         {
             const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
             uint32_t pcur = p_begin, kcur = (p_begin < nitems) ? 0u : iters;
@@ -1423,7 +1534,6 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
             uint32_t t0 = 0, t1 = 0;
             uint32_t spins = 0;
             while (true) {
-                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 if (!have) {
                     if (!has_nxt) break;
                     cur = nxt;
@@ -1442,37 +1552,71 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
                     }
                     r0 = rec_r0(cur);
                     r1 = rec_r1(cur);
-                    if (r0 >= 0) t0 = cur_k * st.deg[r0] + cur.rk0;
-                    if (r1 >= 0) t1 = cur_k * st.deg[r1] + cur.rk1;
+                    if (cur.degs != DEGS_WIDE) {
+                        t0 = cur_k * (cur.degs & 0xffffu) + cur.rk0;
+                        t1 = cur_k * (cur.degs >> 16) + cur.rk1;
+                    } else {
+                        if (r0 >= 0) t0 = cur_k * st.deg[r0] + cur.rk0;
+                        if (r1 >= 0) t1 = cur_k * st.deg[r1] + cur.rk1;
+                    }
                 }
-                bool ready = true;
-                if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
-                if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
+                const bool contact = !(cur.flags & RF_JOINT);
+                BodyState b1{}, b2{};
+                bool ready;
+                if (LDS) {
+                    uint32_t c0 = r0 >= 0 ? __hip_atomic_load(&st.cnt[r0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                          : t0;
+                    uint32_t c1 = r1 >= 0 ? __hip_atomic_load(&st.cnt[r1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                          : t1;
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (contact) {
+                        b1 = body_state(st, cur.a);
+                        b2 = body_state(st, cur.b);
+                    }
+                    ready = c0 == t0 && c1 == t1;
+                } else {
+                    ready = true;
+                    if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
+                    if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
+                    if (ready && contact) {
+                        b1 = body_state_ordered(st, cur.a);
+                        b2 = body_state_ordered(st, cur.b);
+                    }
+                }
                 if (ready) {
-                    if (cur.flags & RF_JOINT) {
+                    if (!contact) {
                         solve_joint(D, st, D.joints[cur.idx], sub_dt);
                     } else {
-                        rec_impulses(cur, st, sub_dt);
+                        contact_impulses(cur, cur.a, cur.b, b1, b2, st);
                         recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
                                                                    __float_as_uint(cur.acc[1]),
                                                                    __float_as_uint(cur.acc[2]),
                                                                    __float_as_uint(cur.acc[3]));
                     }
-                    if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
-                    if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                    if (LDS) {
+                        // the state writes above reach LDS before the counters (in order)
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
+                        if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                    }
                     if (has_nxt && nxt_pos == cur_pos) nxt = cur;
                     have = false;
                     spins = 0;
                 } else {
                     // every item's predecessors precede it, so some item is always
                     // ready; a stall this long means corrupt ordering data: stop
+                    if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                     if (++spins > SPIN_LIMIT) {
                         __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         S.w_status[w] = 5;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
                 }
+                // a wave with no lane ready yields its SIMD
+                if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
             }
         }
         __syncthreads();
@@ -1655,68 +1799,6 @@ __device__ __forceinline__ void slot_solve_joint(const Slot& z, const SlotState&
         Jv = clampn(Jv, pmin * den, pmax * den);
         st.L[a] += Jv * dt;
     }
-}
-
-// one body's solver state as read by a chain link
-struct BodyState {
-    float mx, my, L, im, ii;
-};
-
-__device__ __forceinline__ float lds_relaxed(const float* p)
-{
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP));
-}
-
-__device__ __forceinline__ BodyState body_state(const SlotState& st, uint32_t i)
-{
-    return BodyState{lds_relaxed(&st.mx[i]), lds_relaxed(&st.my[i]), lds_relaxed(&st.L[i]), lds_relaxed(&st.im[i]),
-                     lds_relaxed(&st.ii[i])};
-}
-
-// CollisionManifold.calculateImpulses (collision.zig:135-218) on a slot, with
-// both bodies' state already read (the link's only reads of them) and the
-// Baumgarte bias of the sub-step precomputed ((0.02 max(0, -depth - 0.005)) / dt,
-// collision.zig:172; -1 marks depth >= 0)
-__device__ __forceinline__ void slot_contact(Slot& z, const BodyState& b1, const BodyState& b2, const SlotState& st)
-{
-    const uint32_t l1 = z.pair & 0xffffu, l2 = z.pair >> 16;
-    v2 vl1 = scale2(mk(b1.mx, b1.my), b1.im);
-    float om1 = b1.L * b1.ii;
-    v2 vl2 = scale2(mk(b2.mx, b2.my), b2.im);
-    float om2 = b2.L * b2.ii;
-    v2 n = mk(z.nx, z.ny);
-    v2 t = rot90cw(n);
-    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
-    float rot1 = 0.f, rot2 = 0.f;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        if (!(z.flags & (RF_P0 << p))) continue;
-        if (z.bias[p] < 0.f) { z.acc[2 * p] = 0.f; z.acc[2 * p + 1] = 0.f; continue; }
-        v2 r1 = mk(z.p[p][0], z.p[p][1]), r2 = mk(z.p[p][2], z.p[p][3]);
-        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
-        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
-        v2 dv = sub2(v1, v2v);
-        float num = dot2(dv, n) + z.bias[p];
-        float pn = num * z.p[p][5];
-        if (pn < MIN_MANIFOLD_IMPULSE) continue;
-        num = dot2(dv, t);
-        float pt = num * z.p[p][6];
-        float acc_pn = z.acc[2 * p], acc_pt = z.acc[2 * p + 1];
-        float new_pn = maxn(0.0f, acc_pn + pn);
-        float applied_pn = new_pn - acc_pn;
-        acc_pn = new_pn;
-        float max_pt = z.fric * absf(acc_pn);
-        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
-        float applied_pt = new_pt - acc_pt;
-        z.acc[2 * p] = acc_pn;
-        z.acc[2 * p + 1] = new_pt;
-        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
-        if (!(z.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
-        if (!(z.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
-    }
-    if (z.flags & RF_W1) { st.mx[l1] = b1.mx + app1.x; st.my[l1] = b1.my + app1.y; st.L[l1] = b1.L + rot1; }
-    if (z.flags & RF_W2) { st.mx[l2] = b2.mx + app2.x; st.my[l2] = b2.my + app2.y; st.L[l2] = b2.L + rot2; }
 }
 
 // ordering bodies of a slot (-1: none) and the Baumgarte biases of a sub-step
@@ -1908,7 +1990,7 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
                     if (!contact) {
                         slot_solve_joint(cur, st, sub_dt);
                     } else {
-                        slot_contact(cur, b1, b2, st);
+                        contact_impulses(cur, cur.pair & 0xffffu, cur.pair >> 16, b1, b2, st);
 #pragma unroll
                         for (int z = 0; z < NSLOT; ++z)
                             if ((uint32_t)z == cz)
