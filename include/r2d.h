@@ -211,7 +211,7 @@ int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
 int r2d_ctx_step_times(r2d_ctx* ctx, float* out, size_t cap, size_t* n);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
 /* Launch configuration of the solve kernel: -1 (default) is variant 1;
- * 0..7 select the register-slot kernel's slots x workgroup size and the
+ * 0..8 select the register-slot kernel's slots x workgroup size and the
  * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner
  * (resolve2d_amd/autotune.py) picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _native as N
 
-SOLVE_VARIANTS = 8
+SOLVE_VARIANTS = 9
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "resolve2d_amd", "autotune.json")
 SOLVE_PHASE = 3
 

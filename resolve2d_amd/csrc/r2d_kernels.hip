@@ -1823,7 +1823,12 @@ __device__ __forceinline__ void slot_biases(Slot& z, float dt)
     }
 }
 
-template <int NSLOT, int MAXT, int WAVES>
+// LOCKSTEP: a wave takes its lanes' slot z together, z = 0, 1, ... in turn
+// (each slot's registers addressed directly, no per-lane selection of the
+// current slot), instead of every lane walking its own slots independently.
+// Both orders are deadlock-free: a wave works on positions in increasing
+// (iteration, slot, lane) order, the order of the dependencies.
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP = false>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
 {
@@ -1953,6 +1958,57 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
             }
             slot_biases(sl[z], sub_dt);
         }
+        if constexpr (LOCKSTEP) {
+            // lockstep over slots: see the kernel's comment; reads as in the
+            // independent walk below
+            for (uint32_t k = 0; k < iters; ++k) {
+#pragma unroll
+                for (int z = 0; z < NSLOT; ++z) {
+                    Slot& it = sl[z];
+                    bool done = !it.flags;
+                    int32_t r0 = -1, r1 = -1;
+                    slot_ordering(it, r0, r1);
+                    const uint32_t t0 = k * (it.degs & 0xffffu) + (it.ranks & 0xffffu);
+                    const uint32_t t1 = k * (it.degs >> 16) + (it.ranks >> 16);
+                    const bool contact = !(it.flags & RF_JOINT);
+                    uint32_t spins = 0;
+                    while (!done) {
+                        uint32_t c0 = r0 >= 0 ? __hip_atomic_load(&st.cnt[r0], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
+                                              : t0;
+                        uint32_t c1 = r1 >= 0 ? __hip_atomic_load(&st.cnt[r1], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
+                                              : t1;
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        BodyState b1{}, b2{};
+                        if (contact) {
+                            b1 = body_state(st, it.pair & 0xffffu);
+                            b2 = body_state(st, it.pair >> 16);
+                        }
+                        const bool ready = c0 == t0 && c1 == t1;
+                        if (ready) {
+                            if (!contact) slot_solve_joint(it, st, sub_dt);
+                            else contact_impulses(it, it.pair & 0xffffu, it.pair >> 16, b1, b2, st);
+                            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                            if (r0 >= 0)
+                                __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (r1 >= 0)
+                                __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            done = true;
+                        } else {
+                            if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                                done = true;
+                            } else if (++spins > SPIN_LIMIT) {
+                                __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                S.w_status[w] = 5;
+                                done = true;
+                            }
+                        }
+                        if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+            }
+        } else
         // K iterations as an ordered dataflow over register-resident items.  A
         // link's counters and its bodies' state are read together: LDS serves
         // one workgroup's operations in order, so state read after a counter
@@ -2092,6 +2148,7 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
     {nullptr, 0, 0, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
     {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 128, 2>, 128, k_solve<false, 128, 2>, 128},
+    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
 };
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }

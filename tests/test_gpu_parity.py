@@ -90,7 +90,7 @@ def test_batched_equals_single(gpu_ctx, oracle):
         w.close()
 
 
-@pytest.mark.parametrize("variant", list(range(8)))
+@pytest.mark.parametrize("variant", list(range(9)))
 def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
     """Every launch configuration of the solve kernel (workgroup size per
     world) gives the oracle's bits."""
