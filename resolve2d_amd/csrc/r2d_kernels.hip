@@ -1022,12 +1022,6 @@ __device__ __noinline__ void solve_joint(const WorldDev& D, const ST st, const J
     }
 }
 
-// One workgroup steps one world: S sub-steps of forces, integration,
-// pre-step and K solver iterations (lib.zig:199-250).  Solver items are run
-// as a dataflow: an item may run once every earlier item (in the reference's
-// order) that touches one of its ordering bodies has run, which gives the
-// same bits as the sequential loop.
-
 // ------------------------------------------------------- solver item records
 // Each solver item of a step gets one 128-byte record at its execution
 // position, built in parallel at the first sub-step: the ordering bodies and
@@ -1382,6 +1376,245 @@ __device__ __forceinline__ int solve_route(uint32_t n, uint32_t nitems, uint32_t
     return 2;
 }
 
+// ---------------------------------------------- solve building blocks
+// Per-body and per-item parts of Solver.process's sub-step loop
+// (lib.zig:199-250), shared by the one-workgroup-per-world kernels and by the
+// grid-wide kernels used for worlds whose body state does not fit LDS.
+
+// inverse mass / inertia (static: 0) and item count of body i
+template <class ST>
+__device__ __forceinline__ void body_setup(const StepDev& S, const WorldDev& D, const ST& st, int i, bool copy_state)
+{
+    const uint32_t base = D.body_base;
+    bool stat = (D.flags[i] & BF_STATIC) != 0;
+    if (copy_state) { st.mx[i] = D.mx[i]; st.my[i] = D.my[i]; st.L[i] = D.L[i]; }
+    st.im[i] = stat ? 0.0f : 1.0f / D.mass[i];
+    st.ii[i] = stat ? 0.0f : 1.0f / D.inertia[i];
+    st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
+}
+
+// forces (lib.zig:200-205), AABB refresh + velocity integration
+// (lib.zig:207-216) for body i; clears its solver counter
+template <class ST>
+__device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int i, bool last, float sub_dt)
+{
+    uint32_t f = D.flags[i];
+    bool stat = (f & BF_STATIC) != 0;
+    float fx = D.fx[i], fy = D.fy[i];
+    for (int g = 0; g < D.ngens; ++g) {
+        const Gen& G = D.gens[g];
+        if (stat) continue;
+        if (G.type == 0) {
+            fx += G.gx * D.mass[i];
+            fy += G.gy * D.mass[i];
+        } else if (G.slot == i) {
+            float inv_m = 1.0f / D.mass[i];
+            v2 v = scale2(mk(st.mx[i], st.my[i]), inv_m);
+            v2 d = sub2(mk(D.px[i], D.py[i]), mk(G.ax, G.ay));
+            fx += (-G.k) * d.x - G.c * v.x;
+            fy += (-G.k) * d.y - G.c * v.y;
+        }
+    }
+    if (last) {
+        // Disc.updateAABB / Rectangle.updateAABB; only the last refresh is observable
+        float a = D.ang[i];
+        float hw, hh;
+        if (f & BF_RECT) {
+            float c = cosf_r(a), sn = sinf_r(a);
+            float x = D.hw[i], y = D.hh[i];
+            v2 lv[4] = {mk(-x, -y), mk(-x, y), mk(x, y), mk(x, -y)};
+            hw = 0.f; hh = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                v2 r = rotcs(lv[k], c, sn);
+                if (r.x > hw) hw = r.x;
+                if (r.y > hh) hh = r.y;
+            }
+        } else {
+            hw = D.rad[i]; hh = D.rad[i];
+        }
+        D.ax[i] = D.px[i]; D.ay[i] = D.py[i]; D.ahw[i] = hw; D.ahh[i] = hh;
+    }
+    if (!stat) {
+        st.mx[i] += fx * sub_dt;
+        st.my[i] += fy * sub_dt;
+        st.L[i] += D.tq[i] * sub_dt;
+    }
+    D.fx[i] = fx; D.fy[i] = fy;
+    st.cnt[i] = 0u;
+}
+
+// position integration (lib.zig:238-249) for body i
+template <class ST>
+__device__ __forceinline__ void body_positions(const WorldDev& D, const ST& st, int i, float sub_dt)
+{
+    if (D.flags[i] & BF_STATIC) return;
+    float qd = sub_dt / D.mass[i];
+    D.px[i] += st.mx[i] * qd;
+    D.py[i] += st.my[i] * qd;
+    D.ang[i] += st.L[i] * sub_dt / D.inertia[i];
+    D.fx[i] = 0.0f; D.fy[i] = 0.0f;
+    D.tq[i] = 0.0f;
+}
+
+// the record of execution position pos (first sub-step's preStep)
+template <class ST>
+__device__ __forceinline__ void item_build(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs, uint32_t pos,
+                                           uint32_t ibase, uint32_t mbeg, float sub_dt)
+{
+    Rec r;
+    rec_build(r, S, D, st, pos, ibase, (uint32_t)D.njoints, mbeg, D.body_base);
+    r.degs = rec_degs(r, st);
+    if (!(r.flags & RF_JOINT)) rec_biases(r, sub_dt);
+    rec_store(r, recs + (size_t)pos * 8);
+}
+
+// updateTGSDepth (collision.zig:73-100): live only for records flagged at
+// build time; every manifold's previous angles end as the last sub-step's
+template <class ST>
+__device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs, uint32_t pos,
+                                         bool last, float sub_dt)
+{
+    uint4* q = recs + (size_t)pos * 8;
+    uint4 h = q[0];
+    if (h.x & RF_JOINT) return;
+    if (h.x & RF_TGS) {
+        Rec r;
+        rec_load(r, q);
+        rec_tgs(r, S, D, st);
+        rec_biases(r, sub_dt);
+        rec_store(r, q);
+    } else if (last) {
+        S.m_prev1[h.w] = D.ang[h.y];
+        S.m_prev2[h.w] = D.ang[h.z];
+    }
+}
+
+// K iterations of one sub-step's (constraints, manifolds) for the world of
+// this workgroup, as an ordered dataflow
+template <bool LDS, class ST>
+__device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs,
+                                           uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
+                                           uint32_t& s_abort)
+{
+    // K iterations of (constraints, manifolds) as one ordered dataflow.  With
+    // LDS body state a link's counters and its bodies' state are read
+    // together (LDS serves one workgroup's operations in order, so state read
+    // after a counter that shows the item ready is what its predecessor
+    // released; values read for an item that is not ready are discarded);
+    // with body state in global memory the state is read after an acquire.
+    {
+        const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
+        uint32_t pcur = p_begin, kcur = (p_begin < nitems) ? 0u : iters;
+        Rec cur, nxt;
+        uint32_t cur_pos = 0, cur_k = 0, nxt_pos = 0, nxt_k = 0;
+        bool have = false, has_nxt = false;
+        if (kcur < iters) {
+            rec_load(nxt, recs + (size_t)pcur * 8);
+            nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
+            pcur += p_step;
+            if (pcur >= nitems) { pcur = p_begin; ++kcur; }
+        }
+        int32_t r0 = -1, r1 = -1;
+        uint32_t t0 = 0, t1 = 0;
+        uint32_t spins = 0;
+        while (true) {
+            if (!have) {
+                if (!has_nxt) break;
+                cur = nxt;
+                cur_pos = nxt_pos;
+                cur_k = nxt_k;
+                has_nxt = false;
+                have = true;
+                // prefetch the following position; the same position in the
+                // next iteration reads impulses this record is about to write,
+                // so that case is forwarded from registers instead
+                if (kcur < iters) {
+                    if (pcur != cur_pos) rec_load(nxt, recs + (size_t)pcur * 8);
+                    nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
+                    pcur += p_step;
+                    if (pcur >= nitems) { pcur = p_begin; ++kcur; }
+                }
+                r0 = rec_r0(cur);
+                r1 = rec_r1(cur);
+                if (cur.degs != DEGS_WIDE) {
+                    t0 = cur_k * (cur.degs & 0xffffu) + cur.rk0;
+                    t1 = cur_k * (cur.degs >> 16) + cur.rk1;
+                } else {
+                    if (r0 >= 0) t0 = cur_k * st.deg[r0] + cur.rk0;
+                    if (r1 >= 0) t1 = cur_k * st.deg[r1] + cur.rk1;
+                }
+            }
+            const bool contact = !(cur.flags & RF_JOINT);
+            BodyState b1{}, b2{};
+            bool ready;
+            if (LDS) {
+                uint32_t c0 = r0 >= 0 ? __hip_atomic_load(&st.cnt[r0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                      : t0;
+                uint32_t c1 = r1 >= 0 ? __hip_atomic_load(&st.cnt[r1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                      : t1;
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (contact) {
+                    b1 = body_state(st, cur.a);
+                    b2 = body_state(st, cur.b);
+                }
+                ready = c0 == t0 && c1 == t1;
+            } else {
+                ready = true;
+                if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
+                if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
+                if (ready && contact) {
+                    b1 = body_state_ordered(st, cur.a);
+                    b2 = body_state_ordered(st, cur.b);
+                }
+            }
+            if (ready) {
+                if (!contact) {
+                    solve_joint(D, st, D.joints[cur.idx], sub_dt);
+                } else {
+                    contact_impulses(cur, cur.a, cur.b, b1, b2, st);
+                    recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
+                                                               __float_as_uint(cur.acc[1]),
+                                                               __float_as_uint(cur.acc[2]),
+                                                               __float_as_uint(cur.acc[3]));
+                }
+                if (LDS) {
+                    // the state writes above reach LDS before the counters (in order)
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
+                    if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
+                }
+                if (has_nxt && nxt_pos == cur_pos) nxt = cur;
+                have = false;
+                spins = 0;
+            } else {
+                // every item's predecessors precede it, so some item is always
+                // ready; a stall this long means corrupt ordering data: stop
+                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                if (++spins > SPIN_LIMIT) {
+                    __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    S.w_status[w] = 5;
+                    break;
+                }
+            }
+            // a wave with no lane ready yields its SIMD
+            if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// the records' impulses, effective masses, lever arms and depths -> the
+// step's manifold arrays (what r2d_get_manifolds reports)
+__device__ __forceinline__ void item_writeback(const StepDev& S, uint4* recs, uint32_t pos)
+{
+    Rec r;
+    rec_load(r, recs + (size_t)pos * 8);
+    rec_writeback(r, S);
+}
+
 // One workgroup steps one world: S sub-steps of forces, integration,
 // pre-step and K solver iterations (lib.zig:199-250).  Solver items run as a
 // dataflow: an item may run once every earlier item (in the schedule's order)
@@ -1406,12 +1639,10 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     const WorldDev& D = S.worlds[w];
     const int n = D.nbodies;
     const uint32_t base = D.body_base;
-    const uint32_t nj = D.njoints;
     const uint32_t mbeg = S.w_man_begin[w];
     const uint32_t ibase = S.w_item_base[w];
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
-    const float fsub = (float)sub_steps;
-    const float sub_dt = dt / fsub;
+    const float sub_dt = dt / (float)sub_steps;
     uint4* recs = S.recs + (size_t)ibase * 8;
 
     SolverState<LDS> st;
@@ -1428,220 +1659,98 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
         st.im = S.g_im + base; st.ii = S.g_ii + base;
         st.cnt = S.g_cnt + base; st.deg = S.deg + base;
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        bool stat = (D.flags[i] & BF_STATIC) != 0;
-        if (LDS) { st.mx[i] = D.mx[i]; st.my[i] = D.my[i]; st.L[i] = D.L[i]; }
-        st.im[i] = stat ? 0.0f : 1.0f / D.mass[i];
-        st.ii[i] = stat ? 0.0f : 1.0f / D.inertia[i];
-        st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
-    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) body_setup(S, D, st, i, LDS);
     __syncthreads();
-    // records (first sub-step's preStep for every manifold)
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
-        Rec r;
-        rec_build(r, S, D, st, pos, ibase, nj, mbeg, base);
-        r.degs = rec_degs(r, st);
-        if (!(r.flags & RF_JOINT)) rec_biases(r, sub_dt);
-        rec_store(r, recs + (size_t)pos * 8);
-    }
-
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_build(S, D, st, recs, pos, ibase, mbeg, sub_dt);
     for (uint32_t s = 0; s < sub_steps; ++s) {
-        // forces (lib.zig:200-205), AABB refresh + velocity integration (lib.zig:207-216)
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            uint32_t f = D.flags[i];
-            bool stat = (f & BF_STATIC) != 0;
-            float fx = D.fx[i], fy = D.fy[i];
-            for (int g = 0; g < D.ngens; ++g) {
-                const Gen& G = D.gens[g];
-                if (stat) continue;
-                if (G.type == 0) {
-                    fx += G.gx * D.mass[i];
-                    fy += G.gy * D.mass[i];
-                } else if (G.slot == i) {
-                    float inv_m = 1.0f / D.mass[i];
-                    v2 v = scale2(mk(st.mx[i], st.my[i]), inv_m);
-                    v2 d = sub2(mk(D.px[i], D.py[i]), mk(G.ax, G.ay));
-                    fx += (-G.k) * d.x - G.c * v.x;
-                    fy += (-G.k) * d.y - G.c * v.y;
-                }
-            }
-            if (s + 1 == sub_steps) {
-                // Disc.updateAABB / Rectangle.updateAABB; only the last refresh is observable
-                float a = D.ang[i];
-                float hw, hh;
-                if (f & BF_RECT) {
-                    float c = cosf_r(a), sn = sinf_r(a);
-                    float x = D.hw[i], y = D.hh[i];
-                    v2 lv[4] = {mk(-x, -y), mk(-x, y), mk(x, y), mk(x, -y)};
-                    hw = 0.f; hh = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        v2 r = rotcs(lv[k], c, sn);
-                        if (r.x > hw) hw = r.x;
-                        if (r.y > hh) hh = r.y;
-                    }
-                } else {
-                    hw = D.rad[i]; hh = D.rad[i];
-                }
-                D.ax[i] = D.px[i]; D.ay[i] = D.py[i]; D.ahw[i] = hw; D.ahh[i] = hh;
-            }
-            if (!stat) {
-                st.mx[i] += fx * sub_dt;
-                st.my[i] += fy * sub_dt;
-                st.L[i] += D.tq[i] * sub_dt;
-            }
-            D.fx[i] = fx; D.fy[i] = fy;
-            st.cnt[i] = 0u;
-        }
+        const bool last = s + 1 == sub_steps;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces(D, st, i, last, sub_dt);
         __syncthreads();
-        // updateTGSDepth (collision.zig:73-100): live only for records flagged at
-        // build time; every manifold's previous angles end as the last sub-step's
-        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
-            uint4* q = recs + (size_t)pos * 8;
-            uint4 h = q[0];
-            if (h.x & RF_JOINT) continue;
-            if (h.x & RF_TGS) {
-                Rec r;
-                rec_load(r, q);
-                rec_tgs(r, S, D, st);
-                rec_biases(r, sub_dt);
-                rec_store(r, q);
-            } else if (s + 1 == sub_steps) {
-                S.m_prev1[h.w] = D.ang[h.y];
-                S.m_prev2[h.w] = D.ang[h.z];
-            }
-        }
+        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
         __syncthreads();
-        // K iterations of (constraints, manifolds) as one ordered dataflow.  With
-        // LDS body state a link's counters and its bodies' state are read
-        // together (LDS serves one workgroup's operations in order, so state read
-        // after a counter that shows the item ready is what its predecessor
-        // released; values read for an item that is not ready are discarded);
-        // with body state in global memory the state is read after an acquire.
-        {
-            const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
-            uint32_t pcur = p_begin, kcur = (p_begin < nitems) ? 0u : iters;
-            Rec cur, nxt;
-            uint32_t cur_pos = 0, cur_k = 0, nxt_pos = 0, nxt_k = 0;
-            bool have = false, has_nxt = false;
-            if (kcur < iters) {
-                rec_load(nxt, recs + (size_t)pcur * 8);
-                nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
-                pcur += p_step;
-                if (pcur >= nitems) { pcur = p_begin; ++kcur; }
-            }
-            int32_t r0 = -1, r1 = -1;
-            uint32_t t0 = 0, t1 = 0;
-            uint32_t spins = 0;
-            while (true) {
-                if (!have) {
-                    if (!has_nxt) break;
-                    cur = nxt;
-                    cur_pos = nxt_pos;
-                    cur_k = nxt_k;
-                    has_nxt = false;
-                    have = true;
-                    // prefetch the following position; the same position in the
-                    // next iteration reads impulses this record is about to write,
-                    // so that case is forwarded from registers instead
-                    if (kcur < iters) {
-                        if (pcur != cur_pos) rec_load(nxt, recs + (size_t)pcur * 8);
-                        nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
-                        pcur += p_step;
-                        if (pcur >= nitems) { pcur = p_begin; ++kcur; }
-                    }
-                    r0 = rec_r0(cur);
-                    r1 = rec_r1(cur);
-                    if (cur.degs != DEGS_WIDE) {
-                        t0 = cur_k * (cur.degs & 0xffffu) + cur.rk0;
-                        t1 = cur_k * (cur.degs >> 16) + cur.rk1;
-                    } else {
-                        if (r0 >= 0) t0 = cur_k * st.deg[r0] + cur.rk0;
-                        if (r1 >= 0) t1 = cur_k * st.deg[r1] + cur.rk1;
-                    }
-                }
-                const bool contact = !(cur.flags & RF_JOINT);
-                BodyState b1{}, b2{};
-                bool ready;
-                if (LDS) {
-                    uint32_t c0 = r0 >= 0 ? __hip_atomic_load(&st.cnt[r0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                          : t0;
-                    uint32_t c1 = r1 >= 0 ? __hip_atomic_load(&st.cnt[r1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                          : t1;
-                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                    if (contact) {
-                        b1 = body_state(st, cur.a);
-                        b2 = body_state(st, cur.b);
-                    }
-                    ready = c0 == t0 && c1 == t1;
-                } else {
-                    ready = true;
-                    if (r0 >= 0) ready = ld_acq(&st.cnt[r0]) == t0;
-                    if (ready && r1 >= 0) ready = ld_acq(&st.cnt[r1]) == t1;
-                    if (ready && contact) {
-                        b1 = body_state_ordered(st, cur.a);
-                        b2 = body_state_ordered(st, cur.b);
-                    }
-                }
-                if (ready) {
-                    if (!contact) {
-                        solve_joint(D, st, D.joints[cur.idx], sub_dt);
-                    } else {
-                        contact_impulses(cur, cur.a, cur.b, b1, b2, st);
-                        recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
-                                                                   __float_as_uint(cur.acc[1]),
-                                                                   __float_as_uint(cur.acc[2]),
-                                                                   __float_as_uint(cur.acc[3]));
-                    }
-                    if (LDS) {
-                        // the state writes above reach LDS before the counters (in order)
-                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                        if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    } else {
-                        if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
-                        if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
-                    }
-                    if (has_nxt && nxt_pos == cur_pos) nxt = cur;
-                    have = false;
-                    spins = 0;
-                } else {
-                    // every item's predecessors precede it, so some item is always
-                    // ready; a stall this long means corrupt ordering data: stop
-                    if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                    if (++spins > SPIN_LIMIT) {
-                        __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        S.w_status[w] = 5;
-                        break;
-                    }
-                }
-                // a wave with no lane ready yields its SIMD
-                if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
-            }
-        }
+        item_chain<LDS>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
-        // position integration (lib.zig:238-249)
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            if (D.flags[i] & BF_STATIC) continue;
-            float qd = sub_dt / D.mass[i];
-            D.px[i] += st.mx[i] * qd;
-            D.py[i] += st.my[i] * qd;
-            D.ang[i] += st.L[i] * sub_dt / D.inertia[i];
-            D.fx[i] = 0.0f; D.fy[i] = 0.0f;
-            D.tq[i] = 0.0f;
-        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
         __syncthreads();
     }
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
-        Rec r;
-        rec_load(r, recs + (size_t)pos * 8);
-        rec_writeback(r, S);
-    }
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_writeback(S, recs, pos);
     if (LDS) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
         }
     }
+}
+
+// ------------------------------------- worlds beyond LDS: split sub-steps
+// For a world whose body state does not fit LDS (more than lds_nmax bodies)
+// the per-body and per-item parts run as grid-wide kernels over all such
+// worlds' bodies, and only the dependency chains run one workgroup per world;
+// body state, counters and records stay in global memory between launches.
+__device__ __forceinline__ bool big_world(const WorldDev& D, uint32_t lds_nmax)
+{
+    return (uint32_t)D.nbodies > lds_nmax;
+}
+
+__device__ __forceinline__ SolverState<false> global_state(const StepDev& S, const WorldDev& D)
+{
+    SolverState<false> st;
+    st.mx = D.mx; st.my = D.my; st.L = D.L;
+    st.im = S.g_im + D.body_base; st.ii = S.g_ii + D.body_base;
+    st.cnt = S.g_cnt + D.body_base; st.deg = S.deg + D.body_base;
+    return st;
+}
+
+// phase 0: body setup; 1: forces + velocities (sub-step s); 2: positions
+__global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s, uint32_t sub_steps, float sub_dt)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    for (uint32_t x = grid_tid(); x < S.nb; x += grid_stride()) {
+        const uint32_t w = S.bworld[x];
+        if (S.w_status[w] != 0) continue;
+        const WorldDev& D = S.worlds[w];
+        if (!big_world(D, lds_nmax)) continue;
+        const SolverState<false> st = global_state(S, D);
+        const int i = (int)(x - D.body_base);
+        if (phase == 0) body_setup(S, D, st, i, false);
+        else if (phase == 1) body_forces(D, st, i, s + 1 == sub_steps, sub_dt);
+        else body_positions(D, st, i, sub_dt);
+    }
+}
+
+// phase 0: build records; 1: updateTGSDepth (sub-step s); 2: write back.
+// One workgroup per world.
+__global__ void k_big_items(StepDev S, uint32_t lds_nmax, int phase, uint32_t s, uint32_t sub_steps, float sub_dt)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t w = blockIdx.x;
+    if (S.w_status[w] != 0) return;
+    const WorldDev& D = S.worlds[w];
+    if (!big_world(D, lds_nmax)) return;
+    const SolverState<false> st = global_state(S, D);
+    const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase;
+    uint4* recs = S.recs + (size_t)ibase * 8;
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
+        if (phase == 0) item_build(S, D, st, recs, pos, ibase, S.w_man_begin[w], sub_dt);
+        else if (phase == 1) item_tgs(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
+        else item_writeback(S, recs, pos);
+    }
+}
+
+// one sub-step's K iterations; one workgroup per world
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax, uint32_t iters, float sub_dt)
+{
+    __shared__ uint32_t s_abort;
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t w = S.w_order[blockIdx.x];
+    if (S.w_status[w] != 0) return;
+    const WorldDev& D = S.worlds[w];
+    if (!big_world(D, lds_nmax)) return;
+    if (threadIdx.x == 0) s_abort = 0u;
+    __syncthreads();
+    const SolverState<false> st = global_state(S, D);
+    const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase;
+    item_chain<false>(S, D, st, S.recs + (size_t)ibase * 8, nitems, iters, sub_dt, w, s_abort);
 }
 
 // ------------------------------------------------------ register-slot solver
@@ -2131,8 +2240,6 @@ struct SolveVariant {
     int slots;
     RecFn rec_fn;        // record kernel (LDS body state) for the worlds the slot kernel leaves
     unsigned threads;
-    RecFn big_fn;        // record kernel with body state in global memory (worlds beyond LDS)
-    unsigned big_threads;
 };
 // solve variants: the register-slot kernel's slots x workgroup size, and the
 // record kernel's workgroup size for worlds with more items than slots
@@ -2140,15 +2247,15 @@ struct SolveVariant {
 // SIMD with the world's other waves: 256 threads beat 1,024 by about 3x on a
 // 4,096-body world)
 static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
-    {nullptr, 0, 0, k_solve<true, 1024>, 1024, k_solve<false, 1024>, 1024},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {nullptr, 0, 0, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 128, 2>, 128, k_solve<false, 128, 2>, 128},
-    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<true, 256, 2>, 256, k_solve<false, 256, 2>, 256},
+    {nullptr, 0, 0, k_solve<true, 1024>, 1024},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256},
+    {nullptr, 0, 0, k_solve<true, 256, 2>, 256},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 128, 2>, 128},
+    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<true, 256, 2>, 256},
 };
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
@@ -2319,9 +2426,22 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             if (run_lds) (void)hipStreamWaitEvent(ss.aux[0], ss.fork, 0);
         }
         if (run_big) {
-            hipLaunchKernelGGL(v.big_fn, dim3(S.nworlds), dim3(v.big_threads), 0, ss.aux[1], S, dt, sub_steps, iters,
-                               sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
-            tmark(ss, "k_solve (global)", ss.aux[1]);
+            // worlds beyond LDS: grid-wide body and item phases around a
+            // one-workgroup-per-world chain kernel, all on the second side stream
+            const hipStream_t sb = ss.aux[1];
+            const float sub_dt = dt / (float)sub_steps;
+            const unsigned gb = blocks_for(S.nb, BS);
+            hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
+            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
+            for (uint32_t s = 0; s < sub_steps; ++s) {
+                hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps, sub_dt);
+                hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps,
+                                   sub_dt);
+                hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), 0, sb, S, sp.lds_nmax, iters, sub_dt);
+                hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 2, s, sub_steps, sub_dt);
+            }
+            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps, sub_dt);
+            tmark(ss, "k_big (global state)", sb);
         }
         if (run_lds) {
             hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), solve_lds_bytes((int)sp.lds_maxn),
@@ -2355,8 +2475,6 @@ void set_solve_smem_limit()
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)kSolveVariants[v].rec_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        (void)hipFuncSetAttribute((const void*)kSolveVariants[v].big_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
     }
 }
