@@ -375,6 +375,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.recs, I * 128);
     add((void**)&S.cmask, 2 * N * 8);
     add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
+    add((void**)&S.cmap, N * 4); add((void**)&S.cinv, N * 4); add((void**)&S.w_nc, (W + 1) * 4);
     add((void**)&S.counters, 64 * 4);
     add((void**)&S.partials, 1024 * 4);
     for (auto& p : parts) off += align256(p.second);
@@ -696,10 +697,10 @@ void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
 // plus its share of the step scratch at the initial capacity estimates
 // (per body: cell ranges, 4 cell entries, 8 raw candidates, 4 manifolds and
 // 4 solver items; per hash bucket 12 B; per joint one solver item).
-constexpr size_t kScratchPerBody = 88 + 4 * 24 + 8 * 36 + 4 * 128 + 4 * 160;
+constexpr size_t kScratchPerBody = 96 + 4 * 24 + 8 * 36 + 4 * 128 + 4 * 160;
 constexpr size_t kScratchPerBucket = 12;
 constexpr size_t kScratchPerJoint = 160;
-constexpr size_t kScratchPerWorld = 16 + sizeof(WorldDev);
+constexpr size_t kScratchPerWorld = 20 + sizeof(WorldDev);
 
 size_t world_footprint(const r2d_world* w)
 {

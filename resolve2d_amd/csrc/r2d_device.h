@@ -114,6 +114,9 @@ struct StepDev {
     // global-memory solver state (worlds too large for LDS)
     float *g_im, *g_ii;
     uint32_t* g_cnt;
+    uint32_t* cmap;          // [N] worlds beyond LDS: compact chain index of each body (or ~0)
+    uint32_t* cinv;          // [N] compact index -> world slot, at the world's body base
+    uint32_t* w_nc;          // [W] bodies in the compact chain
     uint32_t* counters;
     uint32_t* partials;                    // scan scratch
 };

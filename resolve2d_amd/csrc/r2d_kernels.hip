@@ -934,93 +934,6 @@ struct SolverState {
 };
 
 
-template <class ST>
-__device__ __noinline__ void solve_joint(const WorldDev& D, const ST st, const Joint& J, float dt)
-{
-    if (J.type == 0) {
-        // DistanceJoint.solve (DistanceJoint.zig:40-75)
-        uint32_t a = J.s1, b = J.s2;
-        float w1 = st.im[a], w2 = st.im[b];
-        v2 v1 = scale2(mk(st.mx[a], st.my[a]), w1);
-        v2 v2v = scale2(mk(st.mx[b], st.my[b]), w2);
-        v2 rv = sub2(v2v, v1);
-        v2 pa = mk(D.px[a], D.py[a]), pb = mk(D.px[b], D.py[b]);
-        v2 n = normalize2(sub2(pb, pa));
-        float dist_err = length2(sub2(pb, pa)) - J.target;
-        float pc = J.beta * dist_err / maxn(J.target, 0.001f);
-        float Jv = -(dot2(rv, n) + pc) / (w1 + w2);
-        float ndv = dot2(n, rv);
-        float den = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
-        Jv = clampn(Jv, J.pmin * den, J.pmax * den);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] -= dp.x; st.my[a] -= dp.y;
-        st.mx[b] += dp.x; st.my[b] += dp.y;
-    } else if (J.type == 1) {
-        // OffsetDistanceJoint.solve (OffsetDistanceJoint.zig:44-105)
-        uint32_t a = J.s1, b = J.s2;
-        float ca = cosf_r(D.ang[a]), sa = sinf_r(D.ang[a]);
-        float cb = cosf_r(D.ang[b]), sb = sinf_r(D.ang[b]);
-        v2 r1 = mk(J.r1x, J.r1y), r2 = mk(J.r2x, J.r2y);
-        v2 rr1 = rotcs(r1, ca, sa), rr2 = rotcs(r2, cb, sb);
-        v2 a1 = add2(rr1, mk(D.px[a], D.py[a]));
-        v2 a2 = add2(rr2, mk(D.px[b], D.py[b]));
-        v2 n = normalize2(sub2(a2, a1));
-        float im1 = st.im[a], im2 = st.im[b], ii1 = st.ii[a], ii2 = st.ii[b];
-        v2 vl1 = scale2(mk(st.mx[a], st.my[a]), im1);
-        float w1 = st.L[a] * ii1;
-        v2 vl2 = scale2(mk(st.mx[b], st.my[b]), im2);
-        float w2 = st.L[b] * ii2;
-        v2 v1 = add2(vl1, mk(-rr1.y * w1, rr1.x * w1));
-        v2 v2v = add2(vl2, mk(-rr2.y * w2, rr2.x * w2));
-        v2 dv = sub2(v2v, v1);
-        float dist_err = length2(sub2(a2, a1)) - J.target;
-        float pc = J.beta * dist_err / maxn(J.target, GRAD_DIV_LIMIT);
-        float num = dot2(n, dv) + pc;
-        float r1xn = cross2(rr1, n), r2xn = cross2(rr2, n);
-        float den = im1 + im2 + r1xn * r1xn * ii1 + r2xn * r2xn * ii2;
-        float Jv = -num / den;
-        float ndv = dot2(n, dv);
-        float den2 = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
-        Jv = clampn(Jv, J.pmin * den2, J.pmax * den2);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] -= dp.x; st.my[a] -= dp.y;
-        st.L[a] -= Jv * r1xn;
-        st.mx[b] += dp.x; st.my[b] += dp.y;
-        st.L[b] += Jv * r2xn;
-    } else if (J.type == 2) {
-        // FixedPositionJoint.solve (FixedPositionJoint.zig:38-72)
-        uint32_t a = J.s1;
-        if (D.flags[a] & BF_STATIC) return;
-        float w = st.im[a];
-        v2 v = scale2(mk(st.mx[a], st.my[a]), w);
-        v2 delta = sub2(mk(J.tx, J.ty), mk(D.px[a], D.py[a]));
-        float dist = length2(delta);
-        if (dist < ALLOWED_CONSTRAINT_VALUE) return;
-        v2 n = scale2(delta, 1.0f / dist);
-        float bias = J.beta * dist;
-        float rv = dot2(v, n);
-        float Jv = -(rv - bias) / w;
-        float den = 1.0f / maxn(absf(rv), ALLOWED_CONSTRAINT_VALUE);
-        Jv = clampn(Jv, J.pmin * den, J.pmax * den);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] += dp.x; st.my[a] += dp.y;
-    } else {
-        // MotorJoint.solve (MotorJoint.zig:38-64)
-        uint32_t a = J.s1;
-        if (D.flags[a] & BF_STATIC) return;
-        float I = D.inertia[a];
-        float omega = st.L[a] / I;
-        float C = omega - J.target;
-        if (absf(C) < ALLOWED_CONSTRAINT_VALUE) return;
-        float bias = J.beta * C;
-        float tq = D.tq[a];
-        float Jv = tq - bias;
-        float rel = tq / I;
-        float den = 1.0f / maxn(absf(rel), ALLOWED_CONSTRAINT_VALUE);
-        Jv = clampn(Jv, J.pmin * den, J.pmax * den);
-        st.L[a] += Jv * dt;
-    }
-}
 
 // ------------------------------------------------------- solver item records
 // Each solver item of a step gets one 128-byte record at its execution
@@ -1040,7 +953,8 @@ constexpr uint32_t NONE = 0xffffffffu;
 // 3 index (manifold in the step arrays / world-local joint), 4-5 ranks at the
 // ordering bodies, 6-8 normal + friction, 9-15 point 0 (r1x r1y r2x r2y depth
 // mass_n mass_t), 16-22 point 1, 24-27 impulses (pn0 pt0 pn1 pt1), 28 the
-// ordering bodies' item counts, 29-30 the points' Baumgarte biases
+// ordering bodies' item counts, 29-30 the points' Baumgarte biases, 31 the
+// compact chain indices (worlds beyond LDS)
 struct Rec {
     uint32_t flags, a, b, idx, rk0, rk1;
     float nx, ny, fric;
@@ -1048,6 +962,7 @@ struct Rec {
     float acc[4];
     uint32_t degs;       // items at r0 | items at r1 << 16; DEGS_WIDE when either needs more than 16 bits
     float bias[2];       // Baumgarte bias of the sub-step per point, -1 when depth >= 0
+    uint32_t cpair;      // worlds beyond LDS: compact chain indices of (a, b) / the joint's (s1, s2)
 };
 constexpr uint32_t DEGS_WIDE = 0xffffffffu;
 
@@ -1069,6 +984,16 @@ __device__ __forceinline__ void rec_load(Rec& r, const uint4* q)
     uint4 v7 = q[7];
     r.degs = v7.x;
     r.bias[0] = __uint_as_float(v7.y); r.bias[1] = __uint_as_float(v7.z);
+    r.cpair = v7.w;
+}
+
+// a record read by the compact chain: its bodies as compact indices
+__device__ __forceinline__ void rec_to_compact(Rec& r)
+{
+    const uint32_t c0 = r.cpair & 0xffffu, c1 = r.cpair >> 16;
+    r.a = c0;
+    if (r.flags & RF_JOINT) r.b = (r.b == NONE) ? NONE : c1;
+    else r.b = c1;
 }
 
 __device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
@@ -1084,7 +1009,7 @@ __device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
     q[5] = make_uint4(__float_as_uint(r.p[1][4]), __float_as_uint(r.p[1][5]), __float_as_uint(r.p[1][6]), 0u);
     q[6] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
                       __float_as_uint(r.acc[3]));
-    q[7] = make_uint4(r.degs, __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), 0u);
+    q[7] = make_uint4(r.degs, __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), r.cpair);
 }
 
 __device__ __forceinline__ int32_t rec_r0(const Rec& r)
@@ -1148,6 +1073,7 @@ __device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldD
     r.nx = r.ny = r.fric = 0.0f;
     r.degs = 0u;
     r.bias[0] = r.bias[1] = -1.0f;
+    r.cpair = 0u;
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -1262,6 +1188,148 @@ __device__ __forceinline__ void rec_impulses(Rec& r, const ST& st, float dt)
     }
     if (r.flags & RF_W1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
     if (r.flags & RF_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
+}
+
+// What a joint link reads besides the bodies' momentum and inverse masses:
+// parameters and the pose of its bodies, constant within a sub-step.
+struct JointIn {
+    uint32_t type;
+    bool stat1;                       // body 1 is static (fixed / motor joints skip)
+    float pmax, pmin, beta, target;
+    float r1x, r1y, r2x, r2y, tx, ty;
+    v2 pa, pb;                        // positions
+    float ca, sa, cb, sb;             // cos / sin of the angles (offset joints)
+    float I1, tq1;                    // inertia and torque of body 1 (motor joints)
+};
+
+// Constraint.solve (Constraints/*.zig) with momentum / inverse masses at st
+// indices a and b
+template <class ST>
+__device__ __forceinline__ void joint_apply(const JointIn& j, const ST& st, uint32_t a, uint32_t b, float dt)
+{
+    const float pmax = j.pmax, pmin = j.pmin, beta = j.beta, target = j.target;
+    if (j.type == 0) {
+        // DistanceJoint.solve (DistanceJoint.zig:40-75)
+        float w1 = st.im[a], w2 = st.im[b];
+        v2 v1 = scale2(mk(st.mx[a], st.my[a]), w1);
+        v2 v2v = scale2(mk(st.mx[b], st.my[b]), w2);
+        v2 rv = sub2(v2v, v1);
+        v2 pa = j.pa, pb = j.pb;
+        v2 n = normalize2(sub2(pb, pa));
+        float dist_err = length2(sub2(pb, pa)) - target;
+        float pc = beta * dist_err / maxn(target, 0.001f);
+        float Jv = -(dot2(rv, n) + pc) / (w1 + w2);
+        float ndv = dot2(n, rv);
+        float den = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] -= dp.x; st.my[a] -= dp.y;
+        st.mx[b] += dp.x; st.my[b] += dp.y;
+    } else if (j.type == 1) {
+        // OffsetDistanceJoint.solve (OffsetDistanceJoint.zig:44-105)
+        v2 r1 = mk(j.r1x, j.r1y), r2 = mk(j.r2x, j.r2y);
+        v2 rr1 = rotcs(r1, j.ca, j.sa), rr2 = rotcs(r2, j.cb, j.sb);
+        v2 a1 = add2(rr1, j.pa);
+        v2 a2 = add2(rr2, j.pb);
+        v2 n = normalize2(sub2(a2, a1));
+        float im1 = st.im[a], im2 = st.im[b], ii1 = st.ii[a], ii2 = st.ii[b];
+        v2 vl1 = scale2(mk(st.mx[a], st.my[a]), im1);
+        float w1 = st.L[a] * ii1;
+        v2 vl2 = scale2(mk(st.mx[b], st.my[b]), im2);
+        float w2 = st.L[b] * ii2;
+        v2 v1 = add2(vl1, mk(-rr1.y * w1, rr1.x * w1));
+        v2 v2v = add2(vl2, mk(-rr2.y * w2, rr2.x * w2));
+        v2 dv = sub2(v2v, v1);
+        float dist_err = length2(sub2(a2, a1)) - target;
+        float pc = beta * dist_err / maxn(target, GRAD_DIV_LIMIT);
+        float num = dot2(n, dv) + pc;
+        float r1xn = cross2(rr1, n), r2xn = cross2(rr2, n);
+        float den = im1 + im2 + r1xn * r1xn * ii1 + r2xn * r2xn * ii2;
+        float Jv = -num / den;
+        float ndv = dot2(n, dv);
+        float den2 = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
+        Jv = clampn(Jv, pmin * den2, pmax * den2);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] -= dp.x; st.my[a] -= dp.y;
+        st.L[a] -= Jv * r1xn;
+        st.mx[b] += dp.x; st.my[b] += dp.y;
+        st.L[b] += Jv * r2xn;
+    } else if (j.type == 2) {
+        // FixedPositionJoint.solve (FixedPositionJoint.zig:38-72)
+        if (j.stat1) return;
+        float w = st.im[a];
+        v2 v = scale2(mk(st.mx[a], st.my[a]), w);
+        v2 delta = sub2(mk(j.tx, j.ty), j.pa);
+        float dist = length2(delta);
+        if (dist < ALLOWED_CONSTRAINT_VALUE) return;
+        v2 n = scale2(delta, 1.0f / dist);
+        float bias = beta * dist;
+        float rv = dot2(v, n);
+        float Jv = -(rv - bias) / w;
+        float den = 1.0f / maxn(absf(rv), ALLOWED_CONSTRAINT_VALUE);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        v2 dp = scale2(n, Jv);
+        st.mx[a] += dp.x; st.my[a] += dp.y;
+    } else {
+        // MotorJoint.solve (MotorJoint.zig:38-64)
+        if (j.stat1) return;
+        float I = j.I1;
+        float omega = st.L[a] / I;
+        float C = omega - target;
+        if (absf(C) < ALLOWED_CONSTRAINT_VALUE) return;
+        float bias = beta * C;
+        float tq = j.tq1;
+        float Jv = tq - bias;
+        float rel = tq / I;
+        float den = 1.0f / maxn(absf(rel), ALLOWED_CONSTRAINT_VALUE);
+        Jv = clampn(Jv, pmin * den, pmax * den);
+        st.L[a] += Jv * dt;
+    }
+}
+
+// Joint records keep what the link reads in the fields a contact uses for its
+// points: p[0] = pmax pmin beta r1x r1y r2x r2y, p[1][0..2] = target tx ty,
+// p[1][3..6] = positions of the two bodies, (nx, ny, fric, acc[0]) = cos / sin
+// of both angles, acc[1] / acc[2] = inertia / torque of body 1; the type in
+// flags bits 24-30 and RF_S1 when body 1 is static.
+__device__ __forceinline__ void rec_joint_params(Rec& r, const Joint& J, const WorldDev& D)
+{
+    r.flags |= (uint32_t)J.type << 24;
+    if (D.flags[J.s1] & BF_STATIC) r.flags |= RF_S1;
+    r.p[0][0] = J.pmax; r.p[0][1] = J.pmin; r.p[0][2] = J.beta;
+    r.p[0][3] = J.r1x; r.p[0][4] = J.r1y; r.p[0][5] = J.r2x; r.p[0][6] = J.r2y;
+    r.p[1][0] = J.target; r.p[1][1] = J.tx; r.p[1][2] = J.ty;
+}
+
+// the sub-step's pose of a joint record's bodies (after the previous
+// sub-step's position integration; torque as set for this sub-step)
+__device__ __forceinline__ void rec_joint_pose(Rec& r, const Joint& J, const WorldDev& D)
+{
+    const int32_t a = J.s1, b = J.s2;
+    r.p[1][3] = D.px[a]; r.p[1][4] = D.py[a];
+    r.acc[1] = D.inertia[a];
+    r.acc[2] = D.tq[a];
+    if (J.type == 1) {
+        float angle_a = D.ang[a], angle_b = D.ang[b];
+        r.nx = cosf_r(angle_a); r.ny = sinf_r(angle_a);
+        r.fric = cosf_r(angle_b); r.acc[0] = sinf_r(angle_b);
+    }
+    if (J.type <= 1) { r.p[1][5] = D.px[b]; r.p[1][6] = D.py[b]; }
+}
+
+template <class ST>
+__device__ __forceinline__ void rec_solve_joint(const Rec& r, const ST& st, uint32_t ua, uint32_t ub, float dt)
+{
+    JointIn j;
+    j.type = (r.flags >> 24) & 0x7fu;
+    j.stat1 = (r.flags & RF_S1) != 0;
+    j.pmax = r.p[0][0]; j.pmin = r.p[0][1]; j.beta = r.p[0][2];
+    j.r1x = r.p[0][3]; j.r1y = r.p[0][4]; j.r2x = r.p[0][5]; j.r2y = r.p[0][6];
+    j.target = r.p[1][0]; j.tx = r.p[1][1]; j.ty = r.p[1][2];
+    j.pa = mk(r.p[1][3], r.p[1][4]); j.pb = mk(r.p[1][5], r.p[1][6]);
+    j.ca = r.nx; j.sa = r.ny; j.cb = r.fric; j.sb = r.acc[0];
+    j.I1 = r.acc[1]; j.tq1 = r.acc[2];
+    joint_apply(j, st, ua, ub, dt);
 }
 
 // one body's solver state as read by a chain link
@@ -1465,7 +1533,13 @@ __device__ __forceinline__ void item_build(const StepDev& S, const WorldDev& D, 
     Rec r;
     rec_build(r, S, D, st, pos, ibase, (uint32_t)D.njoints, mbeg, D.body_base);
     r.degs = rec_degs(r, st);
-    if (!(r.flags & RF_JOINT)) rec_biases(r, sub_dt);
+    if (r.flags & RF_JOINT) {
+        const Joint& J = D.joints[r.idx];
+        rec_joint_params(r, J, D);
+        rec_joint_pose(r, J, D);
+    } else {
+        rec_biases(r, sub_dt);
+    }
     rec_store(r, recs + (size_t)pos * 8);
 }
 
@@ -1477,7 +1551,13 @@ __device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, co
 {
     uint4* q = recs + (size_t)pos * 8;
     uint4 h = q[0];
-    if (h.x & RF_JOINT) return;
+    if (h.x & RF_JOINT) {
+        Rec r;
+        rec_load(r, q);
+        rec_joint_pose(r, D.joints[h.w], D);
+        rec_store(r, q);
+        return;
+    }
     if (h.x & RF_TGS) {
         Rec r;
         rec_load(r, q);
@@ -1492,7 +1572,7 @@ __device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, co
 
 // K iterations of one sub-step's (constraints, manifolds) for the world of
 // this workgroup, as an ordered dataflow
-template <bool LDS, class ST>
+template <bool LDS, bool COMPACT, class ST>
 __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs,
                                            uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                            uint32_t& s_abort)
@@ -1511,6 +1591,7 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
         bool have = false, has_nxt = false;
         if (kcur < iters) {
             rec_load(nxt, recs + (size_t)pcur * 8);
+            if (COMPACT) rec_to_compact(nxt);
             nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
             pcur += p_step;
             if (pcur >= nitems) { pcur = p_begin; ++kcur; }
@@ -1530,7 +1611,10 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
                 // next iteration reads impulses this record is about to write,
                 // so that case is forwarded from registers instead
                 if (kcur < iters) {
-                    if (pcur != cur_pos) rec_load(nxt, recs + (size_t)pcur * 8);
+                    if (pcur != cur_pos) {
+                        rec_load(nxt, recs + (size_t)pcur * 8);
+                        if (COMPACT) rec_to_compact(nxt);
+                    }
                     nxt_pos = pcur; nxt_k = kcur; has_nxt = true;
                     pcur += p_step;
                     if (pcur >= nitems) { pcur = p_begin; ++kcur; }
@@ -1570,7 +1654,9 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
             }
             if (ready) {
                 if (!contact) {
-                    solve_joint(D, st, D.joints[cur.idx], sub_dt);
+                    const uint32_t ua = COMPACT ? (cur.cpair & 0xffffu) : cur.a;
+                    const uint32_t ub = COMPACT ? (cur.cpair >> 16) : (cur.b == NONE ? cur.a : cur.b);
+                    rec_solve_joint(cur, st, ua, ub, sub_dt);
                 } else {
                     contact_impulses(cur, cur.a, cur.b, b1, b2, st);
                     recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
@@ -1668,7 +1754,7 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
         __syncthreads();
         for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
         __syncthreads();
-        item_chain<LDS>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        item_chain<LDS, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
         __syncthreads();
@@ -1717,10 +1803,14 @@ __global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s
     }
 }
 
-// phase 0: build records; 1: updateTGSDepth (sub-step s); 2: write back.
-// One workgroup per world.
-__global__ void k_big_items(StepDev S, uint32_t lds_nmax, int phase, uint32_t s, uint32_t sub_steps, float sub_dt)
+// phase 0: build records, then the compact chain indices; 1: updateTGSDepth
+// (sub-step s); 2: write back.  One workgroup (BIG_THREADS) per world.
+constexpr unsigned BIG_THREADS = 256;
+__global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t lds_nmax, int phase, uint32_t s,
+                                                        uint32_t sub_steps, float sub_dt)
 {
+    __shared__ uint32_t s_scan[BIG_THREADS];
+    __shared__ uint32_t s_total;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = blockIdx.x;
     if (S.w_status[w] != 0) return;
@@ -1728,18 +1818,84 @@ __global__ void k_big_items(StepDev S, uint32_t lds_nmax, int phase, uint32_t s,
     if (!big_world(D, lds_nmax)) return;
     const SolverState<false> st = global_state(S, D);
     const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase;
+    const uint32_t base = D.body_base, T = blockDim.x, t = threadIdx.x;
+    const int n = D.nbodies;
     uint4* recs = S.recs + (size_t)ibase * 8;
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) {
+    for (uint32_t pos = t; pos < nitems; pos += T) {
         if (phase == 0) item_build(S, D, st, recs, pos, ibase, S.w_man_begin[w], sub_dt);
         else if (phase == 1) item_tgs(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
         else item_writeback(S, recs, pos);
     }
+    if (phase != 0) return;
+    // compact chain indices: the bodies the items touch, numbered in slot order
+    const uint32_t UNUSED = 0xffffffffu;
+    for (int i = t; i < n; i += T) S.cmap[base + i] = UNUSED;
+    __syncthreads();
+    for (uint32_t pos = t; pos < nitems; pos += T) {
+        const uint4 h = recs[(size_t)pos * 8];
+        if (h.x & RF_JOINT) {
+            const Joint& J = D.joints[h.w];
+            S.cmap[base + J.s1] = 0u;
+            if (J.s2 >= 0) S.cmap[base + J.s2] = 0u;
+        } else {
+            S.cmap[base + h.y] = 0u;
+            S.cmap[base + h.z] = 0u;
+        }
+    }
+    __syncthreads();
+    // each thread numbers a contiguous range of slots: count, block scan, assign
+    const int chunk = (n + (int)T - 1) / (int)T;
+    const int lo = min((int)t * chunk, n), hi = min(lo + chunk, n);
+    uint32_t cnt = 0;
+    for (int i = lo; i < hi; ++i) cnt += S.cmap[base + i] == 0u;
+    s_scan[t] = cnt;
+    __syncthreads();
+    for (uint32_t d = 1; d < T; d <<= 1) {
+        uint32_t v = t >= d ? s_scan[t - d] : 0u;
+        __syncthreads();
+        s_scan[t] += v;
+        __syncthreads();
+    }
+    uint32_t c = s_scan[t] - cnt;
+    for (int i = lo; i < hi; ++i) {
+        if (S.cmap[base + i] != 0u) continue;
+        S.cmap[base + i] = c;
+        S.cinv[base + c] = (uint32_t)i;
+        ++c;
+    }
+    if (t == T - 1) s_total = s_scan[T - 1];
+    __syncthreads();
+    const uint32_t nc = s_total;
+    if (t == 0) S.w_nc[w] = nc;
+    if (nc >= 0xffffu) return;   // the compact chain is not used (LDS holds far fewer)
+    for (uint32_t pos = t; pos < nitems; pos += T) {
+        uint4* q = recs + (size_t)pos * 8;
+        const uint4 h = q[0];
+        uint32_t c0, c1;
+        if (h.x & RF_JOINT) {
+            const Joint& J = D.joints[h.w];
+            c0 = S.cmap[base + J.s1];
+            c1 = J.s2 >= 0 ? S.cmap[base + J.s2] : 0xffffu;
+        } else {
+            c0 = S.cmap[base + h.y];
+            c1 = S.cmap[base + h.z];
+        }
+        q[7].w = c0 | (c1 << 16);
+    }
 }
 
-// one sub-step's K iterations; one workgroup per world
+// LDS bytes of the compact chain per body (momentum, inverse masses, counter, item count)
+constexpr size_t COMPACT_BYTES = 7 * 4;
+constexpr size_t COMPACT_LDS = 160 * 1024 - 1024;
+
+// one sub-step's K iterations; one workgroup per world.  When the bodies the
+// items touch fit LDS, their state is gathered there under compact indices,
+// the chain runs on LDS, and the state is scattered back; otherwise the
+// chain runs on global memory.
 template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax, uint32_t iters, float sub_dt)
 {
+    extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = S.w_order[blockIdx.x];
@@ -1748,9 +1904,37 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
     if (!big_world(D, lds_nmax)) return;
     if (threadIdx.x == 0) s_abort = 0u;
     __syncthreads();
-    const SolverState<false> st = global_state(S, D);
     const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase;
-    item_chain<false>(S, D, st, S.recs + (size_t)ibase * 8, nitems, iters, sub_dt, w, s_abort);
+    const uint32_t base = D.body_base;
+    uint4* recs = S.recs + (size_t)ibase * 8;
+    const uint32_t nc = S.w_nc[w];
+    if ((size_t)nc * COMPACT_BYTES <= COMPACT_LDS) {
+        SolverState<true> st;
+        st.mx = (float*)smem;
+        st.my = st.mx + nc;
+        st.L = st.my + nc;
+        st.im = st.L + nc;
+        st.ii = st.im + nc;
+        st.cnt = (uint32_t*)(st.ii + nc);
+        st.deg = st.cnt + nc;
+        for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+            const uint32_t i = S.cinv[base + c];
+            st.mx[c] = D.mx[i]; st.my[c] = D.my[i]; st.L[c] = D.L[i];
+            st.im[c] = S.g_im[base + i]; st.ii[c] = S.g_ii[base + i];
+            st.cnt[c] = 0u;
+            st.deg[c] = S.deg[base + i];
+        }
+        __syncthreads();
+        item_chain<true, true>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+            const uint32_t i = S.cinv[base + c];
+            D.mx[i] = st.mx[c]; D.my[i] = st.my[c]; D.L[i] = st.L[c];
+        }
+    } else {
+        const SolverState<false> st = global_state(S, D);
+        item_chain<false, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+    }
 }
 
 // ------------------------------------------------------ register-slot solver
@@ -1824,90 +2008,27 @@ __device__ __forceinline__ void slot_joint(Slot& z, const Joint& J, const WorldD
     z.p[1][0] = J.target; z.p[1][1] = J.tx; z.p[1][2] = J.ty;
 }
 
-// Constraint.solve for a joint slot; the arithmetic of solve_joint with pose,
-// cos / sin and inertia read from LDS
+// Constraint.solve for a joint slot: pose, cos / sin, inertia and torque from LDS
 __device__ __forceinline__ void slot_solve_joint(const Slot& z, const SlotState& st, float dt)
 {
-    const uint32_t type = (z.flags >> 24) & 0x7fu;
     const uint32_t a = z.idx & 0xffffu, b = z.idx >> 16;
-    const float pmax = z.p[0][0], pmin = z.p[0][1], beta = z.p[0][2], target = z.p[1][0];
-    if (type == 0) {
-        // DistanceJoint.solve (DistanceJoint.zig:40-75)
-        float w1 = st.im[a], w2 = st.im[b];
-        v2 v1 = scale2(mk(st.mx[a], st.my[a]), w1);
-        v2 v2v = scale2(mk(st.mx[b], st.my[b]), w2);
-        v2 rv = sub2(v2v, v1);
-        v2 pa = mk(st.px[a], st.py[a]), pb = mk(st.px[b], st.py[b]);
-        v2 n = normalize2(sub2(pb, pa));
-        float dist_err = length2(sub2(pb, pa)) - target;
-        float pc = beta * dist_err / maxn(target, 0.001f);
-        float Jv = -(dot2(rv, n) + pc) / (w1 + w2);
-        float ndv = dot2(n, rv);
-        float den = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
-        Jv = clampn(Jv, pmin * den, pmax * den);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] -= dp.x; st.my[a] -= dp.y;
-        st.mx[b] += dp.x; st.my[b] += dp.y;
-    } else if (type == 1) {
-        // OffsetDistanceJoint.solve (OffsetDistanceJoint.zig:44-105)
-        v2 r1 = mk(z.p[0][3], z.p[0][4]), r2 = mk(z.p[0][5], z.p[0][6]);
-        v2 rr1 = rotcs(r1, st.c[a], st.s[a]), rr2 = rotcs(r2, st.c[b], st.s[b]);
-        v2 a1 = add2(rr1, mk(st.px[a], st.py[a]));
-        v2 a2 = add2(rr2, mk(st.px[b], st.py[b]));
-        v2 n = normalize2(sub2(a2, a1));
-        float im1 = st.im[a], im2 = st.im[b], ii1 = st.ii[a], ii2 = st.ii[b];
-        v2 vl1 = scale2(mk(st.mx[a], st.my[a]), im1);
-        float w1 = st.L[a] * ii1;
-        v2 vl2 = scale2(mk(st.mx[b], st.my[b]), im2);
-        float w2 = st.L[b] * ii2;
-        v2 v1 = add2(vl1, mk(-rr1.y * w1, rr1.x * w1));
-        v2 v2v = add2(vl2, mk(-rr2.y * w2, rr2.x * w2));
-        v2 dv = sub2(v2v, v1);
-        float dist_err = length2(sub2(a2, a1)) - target;
-        float pc = beta * dist_err / maxn(target, GRAD_DIV_LIMIT);
-        float num = dot2(n, dv) + pc;
-        float r1xn = cross2(rr1, n), r2xn = cross2(rr2, n);
-        float den = im1 + im2 + r1xn * r1xn * ii1 + r2xn * r2xn * ii2;
-        float Jv = -num / den;
-        float ndv = dot2(n, dv);
-        float den2 = 1.0f / maxn(absf(ndv), GRAD_DIV_LIMIT);
-        Jv = clampn(Jv, pmin * den2, pmax * den2);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] -= dp.x; st.my[a] -= dp.y;
-        st.L[a] -= Jv * r1xn;
-        st.mx[b] += dp.x; st.my[b] += dp.y;
-        st.L[b] += Jv * r2xn;
-    } else if (type == 2) {
-        // FixedPositionJoint.solve (FixedPositionJoint.zig:38-72)
-        if (z.flags & RF_S1) return;
-        float w = st.im[a];
-        v2 v = scale2(mk(st.mx[a], st.my[a]), w);
-        v2 delta = sub2(mk(z.p[1][1], z.p[1][2]), mk(st.px[a], st.py[a]));
-        float dist = length2(delta);
-        if (dist < ALLOWED_CONSTRAINT_VALUE) return;
-        v2 n = scale2(delta, 1.0f / dist);
-        float bias = beta * dist;
-        float rv = dot2(v, n);
-        float Jv = -(rv - bias) / w;
-        float den = 1.0f / maxn(absf(rv), ALLOWED_CONSTRAINT_VALUE);
-        Jv = clampn(Jv, pmin * den, pmax * den);
-        v2 dp = scale2(n, Jv);
-        st.mx[a] += dp.x; st.my[a] += dp.y;
+    JointIn j;
+    j.type = (z.flags >> 24) & 0x7fu;
+    j.stat1 = (z.flags & RF_S1) != 0;
+    j.pmax = z.p[0][0]; j.pmin = z.p[0][1]; j.beta = z.p[0][2];
+    j.r1x = z.p[0][3]; j.r1y = z.p[0][4]; j.r2x = z.p[0][5]; j.r2y = z.p[0][6];
+    j.target = z.p[1][0]; j.tx = z.p[1][1]; j.ty = z.p[1][2];
+    j.pa = mk(st.px[a], st.py[a]);
+    j.ca = st.c[a]; j.sa = st.s[a];
+    j.I1 = st.I[a]; j.tq1 = st.tq[a];
+    if (j.type <= 1) {
+        j.pb = mk(st.px[b], st.py[b]);
+        j.cb = st.c[b]; j.sb = st.s[b];
     } else {
-        // MotorJoint.solve (MotorJoint.zig:38-64)
-        if (z.flags & RF_S1) return;
-        float I = st.I[a];
-        float omega = st.L[a] / I;
-        float C = omega - target;
-        if (absf(C) < ALLOWED_CONSTRAINT_VALUE) return;
-        float bias = beta * C;
-        float tq = st.tq[a];
-        float Jv = tq - bias;
-        float rel = tq / I;
-        float den = 1.0f / maxn(absf(rel), ALLOWED_CONSTRAINT_VALUE);
-        Jv = clampn(Jv, pmin * den, pmax * den);
-        st.L[a] += Jv * dt;
+        j.pb = mk(0.f, 0.f);
+        j.cb = j.sb = 0.f;
     }
+    joint_apply(j, st, a, b, dt);
 }
 
 // ordering bodies of a slot (-1: none) and the Baumgarte biases of a sub-step
@@ -2432,15 +2553,18 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             const float sub_dt = dt / (float)sub_steps;
             const unsigned gb = blocks_for(S.nb, BS);
             hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
-            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
+            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps,
+                               sub_dt);
             for (uint32_t s = 0; s < sub_steps; ++s) {
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps, sub_dt);
-                hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps,
+                hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s,
+                                   sub_steps, sub_dt);
+                hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S, sp.lds_nmax, iters,
                                    sub_dt);
-                hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), 0, sb, S, sp.lds_nmax, iters, sub_dt);
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 2, s, sub_steps, sub_dt);
             }
-            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps, sub_dt);
+            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps,
+                               sub_dt);
             tmark(ss, "k_big (global state)", sb);
         }
         if (run_lds) {
@@ -2470,6 +2594,7 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
 
 void set_solve_smem_limit()
 {
+    (void)hipFuncSetAttribute((const void*)k_big_chain<256>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     for (int v = 0; v < SOLVE_VARIANTS; ++v) {
         if (kSolveVariants[v].slot_fn)
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,
