@@ -207,9 +207,26 @@ int r2d_ctx_set_profiling(r2d_ctx* ctx, int on);
 /* Accumulated milliseconds and step count per phase: out_ms[4], *steps. */
 int r2d_ctx_phase_times(r2d_ctx* ctx, double* out_ms, uint64_t* steps);
 /* Device time (ms) of each timed step since the last reset, in order: up to
- * cap values into out (may be NULL), the count into *n. */
+ * cap values into out (may be NULL), the count into *n.  Steps taken by the
+ * fused kernel report each world's step (wall clock of its workgroup). */
 int r2d_ctx_step_times(r2d_ctx* ctx, float* out, size_t cap, size_t* n);
 int r2d_ctx_reset_phase_times(r2d_ctx* ctx);
+/* Profiled launches of the fused kernel since the last reset: their summed
+ * device time (ms, HIP events around each launch), their count and the
+ * world-steps they took.  No counterpart in the reference (measurement). */
+int r2d_ctx_fused_times(r2d_ctx* ctx, double* kernel_ms, uint64_t* launches, uint64_t* world_steps);
+/* The same launches' time per phase of a world's step, summed over the
+ * worlds' steps (ms; wall clock of each workgroup): out_ms[8] = cell ranges,
+ * buckets, raw candidates, narrowphase, contact points, schedule, solve, 0. */
+int r2d_ctx_fused_phase_times(r2d_ctx* ctx, double* out_ms);
+/* Whole steps per workgroup (default on): a batch whose worlds all have at
+ * most 1,024 bodies, the reference order and hash tables that fit LDS is
+ * stepped by one kernel per call -- every step of every world of the call,
+ * one workgroup per world -- instead of the phase kernels of each step.
+ * Results are bit-identical either way; off forces the phase kernels.  The
+ * environment variable R2D_NO_FUSED=1 sets the default to off.  No
+ * counterpart in the reference (Solver.process is one world, one step). */
+int r2d_ctx_set_fused(r2d_ctx* ctx, int on);
 /* Launch configuration of the solve kernel: -1 (default) is variant 1;
  * 0..8 select the register-slot kernel's slots x workgroup size and the
  * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner

@@ -141,6 +141,10 @@ SIGNATURES = {
     "r2d_last_error": [ctypes.c_char_p, SZ],
     "r2d_ctx_set_solve_variant": [P, I32],
     "r2d_ctx_set_lds_limit": [P, U32],
+    "r2d_ctx_set_fused": [P, I32],
+    "r2d_ctx_fused_phase_times": [P, ctypes.POINTER(ctypes.c_double)],
+    "r2d_ctx_fused_times": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                            ctypes.POINTER(ctypes.c_uint64)],
     "r2d_make_bodies": [P, SZ, ctypes.POINTER(BodyOptions), ctypes.POINTER(ctypes.c_int32),
                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), U32P],
     "r2d_world_footprint": [P, SZP],

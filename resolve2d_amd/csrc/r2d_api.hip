@@ -105,6 +105,10 @@ struct r2d_world {
     // manifolds of the last step this world took part in
     uint64_t epoch = 0;
     uint32_t batch_index = 0;
+    bool fused_src = false;   // ... computed by the fused kernel (their range: w_man_begin / w_man_end)
+    // capacities of the world's slices of the fused step scratch (cell
+    // entries, raw candidates, manifolds, solver items); 0: initial estimate
+    uint32_t fz[4] = {0, 0, 0, 0};
 
     float* f(int field) { return reinterpret_cast<float*>(block.data() + (size_t)field * cap); }
     uint32_t* u(int field) { return block.data() + (size_t)field * cap; }
@@ -117,6 +121,7 @@ struct r2d_ctx {
     hipStream_t aux[2] = {nullptr, nullptr};      // side streams of the solve phase
     hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;   // brackets of the last launched batch
+    hipEvent_t ev_kend = nullptr;                      // end of the last fused kernel
     r2d_server* server = nullptr;
     // step scratch
     void* arena = nullptr;
@@ -149,6 +154,32 @@ struct r2d_ctx {
     uint32_t lds_nmax = 0;                        // 0: LDS body state up to the LDS capacity
 
     KernelTrace* trace = nullptr;                 // per-kernel timing (R2D_TRACE_KERNELS)
+
+    // whole steps per workgroup (k_fused) for batches of small worlds
+    bool fused_enabled = true;
+    void* farena = nullptr;                       // the worlds' slices (entries, candidates, manifolds, items)
+    size_t farena_bytes = 0;
+    uint32_t fcap_e = 0, fcap_r = 0, fcap_m = 0, fcap_i = 0;   // slice totals the arena holds
+    void* fworld = nullptr;                       // per-world arrays of the fused kernel
+    size_t fworld_cap = 0;
+    StepDev SF{};                                 // the fused kernel's view: S with the slice arrays in farena
+    std::vector<WorldScratch> fscr;               // current slices (host copy)
+    std::vector<WorldScratch> fscr_uploaded;
+    std::vector<uint32_t> forder, forder_uploaded;
+    uint32_t* hdone = nullptr;                    // pinned: steps done per world, last-step / needed counts
+    uint32_t* hcnt = nullptr;
+    WorldScratch* hscr = nullptr;                 // pinned: the slice table as the last launch left it
+    uint32_t* hpool = nullptr;                    // pinned: the arrays' free ends and capacities (S.fpool)
+    size_t hfused_cap = 0;
+    uint64_t* tstamp = nullptr;                   // per-world step times of a profiled fused call
+    size_t tstamp_words = 0;
+    uint64_t* wprof = nullptr;                    // per-world phase ticks of a profiled fused call
+    size_t wprof_worlds = 0;
+    double fused_phase_ms[FP_N] = {};             // summed over the worlds' profiled steps
+    std::vector<double> fused_step_ms;            // world-step latencies of profiled fused calls
+    double fused_ms = 0.0;                        // device time of profiled fused launches
+    uint64_t fused_launches = 0;
+    uint64_t fused_steps = 0;                     // world-steps of profiled fused launches
 
     Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}, trace}; }
 };
@@ -449,7 +480,241 @@ struct Batch {
     int attempts = 0;
     int first_err = R2D_OK;
     std::vector<int32_t> world_status;   // device status per world of the last wait
+    bool fused = false;                  // whole steps per workgroup (k_fused)
+    size_t fused_lds = 0;
+    int variant = 0;
 };
+
+// ------------------------------------------------ fused steps: scratch slices
+constexpr size_t kFusedLdsMax = 160 * 1024 - 1024;
+
+uint32_t grow_cap(uint32_t have, uint64_t need)
+{
+    if (need <= have) return have;
+    uint64_t v = need + need / 2 + 256;
+    return (uint32_t)std::min<uint64_t>(v, 0xffffff00ull);
+}
+
+// The fused arena: every world's slices of the entry, candidate, manifold and
+// item arrays.  With `preserve` the manifold arrays keep their contents (the
+// slices of worlds that finished before an overflow relaunch stay readable).
+int ensure_fused_arena(r2d_ctx* c, uint64_t te, uint64_t tr, uint64_t tm, uint64_t ti, bool preserve)
+{
+    if (te <= c->fcap_e && tr <= c->fcap_r && tm <= c->fcap_m && ti <= c->fcap_i && c->farena) return R2D_OK;
+    if (std::max(std::max(te, tr), std::max(tm, ti)) >= 0x7fffff00ull) return R2D_E_CAPACITY;
+    const uint32_t E = grow_cap(c->fcap_e, te), R = grow_cap(c->fcap_r, tr), M = grow_cap(c->fcap_m, tm),
+                   I = grow_cap(c->fcap_i, ti);
+    StepDev F{};
+    std::vector<std::pair<void**, size_t>> parts;
+    auto add = [&](void** p, size_t bytes) { parts.push_back({p, bytes}); };
+    add((void**)&F.bkt_items, (size_t)E * 4); add((void**)&F.bkt_sorted, (size_t)E * 4);
+    add((void**)&F.ent_body, (size_t)E * 4); add((void**)&F.ent_bkt, (size_t)E * 4);
+    add((void**)&F.ent_cnt, (size_t)E * 4);
+    add((void**)&F.cand, (size_t)R * 16); add((void**)&F.cand_flag, (size_t)R * 4);
+    add((void**)&F.sat_nx, (size_t)R * 4); add((void**)&F.sat_ny, (size_t)R * 4); add((void**)&F.sat_info, (size_t)R * 4);
+    add((void**)&F.m_ref, (size_t)M * 4); add((void**)&F.m_inc, (size_t)M * 4);
+    add((void**)&F.m_nx, (size_t)M * 4); add((void**)&F.m_ny, (size_t)M * 4); add((void**)&F.m_fric, (size_t)M * 4);
+    add((void**)&F.m_prev1, (size_t)M * 4); add((void**)&F.m_prev2, (size_t)M * 4); add((void**)&F.m_valid, (size_t)M * 4);
+    add((void**)&F.m_pt, (size_t)MP_NFIELDS * 2 * M * 4);
+    add((void**)&F.it_r0, (size_t)I * 4); add((void**)&F.it_r1, (size_t)I * 4);
+    add((void**)&F.it_k0, (size_t)I * 4); add((void**)&F.it_k1, (size_t)I * 4);
+    add((void**)&F.it_perm, (size_t)I * 4);
+    add((void**)&F.lists, 2 * (size_t)I * 4);
+    add((void**)&F.recs, (size_t)I * 128);
+    size_t bytes = 0;
+    for (auto& p : parts) bytes += align256(p.second);
+    void* arena = nullptr;
+    if (hipMalloc(&arena, bytes) != hipSuccess) return R2D_E_OOM;
+    size_t o = 0;
+    for (auto& p : parts) { *p.first = (char*)arena + o; o += align256(p.second); }
+    StepDev& O = c->SF;
+    if (c->farena && preserve) {
+        const size_t om = c->fcap_m;
+        void* const dst[8] = {F.m_ref, F.m_inc, F.m_nx, F.m_ny, F.m_fric, F.m_prev1, F.m_prev2, F.m_valid};
+        const void* const src[8] = {O.m_ref, O.m_inc, O.m_nx, O.m_ny, O.m_fric, O.m_prev1, O.m_prev2, O.m_valid};
+        for (int k = 0; k < 8; ++k)
+            HIPCHK(hipMemcpyAsync(dst[k], src[k], om * 4, hipMemcpyDeviceToDevice, c->stream));
+        for (int r = 0; r < MP_NFIELDS * 2; ++r)
+            HIPCHK(hipMemcpyAsync(F.m_pt + (size_t)r * M, O.m_pt + (size_t)r * om, om * 4, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    if (c->farena) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->farena));
+    }
+    c->farena = arena;
+    c->farena_bytes = bytes;
+    c->fcap_e = E; c->fcap_r = R; c->fcap_m = M; c->fcap_i = I;
+    O.bkt_items = F.bkt_items; O.bkt_sorted = F.bkt_sorted; O.ent_body = F.ent_body; O.ent_bkt = F.ent_bkt;
+    O.ent_cnt = F.ent_cnt; O.cand = F.cand; O.cand_flag = F.cand_flag; O.sat_nx = F.sat_nx; O.sat_ny = F.sat_ny;
+    O.sat_info = F.sat_info; O.m_ref = F.m_ref; O.m_inc = F.m_inc; O.m_nx = F.m_nx; O.m_ny = F.m_ny;
+    O.m_fric = F.m_fric; O.m_prev1 = F.m_prev1; O.m_prev2 = F.m_prev2; O.m_valid = F.m_valid; O.m_pt = F.m_pt;
+    O.it_r0 = F.it_r0; O.it_r1 = F.it_r1; O.it_k0 = F.it_k0; O.it_k1 = F.it_k1; O.it_perm = F.it_perm;
+    O.lists = F.lists; O.recs = F.recs;
+    O.cap_e = E; O.cap_r = R; O.cap_m = M; O.cap_i = I;
+    return R2D_OK;
+}
+
+// per-world arrays of the fused kernel (device) and their pinned host copies
+int ensure_fused_world(r2d_ctx* c, size_t W)
+{
+    if (W <= c->fworld_cap && c->fworld) return R2D_OK;
+    const size_t cap = std::max<size_t>(W, 2 * c->fworld_cap);
+    const size_t bytes = align256(cap * sizeof(WorldScratch)) + 4 * align256(cap * 4) + align256(cap * 32) + 256;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return R2D_E_OOM;
+    if (c->fworld) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->fworld));
+    }
+    c->fworld = p;
+    c->fworld_cap = cap;
+    char* q = (char*)p;
+    StepDev& O = c->SF;
+    O.wscr = (WorldScratch*)q; q += align256(cap * sizeof(WorldScratch));
+    O.w_done = (uint32_t*)q; q += align256(cap * 4);
+    O.w_man_begin = (uint32_t*)q; q += align256(cap * 4);
+    O.w_man_end = (uint32_t*)q; q += align256(cap * 4);
+    O.w_order = (uint32_t*)q; q += align256(cap * 4);
+    O.w_cnt = (uint32_t*)q; q += align256(cap * 32);
+    O.fpool = (uint32_t*)q;
+    c->fscr_uploaded.clear();
+    c->forder_uploaded.clear();
+    if (c->hfused_cap < cap) {
+        if (c->hdone) HIPCHK(hipHostFree(c->hdone));
+        if (c->hcnt) HIPCHK(hipHostFree(c->hcnt));
+        if (c->hscr) HIPCHK(hipHostFree(c->hscr));
+        c->hdone = nullptr;
+        c->hcnt = nullptr;
+        c->hscr = nullptr;
+        c->hfused_cap = 0;
+        if (hipHostMalloc((void**)&c->hdone, cap * 4) != hipSuccess) return R2D_E_OOM;
+        if (hipHostMalloc((void**)&c->hcnt, cap * 32) != hipSuccess) return R2D_E_OOM;
+        if (hipHostMalloc((void**)&c->hscr, cap * sizeof(WorldScratch)) != hipSuccess) return R2D_E_OOM;
+        c->hfused_cap = cap;
+    }
+    if (!c->hpool && hipHostMalloc((void**)&c->hpool, 8 * 4) != hipSuccess) return R2D_E_OOM;
+    return R2D_OK;
+}
+
+// slice capacities of a world: what it used before, else estimates (per
+// body 4 cell entries, 32 raw candidates -- a settled pile has about 24 --
+// and 3 manifolds)
+void fused_caps(const r2d_world* w, uint32_t cap[4])
+{
+    const uint32_t n = (uint32_t)w->n, nj = (uint32_t)w->joints.size();
+    cap[0] = w->fz[0] ? w->fz[0] : 4 * n + 64;
+    cap[1] = w->fz[1] ? w->fz[1] : 32 * n + 256;
+    cap[2] = w->fz[2] ? w->fz[2] : 3 * n + 64;
+    cap[3] = std::max(w->fz[3] ? w->fz[3] : nj + cap[2], nj + 1);
+}
+
+// end of the used part of each array (entries, candidates, manifolds, items)
+void fused_used(const r2d_ctx* c, uint64_t end[4])
+{
+    end[0] = end[1] = end[2] = end[3] = 0;
+    for (const WorldScratch& x : c->fscr) {
+        end[0] = std::max<uint64_t>(end[0], (uint64_t)x.e0 + x.ecap + 1);
+        end[1] = std::max<uint64_t>(end[1], (uint64_t)x.r0 + x.rcap + 1);
+        end[2] = std::max<uint64_t>(end[2], (uint64_t)x.m0 + x.mcap);
+        end[3] = std::max<uint64_t>(end[3], (uint64_t)x.i0 + x.icap);
+    }
+}
+
+// upload the slice table and the workgroup order when they changed
+int fused_upload_tables(r2d_ctx* c)
+{
+    const size_t nw = c->fscr.size();
+    if (c->fscr_uploaded.size() != nw ||
+        std::memcmp(c->fscr_uploaded.data(), c->fscr.data(), nw * sizeof(WorldScratch)) != 0) {
+        HIPCHK(hipMemcpyAsync((void*)c->SF.wscr, c->fscr.data(), nw * sizeof(WorldScratch), hipMemcpyHostToDevice,
+                              c->stream));
+        c->fscr_uploaded = c->fscr;
+    }
+    if (c->forder_uploaded != c->forder) {
+        HIPCHK(hipMemcpyAsync(c->SF.w_order, c->forder.data(), nw * 4, hipMemcpyHostToDevice, c->stream));
+        c->forder_uploaded = c->forder;
+    }
+    // the host copies are read by the asynchronous uploads above
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R2D_OK;
+}
+
+// slices of every world of a new batch, one after another
+int fused_layout(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    int rc = ensure_fused_world(c, nw);
+    if (rc) return rc;
+    c->fscr.resize(nw);
+    uint64_t te = 0, tr = 0, tm = 0, ti = 0;
+    for (size_t i = 0; i < nw; ++i) {
+        uint32_t cap[4];
+        fused_caps(b.worlds[i], cap);
+        c->fscr[i] = WorldScratch{(uint32_t)te, cap[0], (uint32_t)tr, cap[1], (uint32_t)tm, cap[2], (uint32_t)ti,
+                                  cap[3]};
+        te += (uint64_t)cap[0] + 1; tr += (uint64_t)cap[1] + 1; tm += cap[2]; ti += cap[3];
+    }
+    // as much again free for worlds that outgrow their slices during the call
+    if ((rc = ensure_fused_arena(c, 2 * te, 2 * tr, 2 * tm, 2 * ti, false))) return rc;
+    // workgroups take the largest worlds first (the longest steps start first)
+    c->forder.resize(nw);
+    for (size_t i = 0; i < nw; ++i) c->forder[i] = (uint32_t)i;
+    std::stable_sort(c->forder.begin(), c->forder.end(),
+                     [&](uint32_t x, uint32_t y) { return b.worlds[x]->n > b.worlds[y]->n; });
+    return fused_upload_tables(c);
+}
+
+// after an overflow: the worlds that stopped get larger slices appended at
+// the end (the other slices, and the manifolds they hold, stay where they are)
+int fused_regrow(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    uint64_t end[4];
+    fused_used(c, end);
+    uint64_t te = end[0], tr = end[1], tm = end[2], ti = end[3];
+    for (size_t i = 0; i < nw; ++i) {
+        if (c->hstatus[i] != WS_OVERFLOW) continue;
+        r2d_world* w = b.worlds[i];
+        const uint32_t* need = c->hcnt + 8 * i + 4;
+        uint32_t cap[4];
+        fused_caps(w, cap);
+        for (int k = 0; k < 4; ++k)
+            if (need[k] > cap[k]) cap[k] = (uint32_t)std::min<uint64_t>((uint64_t)need[k] * 2 + 64, 0x7ffffff0u);
+        cap[3] = std::max(cap[3], (uint32_t)w->joints.size() + cap[2]);
+        for (int k = 0; k < 4; ++k) w->fz[k] = cap[k];
+        c->fscr[i] = WorldScratch{(uint32_t)te, cap[0], (uint32_t)tr, cap[1], (uint32_t)tm, cap[2], (uint32_t)ti,
+                                  cap[3]};
+        te += (uint64_t)cap[0] + 1; tr += (uint64_t)cap[1] + 1; tm += cap[2]; ti += cap[3];
+        c->hstatus[i] = WS_OK;
+    }
+    int rc = ensure_fused_arena(c, te + te / 2, tr + tr / 2, tm + tm / 2, ti + ti / 2, true);
+    if (rc) return rc;
+    if ((rc = fused_upload_tables(c))) return rc;
+    HIPCHK(hipMemcpyAsync(c->S.w_status, c->hstatus, nw * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R2D_OK;
+}
+
+// the fused kernel's view: the step scratch's per-body / per-world arrays
+// and the fused arena's slices
+StepDev fused_view(const r2d_ctx* c)
+{
+    StepDev F = c->S;
+    const StepDev& O = c->SF;
+    F.bkt_items = O.bkt_items; F.bkt_sorted = O.bkt_sorted; F.ent_body = O.ent_body; F.ent_bkt = O.ent_bkt;
+    F.ent_cnt = O.ent_cnt; F.cand = O.cand; F.cand_flag = O.cand_flag; F.sat_nx = O.sat_nx; F.sat_ny = O.sat_ny;
+    F.sat_info = O.sat_info; F.m_ref = O.m_ref; F.m_inc = O.m_inc; F.m_nx = O.m_nx; F.m_ny = O.m_ny;
+    F.m_fric = O.m_fric; F.m_prev1 = O.m_prev1; F.m_prev2 = O.m_prev2; F.m_valid = O.m_valid; F.m_pt = O.m_pt;
+    F.it_r0 = O.it_r0; F.it_r1 = O.it_r1; F.it_k0 = O.it_k0; F.it_k1 = O.it_k1; F.it_perm = O.it_perm;
+    F.lists = O.lists; F.recs = O.recs;
+    F.cap_e = O.cap_e; F.cap_r = O.cap_r; F.cap_m = O.cap_m; F.cap_i = O.cap_i;
+    F.wscr = O.wscr; F.fpool = O.fpool; F.w_done = O.w_done; F.w_cnt = O.w_cnt; F.w_man_begin = O.w_man_begin;
+    F.w_man_end = O.w_man_end; F.w_order = O.w_order;
+    F.w_tstamp = nullptr; F.tcap = 0; F.w_prof = nullptr;
+    return F;
+}
+
 
 int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, size_t sub_steps, size_t iters,
                   size_t steps, Batch& b)
@@ -490,12 +755,7 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         sizes[i] = (uint32_t)w->n;
     }
     if (b.N >= 0x7fffffffull || b.B >= 0x7fffffffull) return R2D_E_CAPACITY;
-    uint32_t est_e = std::max<uint32_t>(c->cap_e, (uint32_t)(4 * b.N + 1024));
-    uint32_t est_r = std::max<uint32_t>(c->cap_r, (uint32_t)(8 * b.N + 1024));
-    uint32_t est_m = std::max<uint32_t>(c->cap_m, (uint32_t)(4 * b.N + 1024));
-    int rc = ensure_scratch(c, (uint32_t)b.N, (uint32_t)b.B, (uint32_t)nw, (uint32_t)b.J, est_e, est_r, est_m,
-                            (uint32_t)b.J + est_m);
-    if (rc) return rc;
+
     // solve variant: context override (autotuner), environment, then the
     // default (register-slot kernel for small worlds, 256-thread record
     // kernels for the others)
@@ -508,7 +768,34 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
     bool colored = false;
     for (size_t i = 0; i < nw; ++i) colored = colored || worlds[i]->sched != 0;
     b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored);
-    return R2D_OK;
+    b.variant = variant;
+    // whole steps per workgroup: worlds of at most FUSED_NMAX bodies in the
+    // reference order whose broadphase and solve arrays fit LDS
+    bool fused = c->fused_enabled && fused_available(variant) && !colored;
+    uint32_t maxn = 0, maxtb = 0;
+    for (size_t i = 0; i < nw && fused; ++i) {
+        fused = sizes[i] <= FUSED_NMAX;
+        maxn = std::max(maxn, sizes[i]);
+        maxtb = std::max(maxtb, b.descs[i].table);
+    }
+    if (fused) {
+        b.fused_lds = fused_lds_bytes(maxn, maxtb);
+        fused = b.fused_lds <= kFusedLdsMax;
+    }
+    b.fused = fused;
+    int rc;
+    if (fused) {
+        // the step scratch holds the per-body and per-world arrays; the
+        // worlds' slices live in the fused arena
+        if ((rc = ensure_scratch(c, (uint32_t)b.N, (uint32_t)b.B, (uint32_t)nw, (uint32_t)b.J, 1, 1, 1, 1)))
+            return rc;
+        return fused_layout(c, b);
+    }
+    uint32_t est_e = std::max<uint32_t>(c->cap_e, (uint32_t)(4 * b.N + 1024));
+    uint32_t est_r = std::max<uint32_t>(c->cap_r, (uint32_t)(8 * b.N + 1024));
+    uint32_t est_m = std::max<uint32_t>(c->cap_m, (uint32_t)(4 * b.N + 1024));
+    return ensure_scratch(c, (uint32_t)b.N, (uint32_t)b.B, (uint32_t)nw, (uint32_t)b.J, est_e, est_r, est_m,
+                          (uint32_t)b.J + est_m);
 }
 
 // sum the phase times of the timed steps recorded so far
@@ -531,6 +818,8 @@ int timer_flush(r2d_ctx* c)
     return R2D_OK;
 }
 
+int fused_launch(r2d_ctx* c, Batch& b);
+
 int batch_launch(r2d_ctx* c, Batch& b)
 {
     const size_t nw = b.worlds.size();
@@ -539,6 +828,7 @@ int batch_launch(r2d_ctx* c, Batch& b)
         HIPCHK(hipMemcpyAsync(c->dworlds, b.descs.data(), nw * sizeof(WorldDev), hipMemcpyHostToDevice, c->stream));
         c->uploaded_descs = b.descs;
     }
+    if (b.fused) return fused_launch(c, b);
     StepDev S = c->S;
     S.nb = (uint32_t)b.N; S.nworlds = (uint32_t)nw; S.nbuckets = (uint32_t)b.B; S.njoints = (uint32_t)b.J;
     S.worlds = c->dworlds;
@@ -608,8 +898,162 @@ int batch_launch(r2d_ctx* c, Batch& b)
     return R2D_OK;
 }
 
+// the steps of a fused launch: one kernel for the whole call
+int fused_launch(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    StepDev F = fused_view(c);
+    F.nb = (uint32_t)b.N; F.nworlds = (uint32_t)nw; F.nbuckets = (uint32_t)b.B; F.njoints = (uint32_t)b.J;
+    F.worlds = c->dworlds;
+    if (c->profiling) {
+        const size_t words = nw * (b.steps + 1);
+        if (c->tstamp_words < words) {
+            if (c->tstamp) HIPCHK(hipFree(c->tstamp));
+            c->tstamp = nullptr;
+            c->tstamp_words = 0;
+            if (hipMalloc((void**)&c->tstamp, words * 8) != hipSuccess) return R2D_E_OOM;
+            c->tstamp_words = words;
+        }
+        if (b.attempts == 0) HIPCHK(hipMemsetAsync(c->tstamp, 0, words * 8, c->stream));
+        F.w_tstamp = c->tstamp;
+        F.tcap = (uint32_t)b.steps;
+        if (c->wprof_worlds < nw) {
+            if (c->wprof) HIPCHK(hipFree(c->wprof));
+            c->wprof = nullptr;
+            c->wprof_worlds = 0;
+            if (hipMalloc((void**)&c->wprof, nw * FP_N * 8) != hipSuccess) return R2D_E_OOM;
+            c->wprof_worlds = nw;
+        }
+        if (b.attempts == 0) HIPCHK(hipMemsetAsync(c->wprof, 0, nw * FP_N * 8, c->stream));
+        F.w_prof = c->wprof;
+    }
+    HIPCHK(hipMemsetAsync(F.counters, 0, 64 * 4, c->stream));
+    // free ends of the arrays: where outgrown worlds take new slices
+    uint64_t end[4];
+    fused_used(c, end);
+    const uint32_t caps[4] = {c->fcap_e, c->fcap_r, c->fcap_m, c->fcap_i};
+    for (int k = 0; k < 4; ++k) {
+        c->hpool[k] = (uint32_t)std::min<uint64_t>(end[k], caps[k]);
+        c->hpool[4 + k] = caps[k];
+    }
+    HIPCHK(hipMemcpyAsync(F.fpool, c->hpool, 8 * 4, hipMemcpyHostToDevice, c->stream));
+    if (b.attempts == 0) {
+        HIPCHK(hipMemsetAsync(F.w_done, 0, nw * 4, c->stream));
+        HIPCHK(hipMemsetAsync(F.w_status, 0, nw * 4, c->stream));
+        HIPCHK(hipMemsetAsync(F.w_cnt, 0, nw * 32, c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev_begin, c->stream));
+    (void)hipGetLastError();   // the launch check below sees this launch only
+    launch_fused(F, b.variant, (uint32_t)nw, b.dt, (uint32_t)b.sub, (uint32_t)b.iters, (uint32_t)b.steps,
+                 b.fused_lds, c->stream);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        hipFuncAttributes fa{};
+        (void)fused_attributes(b.variant, &fa);
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "fused kernel launch failed: %s (%d); variant %d, %zu worlds, %zu B dynamic LDS; kernel: %d "
+                      "regs, %zu B static LDS, %zu B max dynamic LDS, %zu B scratch per lane, %d max threads",
+                      hipGetErrorString(le), (int)le, b.variant, nw, b.fused_lds, fa.numRegs, fa.sharedSizeBytes,
+                      (size_t)fa.maxDynamicSharedSizeBytes, fa.localSizeBytes, fa.maxThreadsPerBlock);
+        return R2D_E_DEVICE;
+    }
+    HIPCHK(hipEventRecord(c->ev_kend, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hcounters, F.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hstatus, F.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hdone, F.w_done, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hcnt, F.w_cnt, nw * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hscr, F.wscr, nw * sizeof(WorldScratch), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->ev_end, c->stream));
+    b.attempts += 1;
+    return R2D_OK;
+}
+
+// per-world step latencies of a profiled fused launch (wall clock at 100 MHz)
+int fused_collect_times(r2d_ctx* c, const Batch& b)
+{
+    const size_t nw = b.worlds.size(), stride = b.steps + 1;
+    std::vector<uint64_t> ts(nw * stride);
+    HIPCHK(hipMemcpy(ts.data(), c->tstamp, ts.size() * 8, hipMemcpyDeviceToHost));
+    int hz = 0;
+    if (hipDeviceGetAttribute(&hz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || hz <= 0) hz = 100000;
+    const double ms_per_tick = 1.0 / (double)hz;   // rate in kHz
+    for (size_t i = 0; i < nw; ++i)
+        for (size_t s = 0; s < b.steps; ++s) {
+            const uint64_t a = ts[i * stride + s], e = ts[i * stride + s + 1];
+            if (a && e && e >= a) c->step_ms.push_back((float)((double)(e - a) * ms_per_tick));
+        }
+    std::vector<uint64_t> ph(nw * FP_N);
+    HIPCHK(hipMemcpy(ph.data(), c->wprof, ph.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nw; ++i)
+        for (int k = 0; k < FP_N; ++k) c->fused_phase_ms[k] += (double)ph[i * FP_N + k] * ms_per_tick;
+    return R2D_OK;
+}
+
+int fused_wait(r2d_ctx* c, Batch& b)
+{
+    const size_t nw = b.worlds.size();
+    for (;;) {
+        HIPCHK(hipEventSynchronize(c->ev_end));
+        if (c->profiling) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev_begin, c->ev_kend));
+            c->fused_ms += ms;
+            c->fused_launches += 1;
+        }
+        b.world_status.assign(c->hstatus, c->hstatus + nw);
+        // slices the worlds moved to during the launch; later layouts start from them
+        c->fscr.assign(c->hscr, c->hscr + nw);
+        c->fscr_uploaded = c->fscr;
+        for (size_t i = 0; i < nw; ++i) {
+            const WorldScratch& x = c->fscr[i];
+            uint32_t* fz = b.worlds[i]->fz;
+            fz[0] = std::max(fz[0], x.ecap); fz[1] = std::max(fz[1], x.rcap);
+            fz[2] = std::max(fz[2], x.mcap); fz[3] = std::max(fz[3], x.icap);
+        }
+        bool overflow = false;
+        for (size_t i = 0; i < nw; ++i) {
+            const int32_t st = c->hstatus[i];
+            if (st == WS_OVERFLOW) { overflow = true; continue; }
+            if (st == WS_STALL) std::snprintf(g_last_error, sizeof g_last_error, "solver schedule stalled in world %zu", i);
+            if (st != WS_OK && b.first_err == R2D_OK)
+                b.first_err = st == WS_NONFINITE ? R2D_E_NONFINITE : st == WS_STALL ? R2D_E_DEVICE : R2D_E_RANGE;
+        }
+        for (int k = 0; k < 4; ++k) c->last_counts[k] = c->hcounters[k];
+        if (!overflow) break;
+        if (b.attempts > 16) return R2D_E_CAPACITY;
+        int rc = fused_regrow(c, b);
+        if (rc) return rc;
+        if ((rc = fused_launch(c, b))) return rc;
+    }
+    if (c->profiling) {
+        int rc = fused_collect_times(c, b);
+        if (rc) return rc;
+        for (size_t i = 0; i < nw; ++i) c->fused_steps += c->hdone[i];
+    }
+    b.done = b.steps;
+    return R2D_OK;
+}
+
+// the worlds of a completed batch: their manifolds are this batch's
+int batch_finish(r2d_ctx* c, Batch& b)
+{
+    c->epoch = g_epoch.fetch_add(1) + 1;   // unique over contexts: a world's epoch names one batch
+    for (size_t i = 0; i < b.worlds.size(); ++i) {
+        b.worlds[i]->host_valid = false;
+        b.worlds[i]->epoch = c->epoch;
+        b.worlds[i]->batch_index = (uint32_t)i;
+        b.worlds[i]->fused_src = b.fused;
+    }
+    return b.first_err;
+}
+
 int batch_wait(r2d_ctx* c, Batch& b)
 {
+    if (b.fused) {
+        int rc = fused_wait(c, b);
+        if (rc) return rc;
+        return batch_finish(c, b);
+    }
     const size_t nw = b.worlds.size();
     for (;;) {
         HIPCHK(hipEventSynchronize(c->ev_end));
@@ -643,13 +1087,7 @@ int batch_wait(r2d_ctx* c, Batch& b)
         int rc = batch_launch(c, b);
         if (rc) return rc;
     }
-    c->epoch = g_epoch.fetch_add(1) + 1;   // unique over contexts: a world's epoch names one batch
-    for (size_t i = 0; i < nw; ++i) {
-        b.worlds[i]->host_valid = false;
-        b.worlds[i]->epoch = c->epoch;
-        b.worlds[i]->batch_index = (uint32_t)i;
-    }
-    return b.first_err;
+    return batch_finish(c, b);
 }
 
 // device time of the last launched batch (its final launch when it was relaunched)
@@ -694,10 +1132,15 @@ void fill_state(r2d_world* w, int32_t s, r2d_body_state* o)
 
 // ---------------------------------------------------------------- serving
 // Admission estimate of one resident world: its device block and topology
-// plus its share of the step scratch at the initial capacity estimates
-// (per body: cell ranges, 4 cell entries, 8 raw candidates, 4 manifolds and
-// 4 solver items; per hash bucket 12 B; per joint one solver item).
-constexpr size_t kScratchPerBody = 96 + 4 * 24 + 8 * 36 + 4 * 128 + 4 * 160;
+// plus its share of the step scratch at the initial capacity estimates: the
+// phase kernels' (per body: cell ranges, 4 cell entries, 8 raw candidates, 4
+// manifolds and 4 solver items) or the fused kernel's slices and their free
+// reserve, whichever is larger (per body 2 x (4 entries x 20 B + 32
+// candidates x 32 B + 3 manifolds x 128 B + 3 items x 156 B)); per hash
+// bucket 12 B; per joint one solver item.
+constexpr size_t kScratchClassic = 96 + 4 * 24 + 8 * 36 + 4 * 128 + 4 * 160;
+constexpr size_t kScratchFused = 96 + 2 * (4 * 20 + 32 * 32 + 3 * 128 + 3 * 156);
+constexpr size_t kScratchPerBody = kScratchClassic > kScratchFused ? kScratchClassic : kScratchFused;
 constexpr size_t kScratchPerBucket = 12;
 constexpr size_t kScratchPerJoint = 160;
 constexpr size_t kScratchPerWorld = 20 + sizeof(WorldDev);
@@ -920,7 +1363,9 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c->ev_begin));
     HIPCHK(hipEventCreate(&c->ev_end));
+    HIPCHK(hipEventCreate(&c->ev_kend));
     if (hipHostMalloc((void**)&c->hcounters, 64 * 4) != hipSuccess) { delete c; return R2D_E_OOM; }
+    if (const char* e = std::getenv("R2D_NO_FUSED")) c->fused_enabled = !(e[0] == '1');
     set_solve_smem_limit();
     if (const char* e = std::getenv("R2D_NO_GRAPH")) c->use_graph = !(e[0] == '1');
     if (const char* e = std::getenv("R2D_TRACE_KERNELS")) {
@@ -952,6 +1397,15 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c->join[i]);
     (void)hipEventDestroy(c->ev_begin);
     (void)hipEventDestroy(c->ev_end);
+    (void)hipEventDestroy(c->ev_kend);
+    if (c->farena) (void)hipFree(c->farena);
+    if (c->fworld) (void)hipFree(c->fworld);
+    if (c->tstamp) (void)hipFree(c->tstamp);
+    if (c->wprof) (void)hipFree(c->wprof);
+    if (c->hdone) (void)hipHostFree(c->hdone);
+    if (c->hcnt) (void)hipHostFree(c->hcnt);
+    if (c->hscr) (void)hipHostFree(c->hscr);
+    if (c->hpool) (void)hipHostFree(c->hpool);
     for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(c->aux[i]);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1261,9 +1715,15 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
         if (n) *n = 0;
         return w->epoch == 0 ? R2D_OK : R2D_E_RANGE;
     }
-    StepDev& S = c->S;
+    // manifolds of the fused kernel are in the world's slice of the fused arena
+    const StepDev S = w->fused_src ? fused_view(c) : c->S;
     uint32_t b[2];
-    HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 8, hipMemcpyDeviceToHost));
+    if (w->fused_src) {
+        HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b + 1, S.w_man_end + w->batch_index, 4, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 8, hipMemcpyDeviceToHost));
+    }
     size_t cnt = b[1] - b[0];
     if (n) *n = cnt;
     if (!out) return R2D_OK;
@@ -1363,6 +1823,33 @@ R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
     for (int i = 0; i < 4; ++i) c->phase_ms[i] = 0;
     c->phase_steps = 0;
     c->step_ms.clear();
+    c->fused_ms = 0.0;
+    c->fused_launches = 0;
+    c->fused_steps = 0;
+    for (int k = 0; k < FP_N; ++k) c->fused_phase_ms[k] = 0.0;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_fused_phase_times(r2d_ctx* c, double* out_ms)
+{
+    if (!c || !out_ms) return R2D_E_INVALID_ARG;
+    for (int k = 0; k < FP_N; ++k) out_ms[k] = c->fused_phase_ms[k];
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_fused_times(r2d_ctx* c, double* kernel_ms, uint64_t* launches, uint64_t* world_steps)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    if (kernel_ms) *kernel_ms = c->fused_ms;
+    if (launches) *launches = c->fused_launches;
+    if (world_steps) *world_steps = c->fused_steps;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_set_fused(r2d_ctx* c, int on)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    c->fused_enabled = on != 0;
     return R2D_OK;
 }
 

@@ -66,6 +66,24 @@ enum : int {
     C_COUNT = 8
 };
 
+// A world's slices of the step scratch when whole steps of the world run in
+// one workgroup (k_fused): cell entries [e0, e0 + ecap] (bkt_items,
+// bkt_sorted, ent_body, ent_bkt, ent_cnt), raw candidates [r0, r0 + rcap]
+// (cand, cand_flag, sat_*), manifolds [m0, m0 + mcap), solver items
+// [i0, i0 + icap) (it_*, recs) and their lists [2 i0, 2 (i0 + icap)).
+struct WorldScratch {
+    uint32_t e0, ecap, r0, rcap, m0, mcap, i0, icap;
+};
+
+// world status codes (w_status)
+enum : int32_t {
+    WS_OK = 0,
+    WS_CELLS = 3,        // cell range too large to enumerate
+    WS_NONFINITE = 4,    // non-finite AABB
+    WS_STALL = 5,        // solver dataflow stalled (corrupt ordering data)
+    WS_OVERFLOW = 6      // k_fused: the world's scratch slice is too small (w_cnt[4..7] = counts needed)
+};
+
 // per-step scratch, sized by capacities chosen on the host
 struct StepDev {
     uint32_t nb;             // batch bodies N
@@ -119,7 +137,19 @@ struct StepDev {
     uint32_t* w_nc;          // [W] bodies in the compact chain
     uint32_t* counters;
     uint32_t* partials;                    // scan scratch
+    // whole steps of each world in one workgroup (k_fused)
+    WorldScratch* wscr;        // [W] the world's slices of the arrays above (a world may move to a larger one)
+    uint32_t* fpool;           // [8] next free index and capacity of the entry, candidate, manifold and item arrays
+    uint32_t* w_done;          // [W] steps the world has completed in the current call
+    uint32_t* w_cnt;           // [8W] E, R, M, items of the world's last step; then the counts an overflow needed
+    uint32_t* w_man_end;       // [W] end of the world's manifolds (from w_man_begin)
+    uint64_t* w_tstamp;        // [W (tcap + 1)] wall clock at the call's first step and after each step, or null
+    uint32_t tcap;
+    uint64_t* w_prof;          // [W FP_N] wall-clock ticks per phase of the fused kernel, or null
 };
+
+// phases of a fused step (profiled with w_prof)
+enum : int { FP_PREP = 0, FP_BUCKETS, FP_CANDS, FP_NARROW, FP_CONTACTS, FP_SCHED, FP_SOLVE, FP_N = 8 };
 
 // manifold point fields: m_pt[(field * 2 + point) * cap_m + m]
 enum : int {

@@ -82,4 +82,16 @@ void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t
 
 void set_solve_smem_limit();
 
+// Whole steps per workgroup (k_fused) for worlds of at most FUSED_NMAX bodies
+// in the reference order; the variant picks the solve's slot shape.
+constexpr uint32_t FUSED_NMAX = 1024;
+bool fused_available(int variant);
+// dynamic LDS of the fused kernel for a world of n bodies and tb hash buckets
+size_t fused_lds_bytes(uint32_t n, uint32_t tb);
+// one workgroup per world (S.w_order), up to nsteps steps per world (S.w_done)
+// the fused kernel of a variant as compiled (registers, LDS, scratch): for error reports
+hipError_t fused_attributes(int variant, hipFuncAttributes* out);
+void launch_fused(const StepDev& S, int variant, uint32_t nworlds, float dt, uint32_t sub_steps, uint32_t iters,
+                  uint32_t nsteps, size_t lds, hipStream_t st);
+
 }  // namespace r2d

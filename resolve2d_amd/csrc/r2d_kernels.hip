@@ -344,51 +344,74 @@ __global__ void k_fill_bworld(StepDev S)
     for (int i = threadIdx.x; i < D.nbodies; i += blockDim.x) S.bworld[D.body_base + i] = blockIdx.x;
 }
 
-// SpatialHash.iterateAABBHashes cell range (SpatialHash.zig:83-96) from the
-// stale AABB, plus per-step trig and scheduling flags.
+// One body's broadphase preparation: cos / sin of the angle, the scheduling
+// flag (a static body orders contacts only if a contact's "+= 0" could change
+// it: a -0 component; joint-touched bodies are added later) and the cell
+// range of SpatialHash.iterateAABBHashes (SpatialHash.zig:83-96) from the
+// stale AABB.  err: 3 cell range too large to enumerate, 4 non-finite AABB
+// (@intFromFloat is undefined in the reference), 0 otherwise.
+struct Prep {
+    float c, s;
+    uint32_t res;
+    int64_t x0, y0;
+    uint32_t nx, ny, ncell;
+    int err;
+};
+
+__device__ __forceinline__ Prep prep_body(const WorldDev& D, uint32_t sl)
+{
+    Prep o;
+    float a = D.ang[sl];
+    o.c = cosf_r(a);
+    o.s = sinf_r(a);
+    uint32_t f = D.flags[sl];
+    o.res = 1u;
+    if (f & BF_STATIC) {
+        float m0 = D.mx[sl], m1 = D.my[sl], m2 = D.L[sl];
+        bool negz = (m0 == 0.0f && __builtin_signbit(m0)) || (m1 == 0.0f && __builtin_signbit(m1)) ||
+                    (m2 == 0.0f && __builtin_signbit(m2));
+        o.res = negz ? 1u : 0u;
+    }
+    o.err = 0;
+    o.ncell = 0;
+    o.x0 = o.y0 = 0;
+    o.nx = o.ny = 0;
+    float cell = D.cell;
+    float ax = D.ax[sl], ay = D.ay[sl], hw = D.ahw[sl], hh = D.ahh[sl];
+    float v0 = floorf((ax - hw) / cell), v1 = floorf((ay - hh) / cell);
+    float v2f = floorf((ax + hw) / cell), v3 = floorf((ay + hh) / cell);
+    const float LIM = 4.0e18f;
+    bool ok = (v0 > -LIM && v0 < LIM) && (v1 > -LIM && v1 < LIM) && (v2f > -LIM && v2f < LIM) &&
+              (v3 > -LIM && v3 < LIM);
+    if (ok) {
+        int64_t x0 = (int64_t)v0, y0 = (int64_t)v1, x1 = (int64_t)v2f, y1 = (int64_t)v3;
+        uint64_t nx = (x1 >= x0) ? (uint64_t)(x1 - x0 + 1) : 0u;
+        uint64_t ny = (y1 >= y0) ? (uint64_t)(y1 - y0 + 1) : 0u;
+        uint64_t tot = nx * ny;
+        if (nx > (1u << 30) || ny > (1u << 30) || tot > (1u << 30)) {
+            o.err = 3;
+        } else {
+            o.ncell = (uint32_t)tot;
+            o.x0 = x0; o.y0 = y0; o.nx = (uint32_t)nx; o.ny = (uint32_t)ny;
+        }
+    } else {
+        o.err = 4;
+    }
+    return o;
+}
+
 __global__ void k_prep(StepDev S)
 {
     for (uint32_t i = grid_tid(); i < S.nb; i += grid_stride()) {
         uint32_t w = S.bworld[i];
         const WorldDev& D = S.worlds[w];
-        uint32_t sl = i - D.body_base;
-        float a = D.ang[sl];
-        S.cc[i] = cosf_r(a);
-        S.cs[i] = sinf_r(a);
-        uint32_t f = D.flags[sl];
-        uint32_t res = 1u;
-        if (f & BF_STATIC) {
-            // a static body orders contacts only if a contact's "+= 0" could
-            // change it: a -0 component (joint-touched bodies are added later)
-            float m0 = D.mx[sl], m1 = D.my[sl], m2 = D.L[sl];
-            bool negz = (m0 == 0.0f && __builtin_signbit(m0)) || (m1 == 0.0f && __builtin_signbit(m1)) ||
-                        (m2 == 0.0f && __builtin_signbit(m2));
-            res = negz ? 1u : 0u;
-        }
-        S.sflag[i] = res;
-        float cell = D.cell;
-        float ax = D.ax[sl], ay = D.ay[sl], hw = D.ahw[sl], hh = D.ahh[sl];
-        float v0 = floorf((ax - hw) / cell), v1 = floorf((ay - hh) / cell);
-        float v2f = floorf((ax + hw) / cell), v3 = floorf((ay + hh) / cell);
-        const float LIM = 4.0e18f;
-        bool ok = (v0 > -LIM && v0 < LIM) && (v1 > -LIM && v1 < LIM) && (v2f > -LIM && v2f < LIM) &&
-                  (v3 > -LIM && v3 < LIM);
-        uint32_t nc = 0;
-        if (ok) {
-            int64_t x0 = (int64_t)v0, y0 = (int64_t)v1, x1 = (int64_t)v2f, y1 = (int64_t)v3;
-            uint64_t nx = (x1 >= x0) ? (uint64_t)(x1 - x0 + 1) : 0u;
-            uint64_t ny = (y1 >= y0) ? (uint64_t)(y1 - y0 + 1) : 0u;
-            uint64_t tot = nx * ny;
-            if (nx > (1u << 30) || ny > (1u << 30) || tot > (1u << 30)) {
-                S.w_status[w] = 3;   // cell range too large to enumerate
-            } else {
-                nc = (uint32_t)tot;
-                S.cx0[i] = x0; S.cy0[i] = y0; S.cnx[i] = (uint32_t)nx; S.cny[i] = (uint32_t)ny;
-            }
-        } else {
-            S.w_status[w] = 4;       // non-finite AABB: @intFromFloat is undefined in the reference
-        }
-        S.ncell[i] = nc;
+        const Prep o = prep_body(D, i - D.body_base);
+        S.cc[i] = o.c;
+        S.cs[i] = o.s;
+        S.sflag[i] = o.res;
+        if (o.err) S.w_status[w] = o.err;
+        if (!o.err && o.ncell) { S.cx0[i] = o.x0; S.cy0[i] = o.y0; S.cnx[i] = o.nx; S.cny[i] = o.ny; }
+        S.ncell[i] = o.ncell;
     }
 }
 
@@ -484,9 +507,55 @@ __global__ void k_cand_emit(StepDev S)
 }
 
 // Solver.updateManifolds filters + performNarrowSAT (lib.zig:272-285,
-// collision.zig:299-320) for each raw candidate.  A candidate counts only at
-// the first position of its partner in the lower body's query list; later
-// occurrences are the reference's "already a manifold / already rejected".
+// collision.zig:299-320) for one raw candidate: bodies sa < sb (world slots),
+// found at cell k of sa.  ba: bucket of each cell of sa; bb: of each of the nb
+// cells of sb.  A candidate counts only at the first position of its partner
+// in the lower body's query list; later occurrences are the reference's
+// "already a manifold / already rejected".  Returns whether it collides, with
+// the reference face id | ref-is-sa << 2 in info and the normal.
+__device__ __forceinline__ bool narrow_pair(const WorldDev& D, uint32_t sa, uint32_t sb, uint32_t k,
+                                            const uint32_t* ba, const uint32_t* bb, uint32_t nb, float ca, float sa_sin,
+                                            float cb, float sb_sin, uint32_t& info, float& nx, float& ny)
+{
+    // the pair filters of updateManifolds (lib.zig:273-282) depend on the
+    // pair only, so they run first: the first-occurrence test below costs
+    // (cells of a before k) x (cells of b), large for wall-sized bodies
+    uint32_t fa = D.flags[sa], fb = D.flags[sb];
+    bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
+    if (ok && D.nexcl) ok = !excluded(D, D.id[sa], D.id[sb]);
+    if (ok) ok = aabb_hit(D, sa, sb);
+    // b first appears in a's query list at cell k unless an earlier cell of a
+    // hashes to a bucket b also occupies (buckets are deduplicated, so
+    // within one bucket b appears once)
+    if (ok) {
+        for (uint32_t j = 0; j < nb && ok; ++j) {
+            uint32_t h = bb[j];
+            for (uint32_t kk = 0; kk < k; ++kk)
+                if (ba[kk] == h) { ok = false; break; }
+        }
+    }
+    if (!ok) return false;
+    Geo ga = load_geo(D, sa, ca, sa_sin);
+    Geo gb = load_geo(D, sb, cb, sb_sin);
+    // performNarrowSAT: lower id is the first reference body
+    bool a_first = ga.id < gb.id;
+    const Geo o1 = a_first ? ga : gb;   // by value: selecting references would put both in scratch
+    const Geo o2 = a_first ? gb : ga;
+    Sat sat;
+    const float undef = -3.0316488252093987e-13f;   // Zig Debug fill of `undefined` (0xAA)
+    sat.normal = mk(undef, undef);
+    sat.pen = __builtin_inff();
+    sat.nid = 0; sat.ref = 0; sat.set = false;
+    if (overlap_sat(sat, o1, o2, 0) && overlap_sat(sat, o2, o1, 1)) {
+        // ref body expressed relative to (a, b)
+        bool ref_is_a = (sat.ref == 0) == a_first;
+        info = (uint32_t)sat.nid | (ref_is_a ? 4u : 0u);
+        nx = sat.normal.x; ny = sat.normal.y;
+        return true;
+    }
+    return false;
+}
+
 __global__ void k_narrow(StepDev S)
 {
     if (S.counters[C_OVERFLOW]) return;
@@ -494,55 +563,15 @@ __global__ void k_narrow(StepDev S)
     for (uint32_t q = grid_tid(); q < R; q += grid_stride()) {
         const uint4 cq = S.cand[q];
         const uint32_t a = cq.x, b = cq.y, k = cq.z;
-        uint32_t flag = 0, info = 0;
+        uint32_t info = 0;
         float nx = 0.f, ny = 0.f;
-        // the pair filters of updateManifolds (lib.zig:273-282) depend on the
-        // pair only, so they run first: the first-occurrence test below costs
-        // (cells of a before k) x (cells of b), large for wall-sized bodies
-        uint32_t w = S.bworld[a];
-        const WorldDev& D = S.worlds[w];
-        uint32_t sa = a - D.body_base, sb = b - D.body_base;
-        uint32_t fa = D.flags[sa], fb = D.flags[sb];
-        bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
-        if (ok && D.nexcl) ok = !excluded(D, D.id[sa], D.id[sb]);
-        if (ok) ok = aabb_hit(D, sa, sb);
-        // b first appears in a's query list at cell k unless an earlier cell of a
-        // hashes to a bucket b also occupies (buckets are deduplicated, so
-        // within one bucket b appears once)
-        if (ok) {
-            const uint32_t* ba = S.ent_bkt + S.cell_off[a];
-            const uint32_t* bb = S.ent_bkt + S.cell_off[b];
-            uint32_t nb = S.ncell[b];
-            for (uint32_t j = 0; j < nb && ok; ++j) {
-                uint32_t h = bb[j];
-                for (uint32_t kk = 0; kk < k; ++kk)
-                    if (ba[kk] == h) { ok = false; break; }
-            }
-        }
-        if (ok) {
-            Geo ga = load_geo(D, sa, S.cc[a], S.cs[a]);
-            Geo gb = load_geo(D, sb, S.cc[b], S.cs[b]);
-            // performNarrowSAT: lower id is the first reference body
-            bool a_first = ga.id < gb.id;
-            const Geo o1 = a_first ? ga : gb;   // by value: selecting references would put both in scratch
-            const Geo o2 = a_first ? gb : ga;
-            Sat sat;
-            const float undef = -3.0316488252093987e-13f;   // Zig Debug fill of `undefined` (0xAA)
-            sat.normal = mk(undef, undef);
-            sat.pen = __builtin_inff();
-            sat.nid = 0; sat.ref = 0; sat.set = false;
-            if (overlap_sat(sat, o1, o2, 0) && overlap_sat(sat, o2, o1, 1)) {
-                flag = 1;
-                // ref body expressed relative to (a, b)
-                bool ref_is_a = (sat.ref == 0) == a_first;
-                info = (uint32_t)sat.nid | (ref_is_a ? 4u : 0u);
-                nx = sat.normal.x; ny = sat.normal.y;
-            }
-        }
-        S.cand_flag[q] = flag;
-        S.sat_info[q] = info;
-        S.sat_nx[q] = nx;
-        S.sat_ny[q] = ny;
+        const WorldDev& D = S.worlds[S.bworld[a]];
+        bool hit = narrow_pair(D, a - D.body_base, b - D.body_base, k, S.ent_bkt + S.cell_off[a],
+                               S.ent_bkt + S.cell_off[b], S.ncell[b], S.cc[a], S.cs[a], S.cc[b], S.cs[b], info, nx, ny);
+        S.cand_flag[q] = hit ? 1u : 0u;
+        S.sat_info[q] = hit ? info : 0u;
+        S.sat_nx[q] = hit ? nx : 0.f;
+        S.sat_ny[q] = hit ? ny : 0.f;
     }
 }
 
@@ -597,7 +626,64 @@ __device__ __forceinline__ void clip_line(v2 aa, v2 ab, v2 ba, v2 bb, v2& o1, v2
 }
 
 // identifyCollisionPoints: Disc (Disc.zig:99-123), Rectangle (Rectangle.zig:171-211),
-// with clipAgainstEdge (Rectangle.zig:213-238, Disc.zig:125-130)
+// with clipAgainstEdge (Rectangle.zig:213-238, Disc.zig:125-130), for the
+// collision of world slots sa, sb (narrow_pair's info and normal) into
+// manifold m; base: the world's first batch body index
+__device__ __forceinline__ void man_build_one(const StepDev& S, const WorldDev& D, uint32_t m, uint32_t base,
+                                              uint32_t sa, uint32_t sb, uint32_t info, v2 normal, float ca,
+                                              float sa_sin, float cb, float sb_sin)
+{
+    int nid = (int)(info & 3u);
+    bool ref_is_a = (info & 4u) != 0;
+    uint32_t sr = ref_is_a ? sa : sb, si = ref_is_a ? sb : sa;
+    Geo gr = ref_is_a ? load_geo(D, sr, ca, sa_sin) : load_geo(D, sr, cb, sb_sin);
+    Geo gi = ref_is_a ? load_geo(D, si, cb, sb_sin) : load_geo(D, si, ca, sa_sin);
+    S.m_ref[m] = base + sr;
+    S.m_inc[m] = base + si;
+    S.m_nx[m] = normal.x;
+    S.m_ny[m] = normal.y;
+    S.m_fric[m] = 0.f;
+    S.m_prev1[m] = D.ang[sr];
+    S.m_prev2[m] = D.ang[si];
+    uint32_t valid = 0;
+    if (!gr.rect) {
+        v2 pos = closest_point(gi, gr.pos);
+        Edge e = get_normal(gr, gi, nid);
+        v2 n = e.dir;
+        if (should_flip(n, gr, gi)) n = negself(n);
+        float d = dot2(n, sub2(pos, gr.pos));
+        write_point(S, m, 0, pos, d - gr.r, gr, gi, n);
+        valid = 1;
+    } else {
+        Edge e = get_normal(gr, gi, nid);
+        if (!gi.rect) {
+            v2 pos = addmult2(gi.pos, e.dir, -gi.r);
+            float d = dot2(sub2(pos, e.a), e.dir);
+            write_point(S, m, 0, pos, d, gr, gi, e.dir);
+            valid = 1;
+        } else {
+            // incident edge: first i minimising dot(normal, rot90ccw(normalize(v[i+1]-v[i])))
+            v2 ba = gi.wv[0], bb = gi.wv[1];
+            float best = __builtin_inff();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int nx = (i == 3) ? 0 : i + 1;
+                v2 tn = rot90ccw(normalize2(sub2(gi.wv[nx], gi.wv[i])));
+                float d = dot2(e.dir, tn);
+                if (d < best) { best = d; ba = gi.wv[i]; bb = gi.wv[nx]; }
+            }
+            v2 ca2, cb2;
+            clip_line(e.a, e.b, ba, bb, ca2, cb2);
+            int i = 0;
+            float d = dot2(sub2(ca2, e.a), e.dir);
+            if (d < COLLISION_MARGIN) { write_point(S, m, i, ca2, d, gr, gi, e.dir); valid |= 1u << i; i += 1; }
+            d = dot2(sub2(cb2, e.b), e.dir);
+            if (d < COLLISION_MARGIN) { write_point(S, m, i, cb2, d, gr, gi, e.dir); valid |= 1u << i; }
+        }
+    }
+    S.m_valid[m] = valid;
+}
+
 __global__ void k_man_build(StepDev S)
 {
     uint32_t R = dev_count(&S.counters[C_RAW], S.cap_r);
@@ -605,62 +691,11 @@ __global__ void k_man_build(StepDev S)
     if (S.counters[C_MANIFOLDS] > S.cap_m) { if (grid_tid() == 0) atomicOr(&S.counters[C_OVERFLOW], 4u); return; }
     for (uint32_t q = grid_tid(); q < R; q += grid_stride()) {
         if (!S.cand_flag[q]) continue;
-        uint32_t m = S.cand_idx[q];
         const uint4 cq = S.cand[q];
         const uint32_t a = cq.x, b = cq.y;
-        uint32_t info = S.sat_info[q];
-        int nid = (int)(info & 3u);
-        bool ref_is_a = (info & 4u) != 0;
-        uint32_t ref = ref_is_a ? a : b, inc = ref_is_a ? b : a;
         const WorldDev& D = S.worlds[S.bworld[a]];
-        uint32_t sr = ref - D.body_base, si = inc - D.body_base;
-        Geo gr = load_geo(D, sr, S.cc[ref], S.cs[ref]);
-        Geo gi = load_geo(D, si, S.cc[inc], S.cs[inc]);
-        v2 normal = mk(S.sat_nx[q], S.sat_ny[q]);
-        S.m_ref[m] = ref;
-        S.m_inc[m] = inc;
-        S.m_nx[m] = normal.x;
-        S.m_ny[m] = normal.y;
-        S.m_fric[m] = 0.f;
-        S.m_prev1[m] = D.ang[sr];
-        S.m_prev2[m] = D.ang[si];
-        uint32_t valid = 0;
-        if (!gr.rect) {
-            v2 pos = closest_point(gi, gr.pos);
-            Edge e = get_normal(gr, gi, nid);
-            v2 n = e.dir;
-            if (should_flip(n, gr, gi)) n = negself(n);
-            float d = dot2(n, sub2(pos, gr.pos));
-            write_point(S, m, 0, pos, d - gr.r, gr, gi, n);
-            valid = 1;
-        } else {
-            Edge e = get_normal(gr, gi, nid);
-            if (!gi.rect) {
-                v2 pos = addmult2(gi.pos, e.dir, -gi.r);
-                float d = dot2(sub2(pos, e.a), e.dir);
-                write_point(S, m, 0, pos, d, gr, gi, e.dir);
-                valid = 1;
-            } else {
-                // incident edge: first i minimising dot(normal, rot90ccw(normalize(v[i+1]-v[i])))
-                v2 ba = gi.wv[0], bb = gi.wv[1];
-                float best = __builtin_inff();
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int nx = (i == 3) ? 0 : i + 1;
-                    v2 tn = rot90ccw(normalize2(sub2(gi.wv[nx], gi.wv[i])));
-                    float d = dot2(e.dir, tn);
-                    if (d < best) { best = d; ba = gi.wv[i]; bb = gi.wv[nx]; }
-                }
-                v2 ca, cb;
-                clip_line(e.a, e.b, ba, bb, ca, cb);
-                int i = 0;
-                float d = dot2(sub2(ca, e.a), e.dir);
-                if (d < COLLISION_MARGIN) { write_point(S, m, i, ca, d, gr, gi, e.dir); valid |= 1u << i; i += 1; }
-                d = dot2(sub2(cb, e.b), e.dir);
-                if (d < COLLISION_MARGIN) { write_point(S, m, i, cb, d, gr, gi, e.dir); valid |= 1u << i; }
-            }
-        }
-        S.m_valid[m] = valid;
+        man_build_one(S, D, S.cand_idx[q], D.body_base, a - D.body_base, b - D.body_base, S.sat_info[q],
+                      mk(S.sat_nx[q], S.sat_ny[q]), S.cc[a], S.cs[a], S.cc[b], S.cs[b]);
     }
 }
 
@@ -1449,16 +1484,17 @@ __device__ __forceinline__ int solve_route(uint32_t n, uint32_t nitems, uint32_t
 // (lib.zig:199-250), shared by the one-workgroup-per-world kernels and by the
 // grid-wide kernels used for worlds whose body state does not fit LDS.
 
-// inverse mass / inertia (static: 0) and item count of body i
+// inverse mass / inertia (static: 0) and item count of body i (degs: the
+// world's item counts per body)
 template <class ST>
-__device__ __forceinline__ void body_setup(const StepDev& S, const WorldDev& D, const ST& st, int i, bool copy_state)
+__device__ __forceinline__ void body_setup(const WorldDev& D, const ST& st, int i, bool copy_state,
+                                           const uint32_t* degs)
 {
-    const uint32_t base = D.body_base;
     bool stat = (D.flags[i] & BF_STATIC) != 0;
     if (copy_state) { st.mx[i] = D.mx[i]; st.my[i] = D.my[i]; st.L[i] = D.L[i]; }
     st.im[i] = stat ? 0.0f : 1.0f / D.mass[i];
     st.ii[i] = stat ? 0.0f : 1.0f / D.inertia[i];
-    st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
+    st.deg[i] = degs[i];
 }
 
 // forces (lib.zig:200-205), AABB refresh + velocity integration
@@ -1707,8 +1743,48 @@ __device__ __forceinline__ void item_writeback(const StepDev& S, uint4* recs, ui
 // that touches one of its ordering bodies has run, which gives the same bits
 // as the sequential loop.  Workers own execution positions tid, tid+blockDim,
 // ... and take them in (iteration, position) order, the order the ranks use,
-// so the earliest unfinished item always has a worker.
-template <bool LDS, int MAXT, int WAVES = 1>
+// so the earliest unfinished item always has a worker.  Body state lives in
+// LDS (lds: solve_lds_bytes(n) bytes); records at the world's item range
+// [ibase, ibase + nitems) of S.recs; degs: the world's item counts per body.
+__device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
+                                          uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
+                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t& s_abort)
+{
+    if (threadIdx.x == 0) s_abort = 0u;
+    const int n = D.nbodies;
+    const float sub_dt = dt / (float)sub_steps;
+    uint4* recs = S.recs + (size_t)ibase * 8;
+
+    SolverState<true> st;
+    st.mx = (float*)lds;
+    st.my = st.mx + n;
+    st.L = st.my + n;
+    st.im = st.L + n;
+    st.ii = st.im + n;
+    st.cnt = (uint32_t*)(st.ii + n);
+    st.deg = st.cnt + n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) body_setup(D, st, i, true, degs);
+    __syncthreads();
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_build(S, D, st, recs, pos, ibase, mbeg, sub_dt);
+    for (uint32_t s = 0; s < sub_steps; ++s) {
+        const bool last = s + 1 == sub_steps;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces(D, st, i, last, sub_dt);
+        __syncthreads();
+        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
+        __syncthreads();
+        item_chain<true, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
+        __syncthreads();
+    }
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_writeback(S, recs, pos);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
+    }
+}
+
+// the record kernel: the worlds solve_route gives it, one workgroup each
+template <int MAXT, int WAVES = 1>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap, uint32_t slot_nmax,
              uint32_t lds_nmax)
@@ -1718,53 +1794,11 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = S.w_order[blockIdx.x];
     if (S.w_status[w] != 0) return;
-    if (solve_route((uint32_t)S.worlds[w].nbodies, S.w_item_base[w + 1] - S.w_item_base[w], slot_cap, slot_nmax,
-                    lds_nmax) != (LDS ? 1 : 2))
-        return;
-    if (threadIdx.x == 0) s_abort = 0u;
-    const WorldDev& D = S.worlds[w];
-    const int n = D.nbodies;
-    const uint32_t base = D.body_base;
-    const uint32_t mbeg = S.w_man_begin[w];
     const uint32_t ibase = S.w_item_base[w];
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
-    const float sub_dt = dt / (float)sub_steps;
-    uint4* recs = S.recs + (size_t)ibase * 8;
-
-    SolverState<LDS> st;
-    if (LDS) {
-        st.mx = (float*)smem;
-        st.my = st.mx + n;
-        st.L = st.my + n;
-        st.im = st.L + n;
-        st.ii = st.im + n;
-        st.cnt = (uint32_t*)(st.ii + n);
-        st.deg = st.cnt + n;
-    } else {
-        st.mx = D.mx; st.my = D.my; st.L = D.L;
-        st.im = S.g_im + base; st.ii = S.g_ii + base;
-        st.cnt = S.g_cnt + base; st.deg = S.deg + base;
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) body_setup(S, D, st, i, LDS);
-    __syncthreads();
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_build(S, D, st, recs, pos, ibase, mbeg, sub_dt);
-    for (uint32_t s = 0; s < sub_steps; ++s) {
-        const bool last = s + 1 == sub_steps;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces(D, st, i, last, sub_dt);
-        __syncthreads();
-        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
-        __syncthreads();
-        item_chain<LDS, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
-        __syncthreads();
-    }
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_writeback(S, recs, pos);
-    if (LDS) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            D.mx[i] = st.mx[i]; D.my[i] = st.my[i]; D.L[i] = st.L[i];
-        }
-    }
+    const WorldDev& D = S.worlds[w];
+    if (solve_route((uint32_t)D.nbodies, nitems, slot_cap, slot_nmax, lds_nmax) != 1) return;
+    rec_world(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, smem, s_abort);
 }
 
 // ------------------------------------- worlds beyond LDS: split sub-steps
@@ -1797,7 +1831,7 @@ __global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s
         if (!big_world(D, lds_nmax)) continue;
         const SolverState<false> st = global_state(S, D);
         const int i = (int)(x - D.body_base);
-        if (phase == 0) body_setup(S, D, st, i, false);
+        if (phase == 0) body_setup(D, st, i, false, S.deg + D.body_base);
         else if (phase == 1) body_forces(D, st, i, s + 1 == sub_steps, sub_dt);
         else body_positions(D, st, i, sub_dt);
     }
@@ -2058,29 +2092,23 @@ __device__ __forceinline__ void slot_biases(Slot& z, float dt)
 // current slot), instead of every lane walking its own slots independently.
 // Both orders are deadlock-free: a wave works on positions in increasing
 // (iteration, slot, lane) order, the order of the dependencies.
-template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP = false>
-__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
-void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
+// One world's step on the register-slot path, one workgroup of MAXT threads
+// (nitems <= NSLOT * MAXT; lds: slot_lds_bytes(n) bytes; degs: the world's
+// item counts per body).
+template <int NSLOT, int MAXT, bool LOCKSTEP>
+__device__ __forceinline__ void slots_world(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
+                                            uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
+                                            uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t& s_abort)
 {
-    extern __shared__ uint32_t smem[];
-    __shared__ uint32_t s_abort;
-    if (S.counters[C_OVERFLOW]) return;
-    const uint32_t w = S.w_order[blockIdx.x];
-    if (S.w_status[w] != 0) return;
-    const uint32_t ibase = S.w_item_base[w];
-    const uint32_t nitems = S.w_item_base[w + 1] - ibase;
-    const WorldDev& D = S.worlds[w];
     const int n = D.nbodies;
-    if (solve_route((uint32_t)n, nitems, (uint32_t)NSLOT * MAXT, slot_nmax, 0u) != 0) return;
     if (threadIdx.x == 0) s_abort = 0u;
     const uint32_t base = D.body_base;
     const uint32_t nj = D.njoints;
-    const uint32_t mbeg = S.w_man_begin[w];
     const float fsub = (float)sub_steps;
     const float sub_dt = dt / fsub;
 
     SlotState st;
-    st.mx = (float*)smem;
+    st.mx = (float*)lds;
     st.my = st.mx + n;
     st.L = st.my + n;
     st.im = st.L + n;
@@ -2098,7 +2126,7 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
         st.mx[i] = D.mx[i]; st.my[i] = D.my[i]; st.L[i] = D.L[i];
         st.im[i] = stat ? 0.0f : 1.0f / D.mass[i];
         st.ii[i] = stat ? 0.0f : 1.0f / D.inertia[i];
-        st.deg[i] = S.deg_off[base + i + 1] - S.deg_off[base + i];
+        st.deg[i] = degs[i];
         st.I[i] = D.inertia[i];
     }
     __syncthreads();
@@ -2329,6 +2357,383 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
     }
 }
 
+// the register-slot kernel: the worlds solve_route gives it, one workgroup each
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP = false>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
+void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
+{
+    extern __shared__ uint32_t smem[];
+    __shared__ uint32_t s_abort;
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t w = S.w_order[blockIdx.x];
+    if (S.w_status[w] != 0) return;
+    const uint32_t ibase = S.w_item_base[w];
+    const uint32_t nitems = S.w_item_base[w + 1] - ibase;
+    const WorldDev& D = S.worlds[w];
+    if (solve_route((uint32_t)D.nbodies, nitems, (uint32_t)NSLOT * MAXT, slot_nmax, 0u) != 0) return;
+    slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps,
+                                       iters, smem, s_abort);
+}
+
+// ------------------------------------------------------ fused world steps
+// Worlds of at most FUSED_NMAX bodies in the reference order can take whole
+// steps in one workgroup: broadphase, narrowphase, schedule and solve of the
+// world with workgroup barriers between the phases instead of kernel
+// boundaries, and all the steps of a call in one launch (worlds are
+// independent, so nothing crosses workgroups).  Each phase computes for its
+// world exactly what the grid-wide kernels above compute: the same
+// per-element functions (prep_body, narrow_pair, man_build_one, the solve
+// paths) over the world's own scratch slices (WorldScratch), with the
+// per-body and per-bucket arrays in LDS.
+
+// LDS of the fused kernel for a world of n bodies and tb hash buckets
+struct FusedLds {
+    uint32_t* coff;    // [n+1] cells per body -> entry offsets
+    float *cc, *cs;    // [n] cos / sin of the angle at step start
+    uint32_t* boff;    // [tb+1] bucket occupancy -> bucket offsets
+    uint32_t* bfill;   // [tb]
+    uint32_t* dcnt;    // [n] solver items per ordering body
+    uint32_t* doff;    // [n+1] -> list offsets
+    uint32_t* dfill;   // [n]
+    uint32_t* solve;   // the solve path's body state
+};
+
+__device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint32_t tb)
+{
+    FusedLds L;
+    uint32_t* p = lds;
+    L.coff = p; p += n + 1;
+    L.cc = (float*)p; p += n;
+    L.cs = (float*)p; p += n;
+    L.boff = p; p += tb + 1;
+    L.bfill = p; p += tb;
+    L.dcnt = p; p += n;
+    L.doff = p; p += n + 1;
+    L.dfill = p; p += n;
+    L.solve = p;
+    return L;
+}
+
+size_t fused_lds_bytes(uint32_t n, uint32_t tb)
+{
+    const size_t front = 6u * (size_t)n + 2u * (size_t)tb + 3u;
+    return front * 4u + std::max(slot_lds_bytes((int)n), solve_lds_bytes((int)n));
+}
+
+// exclusive scan of a[0, len) in place (LDS or global), a[len] = the total,
+// which is returned; called by the whole workgroup after a barrier, ends
+// with one
+__device__ uint32_t group_scan(uint32_t* a, uint32_t len, uint32_t* sm)
+{
+    const uint32_t T = blockDim.x, t = threadIdx.x;
+    const uint32_t chunk = (len + T - 1) / T;
+    const uint32_t lo = umin(t * chunk, len), hi = umin(lo + chunk, len);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += a[i];
+    uint32_t total;
+    uint32_t run = block_excl_scan(sum, sm, &total);
+    for (uint32_t i = lo; i < hi; ++i) {
+        uint32_t x = a[i];
+        a[i] = run;
+        run += x;
+    }
+    if (t == 0) a[len] = total;
+    __syncthreads();
+    return total;
+}
+
+// A world's slice of family k (0 cell entries, 1 raw candidates, 2
+// manifolds, 3 solver items) is too small for `need` elements: the world
+// moves to a larger slice taken from the free end of the arrays (S.fpool).
+// False when the arrays are full; the world then stops (fused_overflow).
+// Called by the whole workgroup; the slice table entry is updated for the host.
+__device__ bool fused_grow(const StepDev& S, uint32_t w, WorldScratch& X, int k, uint32_t need, uint32_t& s_slot)
+{
+    const uint32_t cap = need + need / 2 + 64;
+    const uint32_t take = k < 2 ? cap + 1 : cap;   // entry and candidate slices end with a scan total
+    if (threadIdx.x == 0) {
+        // claim [base, base + take) only if it fits: a refused request leaves
+        // the free end for smaller ones
+        const uint32_t lim = S.fpool[4 + k];
+        uint32_t base = __hip_atomic_load(&S.fpool[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            if (base > lim || take > lim - base) { base = 0xffffffffu; break; }
+            const uint32_t seen = atomicCAS(&S.fpool[k], base, base + take);
+            if (seen == base) break;
+            base = seen;
+        }
+        s_slot = base;
+    }
+    __syncthreads();
+    const uint32_t base = s_slot;
+    __syncthreads();
+    if (base == 0xffffffffu) return false;
+    if (k == 0) { X.e0 = base; X.ecap = cap; }
+    else if (k == 1) { X.r0 = base; X.rcap = cap; }
+    else if (k == 2) { X.m0 = base; X.mcap = cap; }
+    else { X.i0 = base; X.icap = cap; }
+    if (threadIdx.x == 0) S.wscr[w] = X;
+    return true;
+}
+
+// A world's scratch slice is too small for this step and the arrays are
+// full: record what it needs and stop the world before its state changes
+// (the host grows the arrays and runs the remaining steps)
+__device__ __forceinline__ void fused_overflow(const StepDev& S, uint32_t w, uint32_t e, uint32_t r, uint32_t m,
+                                               uint32_t it)
+{
+    if (threadIdx.x == 0) {
+        uint32_t* c = S.w_cnt + 8 * (size_t)w;
+        c[4] = e; c[5] = r; c[6] = m; c[7] = it;
+        S.w_status[w] = WS_OVERFLOW;
+    }
+}
+
+// One workgroup per world (S.w_order), up to nsteps steps per world counted
+// in S.w_done, so a relaunch after an overflow resumes each world where it
+// stopped.  The solve takes the register-slot path when the step's items fit
+// NSLOT * MAXT slots, else the record path; both keep body state in LDS.
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
+void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t nsteps)
+{
+    extern __shared__ uint32_t smem[];
+    __shared__ uint32_t s_sm[16];
+    __shared__ uint32_t s_err, s_abort, s_slot;
+    const uint32_t w = S.w_order[blockIdx.x];
+    const WorldDev& D = S.worlds[w];
+    WorldScratch X = S.wscr[w];
+    const uint32_t n = (uint32_t)D.nbodies, tb = D.table, base = D.body_base, nj = (uint32_t)D.njoints;
+    const uint32_t t = threadIdx.x;
+    constexpr uint32_t T = MAXT;
+    const FusedLds L = fused_carve(smem, n, tb);
+    uint32_t* cnt = S.w_cnt + 8 * (size_t)w;
+    uint64_t* ts = S.w_tstamp ? S.w_tstamp + (size_t)w * (S.tcap + 1) : nullptr;
+    uint32_t step = S.w_done[w];
+    if (ts && t == 0 && step < nsteps && step <= S.tcap) ts[step] = wall_clock64();
+    // phase times (profiling): thread 0 adds the wall clock between marks
+    uint64_t* prof = (S.w_prof && t == 0) ? S.w_prof + (size_t)w * FP_N : nullptr;
+    uint64_t tmark = prof ? wall_clock64() : 0;
+    auto mark = [&](int ph) {
+        if (prof) {
+            const uint64_t now = wall_clock64();
+            prof[ph] += now - tmark;
+            tmark = now;
+        }
+    };
+    for (; step < nsteps; ++step) {
+        if (S.w_status[w] != WS_OK) break;      // uniform: read before the barrier below
+        // ---- broadphase: per body (k_prep)
+        if (t == 0) s_err = 0u;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += T) {
+            const Prep o = prep_body(D, i);
+            L.cc[i] = o.c;
+            L.cs[i] = o.s;
+            S.sflag[base + i] = o.res;
+            if (o.err) atomicOr(&s_err, o.err == WS_NONFINITE ? 2u : 1u);
+            if (!o.err && o.ncell) { S.cx0[base + i] = o.x0; S.cy0[base + i] = o.y0; S.cnx[base + i] = o.nx; }
+            L.coff[i] = o.ncell;
+        }
+        __syncthreads();
+        if (s_err) {
+            if (t == 0) S.w_status[w] = (s_err & 2u) ? WS_NONFINITE : WS_CELLS;
+            break;
+        }
+        const uint32_t E = group_scan(L.coff, n, s_sm);
+        mark(FP_PREP);
+        if (E > X.ecap && !fused_grow(S, w, X, 0, E, s_slot)) { fused_overflow(S, w, E, 0, 0, 0); break; }
+        uint32_t* ent_body = S.ent_body + X.e0;
+        uint32_t* ent_bkt = S.ent_bkt + X.e0;
+        uint32_t* ecnt = S.ent_cnt + X.e0;      // raw candidates per entry -> offsets
+        uint32_t* items = S.bkt_items + X.e0;
+        uint32_t* sorted = S.bkt_sorted + X.e0;
+        for (uint32_t b = t; b <= tb; b += T) {
+            L.boff[b] = 0u;
+            if (b < tb) L.bfill[b] = 0u;
+        }
+        __syncthreads();
+        // cell entries and bucket occupancy (k_entries)
+        for (uint32_t e = t; e < E; e += T) {
+            uint32_t i = upper_idx(L.coff, n, e);
+            uint32_t k = e - L.coff[i];
+            uint32_t nx = S.cnx[base + i];
+            int64_t yi = S.cy0[base + i] + (int64_t)(k / nx);
+            int64_t xi = S.cx0[base + i] + (int64_t)(k % nx);
+            uint32_t b = (uint32_t)cell_hash((uint64_t)tb, xi, yi);
+            ent_body[e] = i;
+            ent_bkt[e] = b;
+            atomicAdd(&L.boff[b], 1u);
+        }
+        __syncthreads();
+        group_scan(L.boff, tb, s_sm);
+        // bucket members (k_bucket_fill), then in descending slot with a
+        // body's repeats adjacent (k_bucket_rank)
+        for (uint32_t e = t; e < E; e += T) {
+            uint32_t b = ent_bkt[e];
+            uint32_t pos = atomicAdd(&L.bfill[b], 1u);
+            items[L.boff[b] + pos] = ent_body[e];
+        }
+        __syncthreads();
+        for (uint32_t e = t; e < E; e += T) {
+            uint32_t a = ent_body[e], b = ent_bkt[e];
+            uint32_t o = L.boff[b], c = L.boff[b + 1] - o;
+            uint32_t greater = 0, same = 0;
+            for (uint32_t j = 0; j < c; ++j) {
+                uint32_t v = items[o + j];
+                greater += v > a;
+                same += v == a;
+            }
+            uint32_t dup = 0;
+            if (same > 1)
+                for (uint32_t x = L.coff[a]; x < e; ++x) dup += ent_bkt[x] == b;
+            sorted[o + greater + dup] = a;
+        }
+        __syncthreads();
+        mark(FP_BUCKETS);
+        // raw candidates per entry: distinct bucket members above the entry's body (k_ent_cnt)
+        for (uint32_t e = t; e < E; e += T) {
+            uint32_t a = ent_body[e], b = ent_bkt[e];
+            uint32_t o = L.boff[b], c = L.boff[b + 1] - o, q = 0;
+            for (uint32_t j = 0; j < c && sorted[o + j] > a; ++j) q += (j == 0 || sorted[o + j] != sorted[o + j - 1]);
+            ecnt[e] = q;
+        }
+        __syncthreads();
+        const uint32_t R = group_scan(ecnt, E, s_sm);
+        if (R > X.rcap && !fused_grow(S, w, X, 1, R, s_slot)) { fused_overflow(S, w, E, R, 0, 0); break; }
+        uint4* cand = S.cand + X.r0;
+        uint32_t* cflag = S.cand_flag + X.r0;   // collides per candidate -> manifold offsets
+        // raw candidates in query order (k_cand_emit)
+        for (uint32_t e = t; e < E; e += T) {
+            uint32_t a = ent_body[e], b = ent_bkt[e];
+            uint32_t o = L.boff[b];
+            uint32_t ro = ecnt[e], c = ecnt[e + 1] - ro;
+            uint32_t k = e - L.coff[a];
+            for (uint32_t j = 0, q = 0; q < c; ++j) {
+                if (j > 0 && sorted[o + j] == sorted[o + j - 1]) continue;
+                cand[ro + q] = make_uint4(a, sorted[o + j], k, 0u);
+                ++q;
+            }
+        }
+        __syncthreads();
+        mark(FP_CANDS);
+        // ---- narrowphase (k_narrow)
+        for (uint32_t q = t; q < R; q += T) {
+            const uint4 cq = cand[q];
+            const uint32_t a = cq.x, b = cq.y;
+            uint32_t info = 0;
+            float nx = 0.f, ny = 0.f;
+            bool hit = narrow_pair(D, a, b, cq.z, ent_bkt + L.coff[a], ent_bkt + L.coff[b], L.coff[b + 1] - L.coff[b],
+                                   L.cc[a], L.cs[a], L.cc[b], L.cs[b], info, nx, ny);
+            cflag[q] = hit ? 1u : 0u;
+            S.sat_info[X.r0 + q] = info;
+            S.sat_nx[X.r0 + q] = nx;
+            S.sat_ny[X.r0 + q] = ny;
+        }
+        __syncthreads();
+        const uint32_t M = group_scan(cflag, R, s_sm);
+        mark(FP_NARROW);
+        if ((M > X.mcap && !fused_grow(S, w, X, 2, M, s_slot)) ||
+            (nj + M > X.icap && !fused_grow(S, w, X, 3, nj + M, s_slot))) {
+            fused_overflow(S, w, E, R, M, nj + M);
+            break;
+        }
+        uint32_t* lists = S.lists + 2 * (size_t)X.i0;
+        // contact points (k_man_build), manifolds in discovery order
+        for (uint32_t q = t; q < R; q += T) {
+            const uint32_t mi = cflag[q];
+            if (cflag[q + 1] == mi) continue;
+            const uint4 cq = cand[q];
+            const uint32_t a = cq.x, b = cq.y;
+            man_build_one(S, D, X.m0 + mi, base, a, b, S.sat_info[X.r0 + q],
+                          mk(S.sat_nx[X.r0 + q], S.sat_ny[X.r0 + q]), L.cc[a], L.cs[a], L.cc[b], L.cs[b]);
+        }
+        // ---- schedule: distance / offset joints make both bodies ordering (k_joint_mark)
+        for (uint32_t j = t; j < nj; j += T) {
+            const Joint& J = D.joints[j];
+            if (J.type <= 1) {
+                if (J.s1 >= 0) atomicOr(&S.sflag[base + J.s1], SF_RESOURCE);
+                if (J.s2 >= 0) atomicOr(&S.sflag[base + J.s2], SF_RESOURCE);
+            }
+        }
+        for (uint32_t i = t; i < n; i += T) { L.dcnt[i] = 0u; L.dfill[i] = 0u; }
+        __syncthreads();
+        mark(FP_CONTACTS);
+        const uint32_t nit = nj + M;
+        // items: joints, then manifolds, with their ordering bodies (k_items_*)
+        for (uint32_t j = t; j < nit; j += T) {
+            int32_t r0 = -1, r1 = -1;
+            if (j < nj) {
+                const Joint& J = D.joints[j];
+                r0 = J.s1 >= 0 ? (int32_t)(base + J.s1) : -1;
+                if (J.type <= 1 && J.s2 >= 0 && J.s2 != J.s1) r1 = (int32_t)(base + J.s2);
+            } else {
+                const uint32_t m = X.m0 + (j - nj);
+                const uint32_t ref = S.m_ref[m], inc = S.m_inc[m];
+                r0 = (S.sflag[ref] & SF_RESOURCE) ? (int32_t)ref : -1;
+                r1 = (S.sflag[inc] & SF_RESOURCE) ? (int32_t)inc : -1;
+            }
+            S.it_r0[X.i0 + j] = r0;
+            S.it_r1[X.i0 + j] = r1;
+            S.it_perm[X.i0 + j] = j;
+            if (r0 >= 0) atomicAdd(&L.dcnt[r0 - (int32_t)base], 1u);
+            if (r1 >= 0) atomicAdd(&L.dcnt[r1 - (int32_t)base], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += T) L.doff[i] = L.dcnt[i];
+        __syncthreads();
+        group_scan(L.doff, n, s_sm);
+        // per-body item lists (k_list_fill), ranks in item order (k_list_rank)
+        for (uint32_t j = t; j < nit; j += T) {
+            const int32_t r0 = S.it_r0[X.i0 + j], r1 = S.it_r1[X.i0 + j];
+            if (r0 >= 0) {
+                const uint32_t x = (uint32_t)r0 - base;
+                lists[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u;
+            }
+            if (r1 >= 0) {
+                const uint32_t x = (uint32_t)r1 - base;
+                lists[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u + 1u;
+            }
+        }
+        __syncthreads();
+        for (uint32_t x = t; x < n; x += T) {
+            uint32_t* l = lists + L.doff[x];
+            const uint32_t c = L.dcnt[x];
+            for (uint32_t i = 1; i < c; ++i) {
+                uint32_t v = l[i];
+                uint32_t j = i;
+                while (j > 0 && l[j - 1] > v) { l[j] = l[j - 1]; --j; }
+                l[j] = v;
+            }
+            for (uint32_t p = 0; p < c; ++p) {
+                uint32_t v = l[p];
+                if (v & 1u) S.it_k1[X.i0 + (v >> 1)] = p; else S.it_k0[X.i0 + (v >> 1)] = p;
+            }
+        }
+        __syncthreads();
+        mark(FP_SCHED);
+        // ---- solve
+        if (nit <= (uint32_t)NSLOT * MAXT)
+            slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
+                                               s_abort);
+        else
+            rec_world(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve, s_abort);
+        __syncthreads();
+        mark(FP_SOLVE);
+        if (s_abort) break;                     // the solve stalled: status set
+        if (t == 0) {
+            cnt[0] = E; cnt[1] = R; cnt[2] = M; cnt[3] = nit;
+            S.w_man_begin[w] = X.m0;
+            S.w_man_end[w] = X.m0 + M;
+            S.w_done[w] = step + 1;
+            if (ts && step < S.tcap) ts[step + 1] = wall_clock64();
+        }
+    }
+    // the counts of the world's last step into the call's totals
+    __syncthreads();
+    if (t < 4) atomicAdd(&S.counters[t], cnt[t]);
+}
+
 // step bookkeeping: counters of one step start at zero; the overflow flag is
 // sticky (it is cleared by the host), so a replayed sequence of steps stops
 // committing at the first step whose scratch did not fit.
@@ -2355,12 +2760,14 @@ size_t solve_lds_bytes(int nbodies) { return (size_t)nbodies * 7u * 4u; }
 
 typedef void (*SlotFn)(StepDev, float, uint32_t, uint32_t, uint32_t);
 typedef void (*RecFn)(StepDev, float, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t);
+typedef void (*FusedFn)(StepDev, float, uint32_t, uint32_t, uint32_t);
 struct SolveVariant {
     SlotFn slot_fn;      // register-slot kernel (nullptr: none)
     unsigned slot_threads;
     int slots;
     RecFn rec_fn;        // record kernel (LDS body state) for the worlds the slot kernel leaves
     unsigned threads;
+    FusedFn fused_fn;    // whole steps per workgroup with the slot kernel's shape (nullptr: none)
 };
 // solve variants: the register-slot kernel's slots x workgroup size, and the
 // record kernel's workgroup size for worlds with more items than slots
@@ -2368,16 +2775,33 @@ struct SolveVariant {
 // SIMD with the world's other waves: 256 threads beat 1,024 by about 3x on a
 // 4,096-body world)
 static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
-    {nullptr, 0, 0, k_solve<true, 1024>, 1024},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<true, 256, 2>, 256},
-    {nullptr, 0, 0, k_solve<true, 256, 2>, 256},
-    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<true, 128, 2>, 128},
-    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<true, 256, 2>, 256},
+    {nullptr, 0, 0, k_solve<1024>, 1024, nullptr},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<256, 2>, 256, k_fused<4, 256, 3, false>},
+    {k_solve_slots<2, 512, 3>, 512, 2, k_solve<256, 2>, 256, k_fused<2, 512, 3, false>},
+    {k_solve_slots<3, 256, 3>, 256, 3, k_solve<256, 2>, 256, k_fused<3, 256, 3, false>},
+    {k_solve_slots<4, 256, 2>, 256, 4, k_solve<256, 2>, 256, k_fused<4, 256, 2, false>},
+    {k_solve_slots<8, 128, 2>, 128, 8, k_solve<256, 2>, 256, k_fused<8, 128, 2, false>},
+    {nullptr, 0, 0, k_solve<256, 2>, 256, nullptr},
+    {k_solve_slots<4, 256, 3>, 256, 4, k_solve<128, 2>, 128, k_fused<4, 256, 3, false>},
+    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<256, 2>, 256, k_fused<3, 256, 3, true>},
 };
+
+bool fused_available(int v)
+{
+    return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].fused_fn != nullptr;
+}
+
+hipError_t fused_attributes(int v, hipFuncAttributes* out)
+{
+    return hipFuncGetAttributes(out, (const void*)kSolveVariants[v].fused_fn);
+}
+
+void launch_fused(const StepDev& S, int v, uint32_t nworlds, float dt, uint32_t sub_steps, uint32_t iters,
+                  uint32_t nsteps, size_t lds, hipStream_t st)
+{
+    const SolveVariant& sv = kSolveVariants[v];
+    hipLaunchKernelGGL(sv.fused_fn, dim3(nworlds), dim3(sv.slot_threads), lds, st, S, dt, sub_steps, iters, nsteps);
+}
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
 int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
@@ -2601,7 +3025,11 @@ void set_solve_smem_limit()
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)kSolveVariants[v].rec_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
+        if (kSolveVariants[v].fused_fn)
+            (void)hipFuncSetAttribute((const void*)kSolveVariants[v].fused_fn,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     }
+    (void)hipGetLastError();   // a refused attribute is not an error of a later call
 }
 
 }  // namespace r2d

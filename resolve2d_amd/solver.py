@@ -64,6 +64,28 @@ class Context:
     def set_lds_limit(self, max_bodies: int):
         N.check(self.lib.r2d_ctx_set_lds_limit(self.handle, max_bodies), "r2d_ctx_set_lds_limit")
 
+    def set_fused(self, on: bool):
+        """Whole steps per workgroup for batches of worlds up to 1,024 bodies
+        (default on); off forces the phase kernels.  Same bits either way."""
+        N.check(self.lib.r2d_ctx_set_fused(self.handle, 1 if on else 0), "r2d_ctx_set_fused")
+
+    def fused_times(self):
+        """(kernel ms, launches, world-steps) of profiled fused launches since the last reset."""
+        ms = ctypes.c_double()
+        launches = ctypes.c_uint64()
+        steps = ctypes.c_uint64()
+        N.check(self.lib.r2d_ctx_fused_times(self.handle, ctypes.byref(ms), ctypes.byref(launches),
+                                             ctypes.byref(steps)), "r2d_ctx_fused_times")
+        return ms.value, launches.value, steps.value
+
+    def fused_phase_times(self):
+        """Per-phase ms of profiled fused launches, summed over world-steps:
+        dict of cell ranges, buckets, candidates, narrowphase, contacts, schedule, solve."""
+        out = (ctypes.c_double * 8)()
+        N.check(self.lib.r2d_ctx_fused_phase_times(self.handle, out), "r2d_ctx_fused_phase_times")
+        names = ["cells", "buckets", "candidates", "narrowphase", "contacts", "schedule", "solve"]
+        return {k: out[i] for i, k in enumerate(names)}
+
     def set_profiling(self, on: bool):
         N.check(self.lib.r2d_ctx_set_profiling(self.handle, 1 if on else 0), "r2d_ctx_set_profiling")
         self.profiling = bool(on)
