@@ -30,7 +30,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# LDS: 256 CUs x 128 B per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+TUNE_AT, TUNE_STEPS = 60, 30   # autotuner: samples of 30 steps from step 60 of a copy of the batch
 G = 9.81
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def traffic_for(kernel, args):
+    """HBM bytes per launch of the kernel on this workload as counted by
+    rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
+    from the committed measurement; None when there is none for it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = f"{kernel}/worlds{args.worlds}/steps{args.steps}/seed{args.seed}"
+    return t.get(key)
 
 
 def log(msg):
@@ -308,7 +325,22 @@ def main():
     ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the timed region")
     ap.add_argument("--tuned-hash", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB between two device buffers (a kernel of known HBM traffic, for "
+                         "scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        assert bool(torch.equal(dst, src))
+        print(json.dumps({"copy_mib": args.calibrate_copy, "launches": 3, "bytes_read_per_launch": n * 4,
+                          "bytes_written_per_launch": n * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: worlds are independent and would be sharded whole across GPUs "
               f"with no exchange; --gpus {args.gpus} is not run here.", file=sys.stderr)
@@ -350,27 +382,32 @@ def main():
         batch.process(dt, S, K, 1)
     tuned = None
     if args.variant < 0 and not args.no_tune:
-        # launch shape picked on a copy of the batch advanced to mid-horizon,
-        # where contacts are as dense as in the timed steps (every shape gives
-        # the same bits, so the copy's extra steps change nothing measured)
+        # launch shape picked on copies of the batch advanced to step 60 (every
+        # shape gives the same bits, so the copies change nothing measured):
+        # each candidate runs one 30-step call from that same state
         from resolve2d_amd import autotune
-        copy = []
-        for sc in specs:
-            w = GpuWorld(ctx)
-            apply_scene(sc, w)
-            copy.append(w)
-        ch = [w.handle for w in copy]
-        ctx.process_batch_handles(ch, dt, S, K, args.steps // 2)
+
+        def fresh_copy():
+            copy = []
+            for sc in specs:
+                w = GpuWorld(ctx)
+                apply_scene(sc, w)
+                copy.append(w)
+            ctx.process_batch_handles([w.handle for w in copy], dt, S, K, TUNE_AT)
+            return copy
+
         cands = autotune.candidates(lds_limits=(0,))
-        tuned = autotune.tune(ctx, ch, dt, S, K, rounds=2, cands=cands, cache_path=None)
-        for w in copy:
-            w.close()
-        log(f"autotune: variant {tuned['variant']} ({tuned['solve_ms']:.3f} ms solve at step {args.steps // 2})")
+        tuned = autotune.tune(ctx, None, dt, S, K, rounds=1, cands=cands, cache_path=None,
+                              steps_per_sample=TUNE_STEPS, make_batch=fresh_copy)
+        log(f"autotune: variant {tuned['variant']} fused {tuned['fused']} ({tuned['step_ms']:.3f} ms per step)")
     ctx.sync()
 
-    # timed region: exactly `steps` batched steps in one call (the hipGraph of a
-    # step replayed back to back); HIP events between the phases of every step
-    # on the library's stream give the per-step device latency
+    # timed region: exactly `steps` batched steps in one call.  Worlds of at
+    # most 1,024 bodies take whole steps per workgroup: one fused kernel runs
+    # every step of every world, with HIP events around the launch and each
+    # workgroup's wall clock per step (the world-step latency).  Otherwise the
+    # hipGraph of a step is replayed back to back with HIP events between the
+    # phases of every step on the library's stream.
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     ctx.sync()
@@ -379,6 +416,9 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t_start
     phase_ms, phase_steps = ctx.phase_times()
+    fused_ms, fused_launches, world_steps = ctx.fused_times()
+    fused_phases = ctx.fused_phase_times()
+    work = ctx.fused_counts()
     lat_ms = np.array(ctx.step_times())
     counts = ctx.last_counts()
     ctx.set_profiling(False)
@@ -411,16 +451,36 @@ def main():
     plain_value = plain_body_steps / plain_t
     log("plain done")
 
-    # roofline of the dominant kernel (solve): algorithmic bytes per launch
-    # = 112 B per body (pose, momentum, force, torque, mass, inertia, flags, shape read;
-    #   pose, momentum, force, torque, AABB written) + 120 B per manifold
-    #   (geometry read once, impulses and effective masses written) + 16 B per solver item
-    names = ["broadphase", "narrowphase", "schedule", "solve"]
-    per_step = {n: phase_ms[i] / max(phase_steps, 1) for i, n in enumerate(names)}
+    # roofline of the dominant kernel.  Fused: k_fused, one launch for the
+    # whole timed region; its algorithmic HBM bytes are the step's
+    # 72 B per body (pose, momentum, angle read; mass, inertia, mu, shape,
+    # flags read; pose, momentum written) + 16 B per cell entry (written and
+    # read once) + 8 B per raw candidate pair + 176 B per manifold (written by
+    # the narrowphase, read by the solve), summed over the launch's
+    # world-steps; its LDS bytes are S x K x (64 B per manifold: two bodies'
+    # momentum and inverse masses read, momentum written; 48 B per joint).
+    # Phase kernels: the solve phase, 112 B per body + 120 B per manifold +
+    # 16 B per solver item.
+    joint_items = max(0, work["items"] - work["manifolds"])
+    if fused_launches:
+        kernel = "k_fused"
+        launch_s = fused_ms / fused_launches / 1e3
+        launch_bytes = (72 * work["bodies"] + 16 * work["entries"] + 8 * work["raw_candidates"] +
+                        176 * work["manifolds"]) / fused_launches
+        lds_bytes = S * K * (64 * work["manifolds"] + 48 * joint_items) / fused_launches
+        per_step = {k: v / max(world_steps, 1) for k, v in fused_phases.items()}
+        per_step_note = "per world-step, workgroup wall clock"
+    else:
+        kernel = "k_solve"
+        names = ["broadphase", "narrowphase", "schedule", "solve"]
+        per_step = {n: phase_ms[i] / max(phase_steps, 1) for i, n in enumerate(names)}
+        launch_s = per_step["solve"] / 1e3
+        launch_bytes = 112 * n_bodies + 120 * counts["manifolds"] + 16 * counts["items"]
+        lds_bytes = S * K * (64 * counts["manifolds"] + 48 * max(0, counts["items"] - counts["manifolds"]))
+        per_step_note = "per step, HIP events between phase launches"
     dom = max(per_step, key=per_step.get)
-    solve_bytes = 112 * n_bodies + 120 * counts["manifolds"] + 16 * counts["items"]
-    solve_s = per_step["solve"] / 1e3
-    achieved = solve_bytes / solve_s / 1e9 if solve_s > 0 else 0.0
+    achieved = launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
+    lds_achieved = lds_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
 
     # configuration 2: one 4,096-body world, full horizon, one world per launch,
     # reference order vs the coloured schedule
@@ -500,12 +560,15 @@ def main():
                                "0-10 springs per world; dt=1/60, 4 sub-steps, 4 iterations",
                    "worlds": args.worlds, "bodies": n_bodies, "manifolds_last_step": counts["manifolds"],
                    "schedule": "reference order (bit-exact)", "parallelism": "1 GPU",
-                   "solve_launch": ({"autotuned": True, "variant": tuned["variant"], "tuned_at_step": args.steps // 2,
-                                     "candidates_solve_ms": tuned["all_ms"]}
+                   "solve_launch": ({"autotuned": True, "variant": tuned["variant"], "fused": tuned["fused"],
+                                     "tuned_at_step": TUNE_AT, "sample_steps": TUNE_STEPS,
+                                     "candidates_step_ms (variant/lds_limit/fused)": tuned["all_ms"]}
                                     if tuned else {"autotuned": False, "variant": args.variant})},
-        "p50_step_ms": float(np.percentile(lat_ms, 50)),
-        "p99_step_ms": float(np.percentile(lat_ms, 99)),
-        "step_latency": "device time per step from HIP events (the timed region's 240 steps)",
+        "p50_step_ms": float(np.percentile(lat_ms, 50)) if len(lat_ms) else None,
+        "p99_step_ms": float(np.percentile(lat_ms, 99)) if len(lat_ms) else None,
+        "step_latency": ("each world's step, wall clock of its workgroup in the fused kernel (all world-steps of the "
+                         "timed region)" if fused_launches else
+                         "device time per step from HIP events (the timed region's steps)"),
         "per_call": {"p50_ms": float(np.percentile(call_lat, 50)) if call_lat else None,
                      "p99_ms": float(np.percentile(call_lat, 99)) if call_lat else None,
                      "calls": len(call_lat), "note": "one step per r2d_process_batch call after the timed "
@@ -518,9 +581,16 @@ def main():
         "single_world_4096": single,
         "large_world_65536": large,
         "accuracy": acc,
-        "phase_ms_per_step": per_step,
-        "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "dominant_phase": dom},
+        "mode": ("fused: one kernel runs every step of every world, one workgroup per world" if fused_launches else
+                 "phase kernels: one launch sequence (hipGraph) per step"),
+        "phase_ms": per_step,
+        "phase_ms_note": per_step_note,
+        "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kernel, args),
+                     "launch_ms": launch_s * 1e3, "algorithmic_bytes_per_launch": launch_bytes,
+                     "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes},
+                     "dominant_phase": dom},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }

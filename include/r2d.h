@@ -219,6 +219,10 @@ int r2d_ctx_fused_times(r2d_ctx* ctx, double* kernel_ms, uint64_t* launches, uin
  * worlds' steps (ms; wall clock of each workgroup): out_ms[8] = cell ranges,
  * buckets, raw candidates, narrowphase, contact points, schedule, solve, 0. */
 int r2d_ctx_fused_phase_times(r2d_ctx* ctx, double* out_ms);
+/* The same launches' work, summed over their world-steps: out[5] = bodies,
+ * cell entries, raw candidates, manifolds, solver items (the units of the
+ * step's algorithmic bytes). */
+int r2d_ctx_fused_counts(r2d_ctx* ctx, uint64_t* out);
 /* Whole steps per workgroup (default on): a batch whose worlds all have at
  * most 1,024 bodies, the reference order and hash tables that fit LDS is
  * stepped by one kernel per call -- every step of every world of the call,

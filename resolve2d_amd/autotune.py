@@ -4,16 +4,21 @@ Knobs (all give bit-identical results, so tuning may run on live steps):
   - solve variant: the register-slot kernel's slots x workgroup size, and the
     record kernels' workgroup sizes (r2d_ctx_set_solve_variant);
   - LDS limit: the largest world whose body state the solve kernel keeps in
-    LDS (r2d_ctx_set_lds_limit).
-Each candidate is timed on real steps of the batch (the solve phase's HIP
-events), candidates interleaved round by round so the slowly changing state
+    LDS (r2d_ctx_set_lds_limit);
+  - whole steps per workgroup (r2d_ctx_set_fused) for variants that have a
+    fused kernel.  A fused launch ends with its slowest world, so it is timed
+    on calls of several steps, as the caller's calls would run.
+Each candidate is timed on real steps of the batch -- the device time of a
+whole step: the fused kernel's launch (HIP events around it) when the batch
+takes whole steps per workgroup, else the sum of the four phases' HIP
+events -- candidates interleaved round by round so the slowly changing state
 does not favour any of them; the median decides.  The winner is kept per
 batch shape -- worlds, largest world and total bodies, each rounded up to a
 power of two -- in a JSON file, so later runs of the same shape reuse it.
 """
 import json
 import os
-from typing import Dict, Iterable, Optional, Sequence, Tuple
+from typing import Callable, Dict, Iterable, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -21,7 +26,15 @@ from . import _native as N
 
 SOLVE_VARIANTS = 9
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "resolve2d_amd", "autotune.json")
-SOLVE_PHASE = 3
+
+
+def step_ms(ctx, steps: int) -> float:
+    """Device time per step of the profiled calls since the last reset."""
+    fused_ms, launches, _ = ctx.fused_times()
+    if launches:
+        return fused_ms / max(steps, 1)
+    ms, n = ctx.phase_times()
+    return sum(ms) / max(n, 1)
 
 
 def _pow2(x: int) -> int:
@@ -63,17 +76,32 @@ def save_cache(cache: Dict[str, dict], path: str = DEFAULT_CACHE):
     os.replace(tmp, path)
 
 
-def candidates(variants: Optional[Iterable[int]] = None, lds_limits: Iterable[int] = (0, 1024)):
+# variants with a fused kernel (the register-slot shapes, lockstep walk aside)
+FUSED_VARIANTS = (1, 2, 3, 4, 5, 7)
+
+
+def candidates(variants: Optional[Iterable[int]] = None, lds_limits: Iterable[int] = (0, 1024),
+               fused: Iterable[bool] = (True, False)):
+    """(variant, lds_limit, fused) triples; fused only where the variant has a
+    fused kernel, and the LDS limit only where the phase kernels run."""
     vs = list(range(SOLVE_VARIANTS)) if variants is None else list(variants)
-    return [(v, lim) for v in vs for lim in lds_limits]
+    out = []
+    for f in fused:
+        for v in vs:
+            if f and v not in FUSED_VARIANTS:
+                continue
+            for lim in ((0,) if f else lds_limits):
+                out.append((v, lim, bool(f)))
+    return out
 
 
 def apply(ctx, choice: dict):
     ctx.set_solve_variant(int(choice["variant"]))
     ctx.set_lds_limit(int(choice["lds_limit"]))
+    ctx.set_fused(bool(choice.get("fused", True)))
 
 
-def pick(times: Dict[Tuple[int, int], list]) -> Tuple[int, int]:
+def pick(times: Dict[tuple, list]) -> tuple:
     """The candidate with the smallest median time (ties: the earlier one)."""
     best, best_t = None, None
     for c, ts in times.items():
@@ -84,12 +112,24 @@ def pick(times: Dict[Tuple[int, int], list]) -> Tuple[int, int]:
 
 
 def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: int = 2,
-         cands=None, cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True) -> dict:
+         cands=None, cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True,
+         steps_per_sample: int = 30, make_batch: Optional[Callable[[], list]] = None) -> dict:
     """Pick the launch shape for stepping `handles` together and apply it to
-    the context.  Spends len(cands) * rounds real steps of the batch (untimed
-    by the caller's measure; the results are the same as with any other
-    shape).  Returns {"variant", "lds_limit", "solve_ms", "key", "cached"}."""
+    the context.  Spends len(cands) * rounds * steps_per_sample real steps of
+    the batch (untimed by the caller's measure; the results are the same as
+    with any other shape), each sample one call of steps_per_sample steps.
+    With make_batch (returns fresh worlds in the same state each time, with
+    handle and close()), every sample starts from that same state instead of
+    where the previous sample left the batch.  Returns {"variant",
+    "lds_limit", "fused", "step_ms", "key", "cached", "all_ms"}."""
+    probe = None
+    if handles is None:
+        probe = make_batch()
+        handles = [w.handle for w in probe]
     key = shape_key(batch_shape(ctx, handles))
+    if probe is not None:
+        for w in probe:
+            w.close()
     cache = load_cache(cache_path) if (cache_path and use_cache) else {}
     if key in cache:
         apply(ctx, cache[key])
@@ -103,15 +143,21 @@ def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: 
             for c in cands:
                 ctx.set_solve_variant(c[0])
                 ctx.set_lds_limit(c[1])
+                ctx.set_fused(c[2] if len(c) > 2 else True)
+                fresh = make_batch() if make_batch else None
+                hs = [w.handle for w in fresh] if fresh else handles
                 ctx.reset_phase_times()
-                ctx.process_batch_handles(handles, dt, sub_steps, iters, 1)
-                ms, n = ctx.phase_times()
-                times[c].append(ms[SOLVE_PHASE] / max(n, 1))
+                ctx.process_batch_handles(hs, dt, sub_steps, iters, steps_per_sample)
+                times[c].append(step_ms(ctx, steps_per_sample))
+                for w in fresh or ():
+                    w.close()
     finally:
         ctx.set_profiling(was)
-    v, lim = pick(times)
-    choice = {"variant": v, "lds_limit": lim, "solve_ms": float(np.median(times[(v, lim)])),
-              "all_ms": {f"{c[0]}/{c[1]}": float(np.median(t)) for c, t in times.items()}}
+    best = pick(times)
+    v, lim = best[0], best[1]
+    fused = bool(best[2]) if len(best) > 2 else True
+    choice = {"variant": v, "lds_limit": lim, "fused": fused, "step_ms": float(np.median(times[best])),
+              "all_ms": {"/".join(str(int(x)) for x in c): float(np.median(t)) for c, t in times.items()}}
     apply(ctx, choice)
     if cache_path:
         cache = load_cache(cache_path)

@@ -180,6 +180,7 @@ struct r2d_ctx {
     double fused_ms = 0.0;                        // device time of profiled fused launches
     uint64_t fused_launches = 0;
     uint64_t fused_steps = 0;                     // world-steps of profiled fused launches
+    uint64_t fused_sums[5] = {};                  // their bodies, entries, raw candidates, manifolds, items
 
     Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}, trace}; }
 };
@@ -999,6 +1000,8 @@ int fused_wait(r2d_ctx* c, Batch& b)
             HIPCHK(hipEventElapsedTime(&ms, c->ev_begin, c->ev_kend));
             c->fused_ms += ms;
             c->fused_launches += 1;
+            const uint64_t* sums = reinterpret_cast<const uint64_t*>(c->hcounters + C_SUMS);
+            for (int k = 0; k < 5; ++k) c->fused_sums[k] += sums[k];
         }
         b.world_status.assign(c->hstatus, c->hstatus + nw);
         // slices the worlds moved to during the launch; later layouts start from them
@@ -1827,6 +1830,14 @@ R2D_EXPORT int r2d_ctx_reset_phase_times(r2d_ctx* c)
     c->fused_launches = 0;
     c->fused_steps = 0;
     for (int k = 0; k < FP_N; ++k) c->fused_phase_ms[k] = 0.0;
+    for (int k = 0; k < 5; ++k) c->fused_sums[k] = 0;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_fused_counts(r2d_ctx* c, uint64_t* out)
+{
+    if (!c || !out) return R2D_E_INVALID_ARG;
+    for (int k = 0; k < 5; ++k) out[k] = c->fused_sums[k];
     return R2D_OK;
 }
 

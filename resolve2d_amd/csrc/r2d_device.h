@@ -63,7 +63,8 @@ enum : int {
     C_ITEMS = 3,       // solver items (joints + manifolds)
     C_OVERFLOW = 4,    // capacity overflow flag (bit per buffer)
     C_STEPS = 5,       // committed steps
-    C_COUNT = 8
+    C_COUNT = 8,
+    C_SUMS = 16        // k_fused: 64-bit sums over the launch's world-steps of bodies, E, R, M, items
 };
 
 // A world's slices of the step scratch when whole steps of the world run in

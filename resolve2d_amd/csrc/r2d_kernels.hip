@@ -186,7 +186,9 @@ struct Geo {
     float c, s;       // cos / sin of the angle
 };
 
-__device__ __forceinline__ Geo load_geo(const WorldDev& D, uint32_t slot, float c, float s)
+// B: a type with the body arrays of WorldDev (px, py, flags, rad, id, hw, hh)
+template <class B>
+__device__ __forceinline__ Geo load_geo(const B& D, uint32_t slot, float c, float s)
 {
     Geo g;
     g.pos = mk(D.px[slot], D.py[slot]);
@@ -317,7 +319,8 @@ __device__ __forceinline__ bool overlap_sat(Sat& ret, const Geo& ref, const Geo&
 }
 
 // aabb.intersects (aabb.zig:12-19)
-__device__ __forceinline__ bool aabb_hit(const WorldDev& D, uint32_t a, uint32_t b)
+template <class B>
+__device__ __forceinline__ bool aabb_hit(const B& D, uint32_t a, uint32_t b)
 {
     float dx = absf(D.ax[b] - D.ax[a]);
     float dy = absf(D.ay[b] - D.ay[a]);
@@ -513,17 +516,18 @@ __global__ void k_cand_emit(StepDev S)
 // in the lower body's query list; later occurrences are the reference's
 // "already a manifold / already rejected".  Returns whether it collides, with
 // the reference face id | ref-is-sa << 2 in info and the normal.
-__device__ __forceinline__ bool narrow_pair(const WorldDev& D, uint32_t sa, uint32_t sb, uint32_t k,
+template <class B>
+__device__ __forceinline__ bool narrow_pair(const WorldDev& D, const B& bodies, uint32_t sa, uint32_t sb, uint32_t k,
                                             const uint32_t* ba, const uint32_t* bb, uint32_t nb, float ca, float sa_sin,
                                             float cb, float sb_sin, uint32_t& info, float& nx, float& ny)
 {
     // the pair filters of updateManifolds (lib.zig:273-282) depend on the
     // pair only, so they run first: the first-occurrence test below costs
     // (cells of a before k) x (cells of b), large for wall-sized bodies
-    uint32_t fa = D.flags[sa], fb = D.flags[sb];
+    uint32_t fa = bodies.flags[sa], fb = bodies.flags[sb];
     bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
-    if (ok && D.nexcl) ok = !excluded(D, D.id[sa], D.id[sb]);
-    if (ok) ok = aabb_hit(D, sa, sb);
+    if (ok && D.nexcl) ok = !excluded(D, bodies.id[sa], bodies.id[sb]);
+    if (ok) ok = aabb_hit(bodies, sa, sb);
     // b first appears in a's query list at cell k unless an earlier cell of a
     // hashes to a bucket b also occupies (buckets are deduplicated, so
     // within one bucket b appears once)
@@ -535,8 +539,8 @@ __device__ __forceinline__ bool narrow_pair(const WorldDev& D, uint32_t sa, uint
         }
     }
     if (!ok) return false;
-    Geo ga = load_geo(D, sa, ca, sa_sin);
-    Geo gb = load_geo(D, sb, cb, sb_sin);
+    Geo ga = load_geo(bodies, sa, ca, sa_sin);
+    Geo gb = load_geo(bodies, sb, cb, sb_sin);
     // performNarrowSAT: lower id is the first reference body
     bool a_first = ga.id < gb.id;
     const Geo o1 = a_first ? ga : gb;   // by value: selecting references would put both in scratch
@@ -566,7 +570,7 @@ __global__ void k_narrow(StepDev S)
         uint32_t info = 0;
         float nx = 0.f, ny = 0.f;
         const WorldDev& D = S.worlds[S.bworld[a]];
-        bool hit = narrow_pair(D, a - D.body_base, b - D.body_base, k, S.ent_bkt + S.cell_off[a],
+        bool hit = narrow_pair(D, D, a - D.body_base, b - D.body_base, k, S.ent_bkt + S.cell_off[a],
                                S.ent_bkt + S.cell_off[b], S.ncell[b], S.cc[a], S.cs[a], S.cc[b], S.cs[b], info, nx, ny);
         S.cand_flag[q] = hit ? 1u : 0u;
         S.sat_info[q] = hit ? info : 0u;
@@ -629,15 +633,16 @@ __device__ __forceinline__ void clip_line(v2 aa, v2 ab, v2 ba, v2 bb, v2& o1, v2
 // with clipAgainstEdge (Rectangle.zig:213-238, Disc.zig:125-130), for the
 // collision of world slots sa, sb (narrow_pair's info and normal) into
 // manifold m; base: the world's first batch body index
-__device__ __forceinline__ void man_build_one(const StepDev& S, const WorldDev& D, uint32_t m, uint32_t base,
-                                              uint32_t sa, uint32_t sb, uint32_t info, v2 normal, float ca,
-                                              float sa_sin, float cb, float sb_sin)
+template <class B>
+__device__ __forceinline__ void man_build_one(const StepDev& S, const WorldDev& D, const B& bodies, uint32_t m,
+                                              uint32_t base, uint32_t sa, uint32_t sb, uint32_t info, v2 normal,
+                                              float ca, float sa_sin, float cb, float sb_sin)
 {
     int nid = (int)(info & 3u);
     bool ref_is_a = (info & 4u) != 0;
     uint32_t sr = ref_is_a ? sa : sb, si = ref_is_a ? sb : sa;
-    Geo gr = ref_is_a ? load_geo(D, sr, ca, sa_sin) : load_geo(D, sr, cb, sb_sin);
-    Geo gi = ref_is_a ? load_geo(D, si, cb, sb_sin) : load_geo(D, si, ca, sa_sin);
+    Geo gr = ref_is_a ? load_geo(bodies, sr, ca, sa_sin) : load_geo(bodies, sr, cb, sb_sin);
+    Geo gi = ref_is_a ? load_geo(bodies, si, cb, sb_sin) : load_geo(bodies, si, ca, sa_sin);
     S.m_ref[m] = base + sr;
     S.m_inc[m] = base + si;
     S.m_nx[m] = normal.x;
@@ -694,7 +699,7 @@ __global__ void k_man_build(StepDev S)
         const uint4 cq = S.cand[q];
         const uint32_t a = cq.x, b = cq.y;
         const WorldDev& D = S.worlds[S.bworld[a]];
-        man_build_one(S, D, S.cand_idx[q], D.body_base, a - D.body_base, b - D.body_base, S.sat_info[q],
+        man_build_one(S, D, D, S.cand_idx[q], D.body_base, a - D.body_base, b - D.body_base, S.sat_info[q],
                       mk(S.sat_nx[q], S.sat_ny[q]), S.cc[a], S.cs[a], S.cc[b], S.cs[b]);
     }
 }
@@ -2489,6 +2494,21 @@ __device__ __forceinline__ void fused_overflow(const StepDev& S, uint32_t w, uin
     }
 }
 
+// The solve of a fused step as a call of its own: its dependency loops get
+// the whole register file instead of sharing it with what the step keeps
+// live around them (values kept across the call are saved once per step).
+// One instance per kernel shape (WAVES sets the register budget).
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
+__device__ __noinline__ void fused_solve(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
+                                         uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
+                                         uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t* s_abort)
+{
+    if (nitems <= (uint32_t)NSLOT * MAXT)
+        slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
+    else
+        rec_world(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
+}
+
 // One workgroup per world (S.w_order), up to nsteps steps per world counted
 // in S.w_done, so a relaunch after an overflow resumes each world where it
 // stopped.  The solve takes the register-slot path when the step's items fit
@@ -2617,18 +2637,17 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         }
         __syncthreads();
         mark(FP_CANDS);
-        // ---- narrowphase (k_narrow)
+        // ---- narrowphase (k_narrow): the collision overwrites its candidate
+        // (partner, face id | ref-is-a, normal)
         for (uint32_t q = t; q < R; q += T) {
             const uint4 cq = cand[q];
             const uint32_t a = cq.x, b = cq.y;
             uint32_t info = 0;
             float nx = 0.f, ny = 0.f;
-            bool hit = narrow_pair(D, a, b, cq.z, ent_bkt + L.coff[a], ent_bkt + L.coff[b], L.coff[b + 1] - L.coff[b],
+            bool hit = narrow_pair(D, D, a, b, cq.z, ent_bkt + L.coff[a], ent_bkt + L.coff[b], L.coff[b + 1] - L.coff[b],
                                    L.cc[a], L.cs[a], L.cc[b], L.cs[b], info, nx, ny);
             cflag[q] = hit ? 1u : 0u;
-            S.sat_info[X.r0 + q] = info;
-            S.sat_nx[X.r0 + q] = nx;
-            S.sat_ny[X.r0 + q] = ny;
+            if (hit) cand[q] = make_uint4(a | (b << 16), info, __float_as_uint(nx), __float_as_uint(ny));
         }
         __syncthreads();
         const uint32_t M = group_scan(cflag, R, s_sm);
@@ -2643,10 +2662,10 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         for (uint32_t q = t; q < R; q += T) {
             const uint32_t mi = cflag[q];
             if (cflag[q + 1] == mi) continue;
-            const uint4 cq = cand[q];
-            const uint32_t a = cq.x, b = cq.y;
-            man_build_one(S, D, X.m0 + mi, base, a, b, S.sat_info[X.r0 + q],
-                          mk(S.sat_nx[X.r0 + q], S.sat_ny[X.r0 + q]), L.cc[a], L.cs[a], L.cc[b], L.cs[b]);
+            const uint4 x = cand[q];
+            const uint32_t a = x.x & 0xffffu, b = x.x >> 16;
+            man_build_one(S, D, D, X.m0 + mi, base, a, b, x.y, mk(__uint_as_float(x.z), __uint_as_float(x.w)), L.cc[a],
+                          L.cs[a], L.cc[b], L.cs[b]);
         }
         // ---- schedule: distance / offset joints make both bodies ordering (k_joint_mark)
         for (uint32_t j = t; j < nj; j += T) {
@@ -2713,16 +2732,19 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         __syncthreads();
         mark(FP_SCHED);
         // ---- solve
-        if (nit <= (uint32_t)NSLOT * MAXT)
-            slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
-                                               s_abort);
-        else
-            rec_world(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve, s_abort);
+        fused_solve<NSLOT, MAXT, WAVES, LOCKSTEP>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
+                                                  &s_abort);
         __syncthreads();
         mark(FP_SOLVE);
         if (s_abort) break;                     // the solve stalled: status set
         if (t == 0) {
             cnt[0] = E; cnt[1] = R; cnt[2] = M; cnt[3] = nit;
+            unsigned long long* sums = reinterpret_cast<unsigned long long*>(S.counters + C_SUMS);
+            atomicAdd(&sums[0], (unsigned long long)n);
+            atomicAdd(&sums[1], (unsigned long long)E);
+            atomicAdd(&sums[2], (unsigned long long)R);
+            atomicAdd(&sums[3], (unsigned long long)M);
+            atomicAdd(&sums[4], (unsigned long long)nit);
             S.w_man_begin[w] = X.m0;
             S.w_man_end[w] = X.m0 + M;
             S.w_done[w] = step + 1;
@@ -2783,7 +2805,9 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     {k_solve_slots<8, 128, 2>, 128, 8, k_solve<256, 2>, 256, k_fused<8, 128, 2, false>},
     {nullptr, 0, 0, k_solve<256, 2>, 256, nullptr},
     {k_solve_slots<4, 256, 3>, 256, 4, k_solve<128, 2>, 128, k_fused<4, 256, 3, false>},
-    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<256, 2>, 256, k_fused<3, 256, 3, true>},
+    // the lockstep slot walk takes the phase kernels only: its fused kernel
+    // hung once in a test with no cause found, so it is not built
+    {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<256, 2>, 256, nullptr},
 };
 
 bool fused_available(int v)

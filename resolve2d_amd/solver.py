@@ -86,6 +86,12 @@ class Context:
         names = ["cells", "buckets", "candidates", "narrowphase", "contacts", "schedule", "solve"]
         return {k: out[i] for i, k in enumerate(names)}
 
+    def fused_counts(self):
+        """Work of profiled fused launches since the last reset, summed over world-steps."""
+        out = (ctypes.c_uint64 * 5)()
+        N.check(self.lib.r2d_ctx_fused_counts(self.handle, out), "r2d_ctx_fused_counts")
+        return dict(bodies=out[0], entries=out[1], raw_candidates=out[2], manifolds=out[3], items=out[4])
+
     def set_profiling(self, on: bool):
         N.check(self.lib.r2d_ctx_set_profiling(self.handle, 1 if on else 0), "r2d_ctx_set_profiling")
         self.profiling = bool(on)

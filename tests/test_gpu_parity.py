@@ -90,10 +90,12 @@ def test_batched_equals_single(gpu_ctx, oracle):
         w.close()
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "phases"])
 @pytest.mark.parametrize("variant", list(range(9)))
-def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
+def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, fused):
     """Every launch configuration of the solve kernel (workgroup size per
-    world) gives the oracle's bits."""
+    world), in the fused kernel where the variant has one and in the phase
+    kernels, gives the oracle's bits."""
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld, apply_scene
     specs = [scenes.batched_world(21, n_min=40, n_max=60), scenes.batched_world(22),
@@ -105,10 +107,12 @@ def test_solve_variants_bit_identical(gpu_ctx, oracle, variant):
         apply_scene(sc, g)
         apply_scene(sc, o)
     gpu_ctx.set_solve_variant(variant)
+    gpu_ctx.set_fused(fused)
     try:
         gpu_ctx.process_batch_handles([g.handle for g in gw], scenes.DT, 4, 4, 60)
     finally:
         gpu_ctx.set_solve_variant(-1)
+        gpu_ctx.set_fused(True)
     for o in ow:
         for _ in range(60):
             assert o.process(scenes.DT, 4, 4) == 0
@@ -131,17 +135,21 @@ def test_autotune_picks_and_caches(gpu_ctx, oracle, tmp_path):
         apply_scene(sc, g)
         apply_scene(sc, o)
     path = os.path.join(str(tmp_path), "tune.json")
-    cands = [(1, 0), (6, 0), (7, 1024)]
+    cands = [(1, 0, True), (6, 0, False), (7, 1024, False)]   # fused; phase kernels
+    per = 2                                 # steps per timed sample
     try:
-        r = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, rounds=1, cands=cands, cache_path=path)
-        assert (r["variant"], r["lds_limit"]) in cands and not r["cached"]
+        r = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, rounds=1, cands=cands, cache_path=path,
+                          steps_per_sample=per)
+        assert (r["variant"], r["lds_limit"], r["fused"]) in cands and not r["cached"]
         r2 = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, cands=cands, cache_path=path)
         assert r2["cached"] and r2["variant"] == r["variant"]
     finally:
         gpu_ctx.set_solve_variant(-1)
         gpu_ctx.set_lds_limit(0)
+        gpu_ctx.set_fused(True)
+    assert r["step_ms"] > 0.0 and all(v > 0.0 for v in r["all_ms"].values())
     for o in ow:
-        for _ in range(len(cands)):
+        for _ in range(len(cands) * per):
             assert o.process(scenes.DT, 4, 4) == 0
     for g, o in zip(gw, ow):
         assert bits_equal(g.bodies(), o.bodies())
