@@ -516,7 +516,10 @@ __global__ void k_cand_emit(StepDev S)
 // in the lower body's query list; later occurrences are the reference's
 // "already a manifold / already rejected".  Returns whether it collides, with
 // the reference face id | ref-is-sa << 2 in info and the normal.
-template <class B>
+// PREFILTERED: the both-static and AABB filters were applied when the
+// candidate was emitted (the filters are pure predicates of the pair, so
+// their order does not change which pairs pass)
+template <bool PREFILTERED = false, class B>
 __device__ __forceinline__ bool narrow_pair(const WorldDev& D, const B& bodies, uint32_t sa, uint32_t sb, uint32_t k,
                                             const uint32_t* ba, const uint32_t* bb, uint32_t nb, float ca, float sa_sin,
                                             float cb, float sb_sin, uint32_t& info, float& nx, float& ny)
@@ -524,10 +527,13 @@ __device__ __forceinline__ bool narrow_pair(const WorldDev& D, const B& bodies, 
     // the pair filters of updateManifolds (lib.zig:273-282) depend on the
     // pair only, so they run first: the first-occurrence test below costs
     // (cells of a before k) x (cells of b), large for wall-sized bodies
-    uint32_t fa = bodies.flags[sa], fb = bodies.flags[sb];
-    bool ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
+    bool ok = true;
+    if (!PREFILTERED) {
+        uint32_t fa = bodies.flags[sa], fb = bodies.flags[sb];
+        ok = !((fa & BF_STATIC) && (fb & BF_STATIC));
+    }
     if (ok && D.nexcl) ok = !excluded(D, bodies.id[sa], bodies.id[sb]);
-    if (ok) ok = aabb_hit(bodies, sa, sb);
+    if (!PREFILTERED && ok) ok = aabb_hit(bodies, sa, sb);
     // b first appears in a's query list at cell k unless an earlier cell of a
     // hashes to a bucket b also occupies (buckets are deduplicated, so
     // within one bucket b appears once)
@@ -2391,6 +2397,28 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
 // paths) over the world's own scratch slices (WorldScratch), with the
 // per-body and per-bucket arrays in LDS.
 
+// The stale AABBs (aabb.zig:6-8) and flags of a world's bodies in LDS,
+// for the pair filters of the fused broadphase
+struct PairCache {
+    float *ax, *ay, *ahw, *ahh;
+    uint32_t* flags;
+};
+constexpr uint32_t PAIR_CACHE_WORDS = 5;   // per body
+constexpr uint32_t SOLVE_WORDS = 13;       // per body: the register-slot path's LDS body state (slot_lds_bytes)
+
+__device__ __forceinline__ PairCache pair_cache(uint32_t* p, uint32_t n)
+{
+    return PairCache{(float*)p, (float*)p + n, (float*)p + 2 * n, (float*)p + 3 * n, p + 4 * n};
+}
+
+// updateManifolds' pair filters that need no more than the cache: both
+// bodies static (lib.zig:273), padded AABB overlap (lib.zig:282)
+__device__ __forceinline__ uint32_t pair_passes(const PairCache& P, uint32_t a, uint32_t b)
+{
+    if ((P.flags[a] & BF_STATIC) && (P.flags[b] & BF_STATIC)) return 0u;
+    return aabb_hit(P, a, b) ? 1u : 0u;
+}
+
 // LDS of the fused kernel for a world of n bodies and tb hash buckets
 struct FusedLds {
     uint32_t* coff;    // [n+1] cells per body -> entry offsets
@@ -2422,7 +2450,8 @@ __device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint3
 size_t fused_lds_bytes(uint32_t n, uint32_t tb)
 {
     const size_t front = 6u * (size_t)n + 2u * (size_t)tb + 3u;
-    return front * 4u + std::max(slot_lds_bytes((int)n), solve_lds_bytes((int)n));
+    return front * 4u + std::max(std::max(slot_lds_bytes((int)n), solve_lds_bytes((int)n)),
+                                 (size_t)PAIR_CACHE_WORDS * n * 4u);
 }
 
 // exclusive scan of a[0, len) in place (LDS or global), a[len] = the total,
@@ -2494,12 +2523,12 @@ __device__ __forceinline__ void fused_overflow(const StepDev& S, uint32_t w, uin
     }
 }
 
-// The solve of a fused step as a call of its own: its dependency loops get
-// the whole register file instead of sharing it with what the step keeps
-// live around them (values kept across the call are saved once per step).
-// One instance per kernel shape (WAVES sets the register budget).
+// The solve of a fused step: the register-slot path when the step's items
+// fit the slots, else the record path.  Inlined: a call would save and
+// restore the step's registers around it every step (scratch traffic of the
+// order of the step's own); WAVES sets the register budget of the shape.
 template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
-__device__ __noinline__ void fused_solve(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
+__device__ __forceinline__ void fused_solve(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
                                          uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t* s_abort)
 {
@@ -2546,8 +2575,13 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         // ---- broadphase: per body (k_prep)
         if (t == 0) s_err = 0u;
         __syncthreads();
+        // stale AABBs and flags in LDS for the pair filters below (the solve's
+        // words are free until the solve)
+        const PairCache P = pair_cache(L.solve, n);
         for (uint32_t i = t; i < n; i += T) {
             const Prep o = prep_body(D, i);
+            P.ax[i] = D.ax[i]; P.ay[i] = D.ay[i]; P.ahw[i] = D.ahw[i]; P.ahh[i] = D.ahh[i];
+            P.flags[i] = D.flags[i];
             L.cc[i] = o.c;
             L.cs[i] = o.s;
             S.sflag[base + i] = o.res;
@@ -2564,10 +2598,15 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         mark(FP_PREP);
         if (E > X.ecap && !fused_grow(S, w, X, 0, E, s_slot)) { fused_overflow(S, w, E, 0, 0, 0); break; }
         uint32_t* ent_body = S.ent_body + X.e0;
-        uint32_t* ent_bkt = S.ent_bkt + X.e0;
         uint32_t* ecnt = S.ent_cnt + X.e0;      // raw candidates per entry -> offsets
-        uint32_t* items = S.bkt_items + X.e0;
-        uint32_t* sorted = S.bkt_sorted + X.e0;
+        // the entries' buckets and the bucket members are read in serial
+        // loops: in LDS after the pair cache when they fit the solve's words
+        // (3 E <= 8 n; E is about 1.5 n for bodies up to a cell wide)
+        const bool e_lds = 3 * (size_t)E <= (size_t)(SOLVE_WORDS - PAIR_CACHE_WORDS) * n;
+        uint32_t* e_free = L.solve + PAIR_CACHE_WORDS * n;
+        uint32_t* ent_bkt = e_lds ? e_free : S.ent_bkt + X.e0;
+        uint32_t* items = e_lds ? e_free + E : S.bkt_items + X.e0;
+        uint32_t* sorted = e_lds ? e_free + 2 * E : S.bkt_sorted + X.e0;
         for (uint32_t b = t; b <= tb; b += T) {
             L.boff[b] = 0u;
             if (b < tb) L.bfill[b] = 0u;
@@ -2611,11 +2650,13 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         }
         __syncthreads();
         mark(FP_BUCKETS);
-        // raw candidates per entry: distinct bucket members above the entry's body (k_ent_cnt)
+        // raw candidates per entry: distinct bucket members above the entry's
+        // body (k_ent_cnt) that pass the both-static and AABB filters
         for (uint32_t e = t; e < E; e += T) {
             uint32_t a = ent_body[e], b = ent_bkt[e];
             uint32_t o = L.boff[b], c = L.boff[b + 1] - o, q = 0;
-            for (uint32_t j = 0; j < c && sorted[o + j] > a; ++j) q += (j == 0 || sorted[o + j] != sorted[o + j - 1]);
+            for (uint32_t j = 0; j < c && sorted[o + j] > a; ++j)
+                if (j == 0 || sorted[o + j] != sorted[o + j - 1]) q += pair_passes(P, a, sorted[o + j]);
             ecnt[e] = q;
         }
         __syncthreads();
@@ -2630,8 +2671,9 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
             uint32_t ro = ecnt[e], c = ecnt[e + 1] - ro;
             uint32_t k = e - L.coff[a];
             for (uint32_t j = 0, q = 0; q < c; ++j) {
-                if (j > 0 && sorted[o + j] == sorted[o + j - 1]) continue;
-                cand[ro + q] = make_uint4(a, sorted[o + j], k, 0u);
+                const uint32_t bj = sorted[o + j];
+                if ((j > 0 && bj == sorted[o + j - 1]) || !pair_passes(P, a, bj)) continue;
+                cand[ro + q] = make_uint4(a, bj, k, 0u);
                 ++q;
             }
         }
@@ -2644,8 +2686,8 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
             const uint32_t a = cq.x, b = cq.y;
             uint32_t info = 0;
             float nx = 0.f, ny = 0.f;
-            bool hit = narrow_pair(D, D, a, b, cq.z, ent_bkt + L.coff[a], ent_bkt + L.coff[b], L.coff[b + 1] - L.coff[b],
-                                   L.cc[a], L.cs[a], L.cc[b], L.cs[b], info, nx, ny);
+            bool hit = narrow_pair<true>(D, D, a, b, cq.z, ent_bkt + L.coff[a], ent_bkt + L.coff[b],
+                                         L.coff[b + 1] - L.coff[b], L.cc[a], L.cs[a], L.cc[b], L.cs[b], info, nx, ny);
             cflag[q] = hit ? 1u : 0u;
             if (hit) cand[q] = make_uint4(a | (b << 16), info, __float_as_uint(nx), __float_as_uint(ny));
         }
