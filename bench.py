@@ -6,15 +6,23 @@ in [50, 500] (mixed discs / rectangles), each with 0-20 distance joints and
 iterations).  A step is one Solver.process over every world of the batch;
 inputs are resident on the device before timing starts.
 
+The timed region is one call of `steps` steps.  Worlds of at most 1,024
+bodies take whole steps per workgroup: one launch of the fused kernel runs
+every step of every world; `value` is bodies x steps / the call's wall time,
+`p50/p99_step_ms` are the world-steps' latencies (each workgroup's wall
+clock).  The launch shape (solve variant, fused or phase kernels) is picked
+by the autotuner on copies of the batch.
+
 The printed JSON line also carries: the same scenes stepped one world per
 launch ("plain"); the 4,096-body single world in the reference order and in
-the coloured schedule; a served Poisson stream of scenes of 16-8192 bodies
-(continuous batching, p50/p99 step and request latency); accuracy (bit
-equality with the f32 oracle, error against an f64 run of the same method,
-maximum penetration, energy drift of the resting pyramid); a roofline entry
-for the dominant kernel (phase times from HIP events on the library's
-stream); and a CPU baseline (the oracle restatement on a bounded sample,
-worlds in parallel over host cores).
+the coloured schedule; the 65,536-body constrained world; a served Poisson
+stream of scenes of 16-8192 bodies (continuous batching, p50/p99 step and
+request latency); accuracy (bit equality with the f32 oracle, error against
+an f64 run of the same method, maximum penetration, energy drift of the
+resting pyramid); a roofline entry for the dominant kernel (its launch timed
+with HIP events on the library's stream; HBM traffic from the committed
+rocprofv3 counter measurement; an LDS figure); and a CPU baseline (the
+oracle restatement on a bounded sample, worlds in parallel over host cores).
 """
 import argparse
 import json
@@ -37,7 +45,11 @@ G = 9.81
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def traffic_for(kernel, args):
+# fused kernel shape of each solve variant: <slots, workgroup size, waves per SIMD>
+FUSED_SHAPES = {1: "4,256,3", 2: "2,512,3", 3: "3,256,3", 4: "4,256,2", 5: "8,128,2", 7: "4,256,3"}
+
+
+def traffic_for(kernel, variant, args):
     """HBM bytes per launch of the kernel on this workload as counted by
     rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
     from the committed measurement; None when there is none for it."""
@@ -46,8 +58,9 @@ def traffic_for(kernel, args):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    key = f"{kernel}/worlds{args.worlds}/steps{args.steps}/seed{args.seed}"
-    return t.get(key)
+    shape = FUSED_SHAPES.get(variant) if kernel == "k_fused" else None
+    name = f"{kernel}<{shape}>" if shape else kernel
+    return t.get(f"{name}/worlds{args.worlds}/steps{args.steps}/seed{args.seed}")
 
 
 def log(msg):
@@ -479,6 +492,7 @@ def main():
         lds_bytes = S * K * (64 * counts["manifolds"] + 48 * max(0, counts["items"] - counts["manifolds"]))
         per_step_note = "per step, HIP events between phase launches"
     dom = max(per_step, key=per_step.get)
+    used_variant = tuned["variant"] if tuned else (args.variant if args.variant >= 0 else 1)
     achieved = launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
     lds_achieved = lds_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
 
@@ -585,8 +599,9 @@ def main():
                  "phase kernels: one launch sequence (hipGraph) per step"),
         "phase_ms": per_step,
         "phase_ms_note": per_step_note,
-        "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kernel, args),
+        "roofline": {"kernel": (f"{kernel}<{FUSED_SHAPES[used_variant]}>" if kernel == "k_fused" and
+                                used_variant in FUSED_SHAPES else kernel), "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kernel, used_variant, args),
                      "launch_ms": launch_s * 1e3, "algorithmic_bytes_per_launch": launch_bytes,
                      "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                              "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes},
