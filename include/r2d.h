@@ -232,9 +232,11 @@ int r2d_ctx_fused_counts(r2d_ctx* ctx, uint64_t* out);
  * counterpart in the reference (Solver.process is one world, one step). */
 int r2d_ctx_set_fused(r2d_ctx* ctx, int on);
 /* Launch configuration of the solve kernel: -1 (default) is variant 1;
- * 0..8 select the register-slot kernel's slots x workgroup size and the
- * record kernel's workgroup size (table in r2d_kernels.hip).  Every variant gives the same bits; the autotuner
- * (resolve2d_amd/autotune.py) picks the fastest for a batch shape. */
+ * 0..11 select the register-slot kernel's slots x workgroup size, the record
+ * kernel's workgroup size, and the fused kernel's shape (table in
+ * r2d_kernels.hip; 9-11 solve with records only).
+ * Every variant gives the same bits; the autotuner (resolve2d_amd/autotune.py)
+ * picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);
 /* Largest world (bodies) whose solver body state is kept in LDS; larger
  * worlds use the solve kernel with body state in global memory.  0 (default)

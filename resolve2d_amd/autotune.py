@@ -24,7 +24,10 @@ import numpy as np
 
 from . import _native as N
 
-SOLVE_VARIANTS = 9
+SOLVE_VARIANTS = 12
+# variants that differ on the phase kernels (9 and 10 are variant 6 there; 11
+# is the record kernel with 192 threads)
+PHASE_VARIANTS = tuple(range(9)) + (11,)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "resolve2d_amd", "autotune.json")
 
 
@@ -76,8 +79,9 @@ def save_cache(cache: Dict[str, dict], path: str = DEFAULT_CACHE):
     os.replace(tmp, path)
 
 
-# variants with a fused kernel (the register-slot shapes, lockstep walk aside)
-FUSED_VARIANTS = (1, 2, 3, 4, 5, 7)
+# variants with a fused kernel (the register-slot shapes, lockstep walk aside,
+# and the records-only shapes)
+FUSED_VARIANTS = (1, 2, 3, 4, 5, 7, 9, 10, 11)
 
 
 def candidates(variants: Optional[Iterable[int]] = None, lds_limits: Iterable[int] = (0, 1024),
@@ -88,7 +92,7 @@ def candidates(variants: Optional[Iterable[int]] = None, lds_limits: Iterable[in
     out = []
     for f in fused:
         for v in vs:
-            if f and v not in FUSED_VARIANTS:
+            if (f and v not in FUSED_VARIANTS) or (not f and v not in PHASE_VARIANTS):
                 continue
             for lim in ((0,) if f else lds_limits):
                 out.append((v, lim, bool(f)))

@@ -24,7 +24,7 @@ enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
 // solve-kernel variants (register-slot kernel shape + record-kernel workgroup
 // size; table in r2d_kernels.hip)
-constexpr int SOLVE_VARIANTS = 9;
+constexpr int SOLVE_VARIANTS = 12;
 unsigned solve_variant_threads(int v);
 int solve_variant_slots(int v);
 
@@ -87,7 +87,9 @@ void set_solve_smem_limit();
 constexpr uint32_t FUSED_NMAX = 1024;
 bool fused_available(int variant);
 // dynamic LDS of the fused kernel for a world of n bodies and tb hash buckets
-size_t fused_lds_bytes(uint32_t n, uint32_t tb);
+// (slots: the variant solves with register slots, else with records)
+size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots);
+bool fused_slots(int variant);
 // one workgroup per world (S.w_order), up to nsteps steps per world (S.w_done)
 // the fused kernel of a variant as compiled (registers, LDS, scratch): for error reports
 hipError_t fused_attributes(int variant, hipFuncAttributes* out);

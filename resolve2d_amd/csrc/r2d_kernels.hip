@@ -847,7 +847,12 @@ __device__ __forceinline__ void st_rel(uint32_t* p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-constexpr uint32_t SPIN_LIMIT = 1u << 22;
+// polls of one item before its world is stopped as stalled: ~2 s of waiting
+// with the poll's sleep; a wave can wait behind other worlds' waves on its
+// SIMD for a tenth of that and still be making progress
+constexpr uint32_t SPIN_LIMIT = 1u << 26;
+// the colouring polls global memory (~1 us a poll): its own, smaller limit
+constexpr uint32_t COLOR_SPIN_LIMIT = 1u << 22;
 
 // Coloured schedule (a world with r2d_set_schedule(w, 1)): greedy colouring
 // of the world's items in item order -- an item takes the smallest colour
@@ -908,7 +913,7 @@ __global__ __launch_bounds__(256) void k_color(StepDev S)
             spins = 0;
         } else {
             // every item's predecessors come first, so some item is always ready
-            if (++spins > SPIN_LIMIT) {
+            if (++spins > COLOR_SPIN_LIMIT) {
                 __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 S.w_status[w] = 5;
                 break;
@@ -1700,6 +1705,8 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
                 }
             }
             if (ready) {
+                // a wave with a ready link issues ahead of the waves that poll
+                __builtin_amdgcn_s_setprio(2);
                 if (!contact) {
                     const uint32_t ua = COMPACT ? (cur.cpair & 0xffffu) : cur.a;
                     const uint32_t ub = COMPACT ? (cur.cpair >> 16) : (cur.b == NONE ? cur.a : cur.b);
@@ -1720,6 +1727,7 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
                     if (r0 >= 0) st_rel(&st.cnt[r0], t0 + 1u);
                     if (r1 >= 0) st_rel(&st.cnt[r1], t1 + 1u);
                 }
+                __builtin_amdgcn_s_setprio(0);
                 if (has_nxt && nxt_pos == cur_pos) nxt = cur;
                 have = false;
                 spins = 0;
@@ -2312,6 +2320,8 @@ __device__ __forceinline__ void slots_world(const StepDev& S, const WorldDev& D,
                 }
                 const bool ready = c0 == t0 && c1 == t1;
                 if (ready) {
+                    // a wave with a ready link issues ahead of the waves that poll
+                    __builtin_amdgcn_s_setprio(2);
                     if (!contact) {
                         slot_solve_joint(cur, st, sub_dt);
                     } else {
@@ -2326,6 +2336,7 @@ __device__ __forceinline__ void slots_world(const StepDev& S, const WorldDev& D,
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __builtin_amdgcn_s_setprio(0);
                     have = false;
                     spins = 0;
                     if (++cz == nslot) { cz = 0; ++k; }
@@ -2404,7 +2415,6 @@ struct PairCache {
     uint32_t* flags;
 };
 constexpr uint32_t PAIR_CACHE_WORDS = 5;   // per body
-constexpr uint32_t SOLVE_WORDS = 13;       // per body: the register-slot path's LDS body state (slot_lds_bytes)
 
 __device__ __forceinline__ PairCache pair_cache(uint32_t* p, uint32_t n)
 {
@@ -2447,11 +2457,15 @@ __device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint3
     return L;
 }
 
-size_t fused_lds_bytes(uint32_t n, uint32_t tb)
+// words per body after the front part: the solve's LDS body state (register
+// slots 13, records 7), which also holds the pair cache and, when they fit,
+// the bucket arrays until the solve
+__host__ __device__ constexpr uint32_t fused_solve_words(bool slots) { return slots ? 13u : 7u; }
+
+size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots)
 {
     const size_t front = 6u * (size_t)n + 2u * (size_t)tb + 3u;
-    return front * 4u + std::max(std::max(slot_lds_bytes((int)n), solve_lds_bytes((int)n)),
-                                 (size_t)PAIR_CACHE_WORDS * n * 4u);
+    return (front + (size_t)fused_solve_words(slots) * n) * 4u;
 }
 
 // exclusive scan of a[0, len) in place (LDS or global), a[len] = the total,
@@ -2532,10 +2546,14 @@ __device__ __forceinline__ void fused_solve(const StepDev& S, const WorldDev& D,
                                          uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t* s_abort)
 {
-    if (nitems <= (uint32_t)NSLOT * MAXT)
-        slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
-    else
-        rec_world(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
+    if constexpr (NSLOT > 0) {
+        if (nitems <= (uint32_t)NSLOT * MAXT) {
+            slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds,
+                                               *s_abort);
+            return;
+        }
+    }
+    rec_world(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
 }
 
 // One workgroup per world (S.w_order), up to nsteps steps per world counted
@@ -2602,7 +2620,7 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         // the entries' buckets and the bucket members are read in serial
         // loops: in LDS after the pair cache when they fit the solve's words
         // (3 E <= 8 n; E is about 1.5 n for bodies up to a cell wide)
-        const bool e_lds = 3 * (size_t)E <= (size_t)(SOLVE_WORDS - PAIR_CACHE_WORDS) * n;
+        const bool e_lds = 3 * (size_t)E <= (size_t)(fused_solve_words(NSLOT > 0) - PAIR_CACHE_WORDS) * n;
         uint32_t* e_free = L.solve + PAIR_CACHE_WORDS * n;
         uint32_t* ent_bkt = e_lds ? e_free : S.ent_bkt + X.e0;
         uint32_t* items = e_lds ? e_free + E : S.bkt_items + X.e0;
@@ -2850,7 +2868,19 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     // the lockstep slot walk takes the phase kernels only: its fused kernel
     // hung once in a test with no cause found, so it is not built
     {k_solve_slots<3, 256, 3, true>, 256, 3, k_solve<256, 2>, 256, nullptr},
+    // records only (as variant 6), fused with 3 or 4 waves per SIMD: fewer
+    // registers than the slots, so more worlds resident per CU
+    {nullptr, 0, 0, k_solve<256, 2>, 256, k_fused<0, 256, 3, false>},
+    {nullptr, 0, 0, k_solve<256, 2>, 256, k_fused<0, 256, 4, false>},
+    // records only, 192-thread workgroups: with the records' smaller LDS
+    // state, four worlds of 500 bodies are resident per CU
+    {nullptr, 0, 0, k_solve<192, 3>, 192, k_fused<0, 192, 3, false>},
 };
+
+bool fused_slots(int v)
+{
+    return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].slot_fn != nullptr;
+}
 
 bool fused_available(int v)
 {
@@ -2866,7 +2896,8 @@ void launch_fused(const StepDev& S, int v, uint32_t nworlds, float dt, uint32_t 
                   uint32_t nsteps, size_t lds, hipStream_t st)
 {
     const SolveVariant& sv = kSolveVariants[v];
-    hipLaunchKernelGGL(sv.fused_fn, dim3(nworlds), dim3(sv.slot_threads), lds, st, S, dt, sub_steps, iters, nsteps);
+    const unsigned threads = sv.slot_fn ? sv.slot_threads : sv.threads;
+    hipLaunchKernelGGL(sv.fused_fn, dim3(nworlds), dim3(threads), lds, st, S, dt, sub_steps, iters, nsteps);
 }
 
 unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }

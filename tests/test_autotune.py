@@ -19,7 +19,7 @@ def test_pick_median_and_ties():
 def test_candidates_cover_variants():
     c = autotune.candidates(lds_limits=(0, 1024))
     # phase kernels: every variant x both LDS limits; fused: the variants with a fused kernel
-    assert len(c) == 2 * autotune.SOLVE_VARIANTS + len(autotune.FUSED_VARIANTS)
+    assert len(c) == 2 * len(autotune.PHASE_VARIANTS) + len(autotune.FUSED_VARIANTS)
     assert (1, 0, False) in c and (7, 1024, False) in c and (3, 0, True) in c
     assert (0, 0, True) not in c and (8, 0, True) not in c and (3, 1024, True) not in c
     assert autotune.candidates(variants=[3], lds_limits=(0,), fused=(True,)) == [(3, 0, True)]
