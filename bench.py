@@ -264,7 +264,7 @@ def served_arrivals(capacity, seconds, load, seed, steps):
     return rate, [(t, (n, s)) for t, n, s in arr]
 
 
-def serve_stream(ctx, rate, arrivals, steps, load):
+def serve_stream(ctx, rate, arrivals, steps, load, per_tick):
     """Serve a stream with continuous batching; returns its figures."""
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld
@@ -277,7 +277,7 @@ def serve_stream(ctx, rate, arrivals, steps, load):
     def done(payload, w, st):
         w.close()
 
-    sv = Server(ctx, hbm_budget_bytes=64 << 30, max_worlds=4096)
+    sv = Server(ctx, hbm_budget_bytes=64 << 30, max_worlds=4096, steps_per_tick=per_tick)
     try:
         r = run_stream(sv, arrivals, build, steps, on_done=done)
     finally:
@@ -289,6 +289,7 @@ def serve_stream(ctx, rate, arrivals, steps, load):
             "p50_batch_ms": st["p50_batch_ms"], "p99_batch_ms": st["p99_batch_ms"],
             "p50_request_ms": r.get("request_p50_ms"), "p99_request_ms": r.get("request_p99_ms"),
             "mean_scenes_per_batch": st["scene_steps"] / max(1, st["batch_steps"]), "lanes": st["lanes"],
+            "steps_per_launch": per_tick,
             "peak_resident_bytes": st["peak_resident_bytes"]}
 
 
@@ -333,6 +334,8 @@ def main():
     ap.add_argument("--serve-sat-seconds", type=float, default=4.0,
                     help="also serve an overloaded stream (3x capacity) for this long (0: skip)")
     ap.add_argument("--serve-plain-scenes", type=int, default=6)
+    ap.add_argument("--serve-steps-per-tick", type=int, default=8,
+                    help="steps a serving lane's launch takes (fewer when a running scene has fewer left)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
@@ -542,11 +545,11 @@ def main():
     if not args.no_serve:
         log("serving stream")
         rate, arrivals = served_arrivals(value, args.serve_seconds, args.serve_load, args.seed, args.steps)
-        serve = serve_stream(ctx, rate, arrivals, args.steps, args.serve_load)
+        serve = serve_stream(ctx, rate, arrivals, args.steps, args.serve_load, args.serve_steps_per_tick)
         log(f"served {serve['scenes']} scenes, {serve['value'] / 1e6:.1f} M body-steps/s")
         if args.serve_sat_seconds > 0:
             rate2, arr2 = served_arrivals(value, args.serve_sat_seconds, 3.0, args.seed + 1, args.steps)
-            sat = serve_stream(ctx, rate2, arr2, args.steps, 3.0)
+            sat = serve_stream(ctx, rate2, arr2, args.steps, 3.0, args.serve_steps_per_tick)
             serve["saturated"] = {k: sat[k] for k in ("value", "offered_load", "scenes", "wall_s", "p50_step_ms",
                                                       "p99_step_ms", "p50_request_ms", "p99_request_ms")}
             log(f"saturated: {sat['value'] / 1e6:.1f} M body-steps/s")

@@ -294,6 +294,11 @@ typedef struct {
     uint32_t split_bodies[2];   /* lanes by world size: n <= split[0], n <= split[1], larger;
                                    {0, 0} = one lane.  Each lane has its own stream and scratch
                                    and steps at its own pace. */
+    uint32_t steps_per_tick;    /* steps a lane's launch takes for its running worlds: the
+                                   smaller of this and the fewest steps any of them has left
+                                   (0 or 1: one step per launch).  More steps per launch
+                                   amortise launch and collection; admissions wait for the
+                                   launch in flight. */
 } r2d_serve_options;
 
 typedef struct {

@@ -70,7 +70,7 @@ class ManifoldState(ctypes.Structure):
 class ServeOptions(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_worlds", ctypes.c_uint32), ("dt", ctypes.c_float),
                 ("sub_steps", ctypes.c_uint32), ("collision_iters", ctypes.c_uint32),
-                ("split_bodies", ctypes.c_uint32 * 2)]
+                ("split_bodies", ctypes.c_uint32 * 2), ("steps_per_tick", ctypes.c_uint32)]
 
 
 class ServeTickInfo(ctypes.Structure):

@@ -1192,8 +1192,9 @@ struct r2d_server {
         std::deque<uint64_t> queue;           // waiting for admission, in arrival order
         std::vector<uint64_t> running;        // admitted, in order of admission
         bool inflight = false;
-        Batch batch;                          // the launched batch step
+        Batch batch;                          // the launched batch step(s)
         std::vector<uint64_t> batch_tickets;
+        uint32_t batch_steps = 1;             // steps of the launched batch
         double t_launch = 0;
     };
     r2d_ctx* home = nullptr;
@@ -1242,8 +1243,9 @@ int lane_collect(r2d_server* sv, r2d_server::Lane& L)
     if (!L.inflight) return R2D_OK;
     L.inflight = false;
     int rc = batch_wait(L.ctx, L.batch);
-    double wall = now_ms() - L.t_launch;
-    float gms = batch_gpu_ms(L.ctx);
+    const uint32_t k = std::max<uint32_t>(L.batch_steps, 1);
+    double wall = (now_ms() - L.t_launch) / k;   // per step of the launch
+    float gms = batch_gpu_ms(L.ctx) / (float)k;
     const size_t nw = L.batch_tickets.size();
     // per-world failures arrive as world statuses; anything else fails the batch
     const bool per_world = rc == R2D_OK || L.batch.world_status.size() == nw;
@@ -1256,21 +1258,21 @@ int lane_collect(r2d_server* sv, r2d_server::Lane& L)
             retire(sv, t, R2D_REQ_FAILED, ws == 4 ? R2D_E_NONFINITE : ws == 5 ? R2D_E_DEVICE : R2D_E_RANGE);
             continue;
         }
-        q.done += 1;
+        q.done += k;
         q.max_step_ms = std::max(q.max_step_ms, (double)gms);
-        sv->body_steps += (uint64_t)q.w->n;
+        sv->body_steps += (uint64_t)q.w->n * k;
         if (q.done >= q.steps) retire(sv, t, R2D_REQ_DONE, R2D_OK);
     }
     std::vector<uint64_t> still;
     for (uint64_t t : L.running)
         if (sv->reqs[t - 1].state == R2D_REQ_RUNNING) still.push_back(t);
     L.running.swap(still);
-    sv->batch_steps += 1;
-    sv->scene_steps += nw;
-    sv->gpu_ms += gms;
-    sv->wall_ms += wall;
+    sv->batch_steps += k;
+    sv->scene_steps += nw * k;
+    sv->gpu_ms += (double)gms * k;
+    sv->wall_ms += wall * k;
     sv->step_ms.push_back(gms);
-    sv->step_worlds.push_back((uint32_t)nw);
+    sv->step_worlds.push_back((uint32_t)(nw * k));
     sv->last_gpu_ms = std::max(sv->last_gpu_ms, (double)gms);
     sv->last_wall_ms = std::max(sv->last_wall_ms, wall);
     return per_world ? R2D_OK : rc;
@@ -1316,7 +1318,14 @@ int lane_launch(r2d_server* sv, r2d_server::Lane& L)
     ws.reserve(L.running.size());
     for (uint64_t t : L.running) ws.push_back(sv->reqs[t - 1].w);
     L.ctx->solve_variant = sv->home->solve_variant;
-    int rc = batch_prepare(L.ctx, ws.data(), ws.size(), sv->opt.dt, sv->opt.sub_steps, sv->opt.collision_iters, 1,
+    L.ctx->fused_enabled = sv->home->fused_enabled;
+    // as many steps as every running world still needs, up to steps_per_tick
+    uint32_t k = std::max<uint32_t>(sv->opt.steps_per_tick, 1);
+    for (uint64_t t : L.running) {
+        const r2d_server::Req& q = sv->reqs[t - 1];
+        k = std::min<uint32_t>(k, q.steps > q.done ? q.steps - q.done : 1u);
+    }
+    int rc = batch_prepare(L.ctx, ws.data(), ws.size(), sv->opt.dt, sv->opt.sub_steps, sv->opt.collision_iters, k,
                            L.batch);
     if (!rc) rc = batch_launch(L.ctx, L.batch);
     if (rc) {
@@ -1325,6 +1334,7 @@ int lane_launch(r2d_server* sv, r2d_server::Lane& L)
         return rc;
     }
     L.batch_tickets = L.running;
+    L.batch_steps = k;
     L.t_launch = now_ms();
     L.inflight = true;
     return R2D_OK;

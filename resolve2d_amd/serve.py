@@ -18,15 +18,17 @@ from .scenes import DT, ITERS, SUB_STEPS
 
 class Server:
     def __init__(self, ctx, hbm_budget_bytes: int, max_worlds: int, dt: float = DT, sub_steps: int = SUB_STEPS,
-                 collision_iters: int = ITERS, split_bodies=(1024, 4096)):
+                 collision_iters: int = ITERS, split_bodies=(1024, 4096), steps_per_tick: int = 1):
         """split_bodies: lanes by world size (n <= split[0], n <= split[1],
-        larger); (0, 0) steps every world in one lane."""
+        larger); (0, 0) steps every world in one lane.  steps_per_tick: steps
+        a lane's launch takes (fewer when a running world needs fewer)."""
         self.ctx = ctx
         self.lib = ctx.lib
         o = N.ServeOptions()
         o.hbm_budget_bytes, o.max_worlds, o.dt = hbm_budget_bytes, max_worlds, dt
         o.sub_steps, o.collision_iters = sub_steps, collision_iters
         o.split_bodies[0], o.split_bodies[1] = split_bodies
+        o.steps_per_tick = steps_per_tick
         h = ctypes.c_void_p()
         N.check(self.lib.r2d_serve_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)), "r2d_serve_create")
         self.handle = h

@@ -47,14 +47,17 @@ def _alone(ctx, sp):
     return w
 
 
-def test_served_results_equal_alone(gpu_ctx):
+@pytest.mark.parametrize("per_tick", [1, 5])
+def test_served_results_equal_alone(gpu_ctx, per_tick):
+    """per_tick: steps a lane's launch takes (5: fewer when a running world
+    has fewer left, so requests still end after exactly STEPS steps)."""
     from resolve2d_amd.serve import Server
     mix = _mix(gpu_ctx)
     fps = [w.footprint() for _, w in mix]
     # room for about three of the larger worlds at once: requests queue and
     # are admitted as others finish, at every stage of their neighbours' runs
     budget = sorted(fps)[-1] + sorted(fps)[-2] + sorted(fps)[-3]
-    sv = Server(gpu_ctx, budget, max_worlds=4)
+    sv = Server(gpu_ctx, budget, max_worlds=4, steps_per_tick=per_tick)
     try:
         tickets = []
         order = list(range(len(mix)))
