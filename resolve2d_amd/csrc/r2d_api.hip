@@ -780,7 +780,7 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         maxtb = std::max(maxtb, b.descs[i].table);
     }
     if (fused) {
-        b.fused_lds = fused_lds_bytes(maxn, maxtb, fused_slots(variant));
+        b.fused_lds = fused_lds_bytes(maxn, maxtb, fused_slots(variant), fused_threads(variant));
         fused = b.fused_lds <= kFusedLdsMax;
     }
     b.fused = fused;

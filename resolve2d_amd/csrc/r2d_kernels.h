@@ -87,9 +87,11 @@ void set_solve_smem_limit();
 constexpr uint32_t FUSED_NMAX = 1024;
 bool fused_available(int variant);
 // dynamic LDS of the fused kernel for a world of n bodies and tb hash buckets
-// (slots: the variant solves with register slots, else with records)
-size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots);
+// (slots: the variant solves with register slots, else with records;
+// threads: its workgroup size, which sizes the record chain's lane slots)
+size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots, uint32_t threads);
 bool fused_slots(int variant);
+uint32_t fused_threads(int variant);
 // one workgroup per world (S.w_order), up to nsteps steps per world (S.w_done)
 // the fused kernel of a variant as compiled (registers, LDS, scratch): for error reports
 hipError_t fused_attributes(int variant, hipFuncAttributes* out);

@@ -35,6 +35,7 @@ namespace r2d {
 __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a < b ? b : a; }
 
 // number of valid elements given a device count clamped to capacity
 __device__ __forceinline__ uint32_t dev_count(const uint32_t* c, uint32_t cap)
@@ -1426,31 +1427,37 @@ __device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t 
     v2 t = rot90cw(n);
     v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
     float rot1 = 0.f, rot2 = 0.f;
+    // straight-line code with selects: a point that is absent, separated
+    // (bias < 0: its impulses reset) or below the minimum impulse changes
+    // nothing else, exactly as skipping it would
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (!(z.flags & (RF_P0 << p))) continue;
-        if (z.bias[p] < 0.f) { z.acc[2 * p] = 0.f; z.acc[2 * p + 1] = 0.f; continue; }
+        const bool on = (z.flags & (RF_P0 << p)) != 0u;
+        const bool sep = z.bias[p] < 0.f;
         v2 r1 = mk(z.p[p][0], z.p[p][1]), r2 = mk(z.p[p][2], z.p[p][3]);
         v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
         v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
         v2 dv = sub2(v1, v2v);
         float num = dot2(dv, n) + z.bias[p];
         float pn = num * z.p[p][5];
-        if (pn < MIN_MANIFOLD_IMPULSE) continue;
+        const bool go = on && !sep && !(pn < MIN_MANIFOLD_IMPULSE);
         num = dot2(dv, t);
         float pt = num * z.p[p][6];
         float acc_pn = z.acc[2 * p], acc_pt = z.acc[2 * p + 1];
         float new_pn = maxn(0.0f, acc_pn + pn);
         float applied_pn = new_pn - acc_pn;
-        acc_pn = new_pn;
-        float max_pt = z.fric * absf(acc_pn);
+        float max_pt = z.fric * absf(new_pn);
         float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
         float applied_pt = new_pt - acc_pt;
-        z.acc[2 * p] = acc_pn;
-        z.acc[2 * p + 1] = new_pt;
+        const bool clr = on && sep;
+        z.acc[2 * p] = clr ? 0.f : (go ? new_pn : acc_pn);
+        z.acc[2 * p + 1] = clr ? 0.f : (go ? new_pt : acc_pt);
         v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
-        if (!(z.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
-        if (!(z.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
+        const bool g1 = go && !(z.flags & RF_S1), g2 = go && !(z.flags & RF_S2);
+        const float a1x = app1.x - dp.x, a1y = app1.y - dp.y, q1 = rot1 - cross2(r1, dp);
+        const float a2x = app2.x + dp.x, a2y = app2.y + dp.y, q2 = rot2 + cross2(r2, dp);
+        app1.x = g1 ? a1x : app1.x; app1.y = g1 ? a1y : app1.y; rot1 = g1 ? q1 : rot1;
+        app2.x = g2 ? a2x : app2.x; app2.y = g2 ? a2y : app2.y; rot2 = g2 ? q2 : rot2;
     }
     if (z.flags & RF_W1) { st.mx[l1] = b1.mx + app1.x; st.my[l1] = b1.my + app1.y; st.L[l1] = b1.L + rot1; }
     if (z.flags & RF_W2) { st.mx[l2] = b2.mx + app2.x; st.my[l2] = b2.my + app2.y; st.L[l2] = b2.L + rot2; }
@@ -1747,6 +1754,152 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
     }
 }
 
+// Keeps the compiler from moving LDS accesses across this point, without a
+// wait: one workgroup's LDS operations are served in the order they are
+// issued, so program order is all the dataflow needs.
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+// A lane's slot: the fields of its current record that a chain link reads
+// besides the header (flags, bodies), as six 16-byte rows in LDS:
+// (nx ny fric p00) (p01 p02 p03 p04) (p05 p06 p10 p11) (p12 p13 p14 p15)
+// (p16 bias0 bias1 -) (acc0 acc1 acc2 acc3).
+constexpr uint32_t LANE_ROWS = 6;
+constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
+
+__device__ __forceinline__ void lane_store(uint4* q, const Rec& r)
+{
+    q[0] = make_uint4(__float_as_uint(r.nx), __float_as_uint(r.ny), __float_as_uint(r.fric),
+                      __float_as_uint(r.p[0][0]));
+    q[1] = make_uint4(__float_as_uint(r.p[0][1]), __float_as_uint(r.p[0][2]), __float_as_uint(r.p[0][3]),
+                      __float_as_uint(r.p[0][4]));
+    q[2] = make_uint4(__float_as_uint(r.p[0][5]), __float_as_uint(r.p[0][6]), __float_as_uint(r.p[1][0]),
+                      __float_as_uint(r.p[1][1]));
+    q[3] = make_uint4(__float_as_uint(r.p[1][2]), __float_as_uint(r.p[1][3]), __float_as_uint(r.p[1][4]),
+                      __float_as_uint(r.p[1][5]));
+    q[4] = make_uint4(__float_as_uint(r.p[1][6]), __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), 0u);
+    q[5] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
+                      __float_as_uint(r.acc[3]));
+}
+
+__device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
+{
+    const uint4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+    r.nx = __uint_as_float(a.x); r.ny = __uint_as_float(a.y); r.fric = __uint_as_float(a.z);
+    r.p[0][0] = __uint_as_float(a.w);
+    r.p[0][1] = __uint_as_float(b.x); r.p[0][2] = __uint_as_float(b.y); r.p[0][3] = __uint_as_float(b.z);
+    r.p[0][4] = __uint_as_float(b.w);
+    r.p[0][5] = __uint_as_float(c.x); r.p[0][6] = __uint_as_float(c.y); r.p[1][0] = __uint_as_float(c.z);
+    r.p[1][1] = __uint_as_float(c.w);
+    r.p[1][2] = __uint_as_float(d.x); r.p[1][3] = __uint_as_float(d.y); r.p[1][4] = __uint_as_float(d.z);
+    r.p[1][5] = __uint_as_float(d.w);
+    r.p[1][6] = __uint_as_float(e.x); r.bias[0] = __uint_as_float(e.y); r.bias[1] = __uint_as_float(e.z);
+    r.acc[0] = __uint_as_float(f.x); r.acc[1] = __uint_as_float(f.y); r.acc[2] = __uint_as_float(f.z);
+    r.acc[3] = __uint_as_float(f.w);
+}
+
+// The same ordered dataflow as item_chain (body state in LDS), with a
+// lane's current record kept in its LDS slot (lanes: blockDim.x slots of
+// LANE_WORDS, 16-byte aligned) instead of registers: the loop carries only
+// the header (flags, bodies, ranks, targets), so an iteration moves no
+// record between registers, and a poll reads both counters and both bodies'
+// state in one batch of LDS reads.  The record is read from the slot once the
+// item is ready; a lane whose next position is its current one (one item per
+// iteration) keeps the slot, whose impulses are the ones just written.
+template <class ST>
+__device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
+                                                 uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
+                                                 uint32_t& s_abort)
+{
+    const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
+    uint4* slot = lanes + (size_t)threadIdx.x * LANE_ROWS;
+    uint32_t pos = p_begin, k = (p_begin < nitems) ? 0u : iters;
+    uint32_t held = NONE;                    // position whose record the slot holds
+    uint32_t flags = 0, a = 0, b = 0, rk0 = 0, rk1 = 0, degs = 0;
+    int32_t r0 = -1, r1 = -1;
+    uint32_t t0 = 0, t1 = 0;
+    bool have = false;
+    uint32_t spins = 0;
+    while (true) {
+        if (!have) {
+            if (k >= iters) break;
+            if (pos != held) {
+                Rec r;
+                rec_load(r, recs + (size_t)pos * 8);
+                lane_store(slot, r);
+                flags = r.flags; a = r.a; b = r.b; rk0 = r.rk0; rk1 = r.rk1; degs = r.degs;
+                held = pos;
+            }
+            if (flags & RF_JOINT) {
+                r0 = a == NONE ? -1 : (int32_t)a;
+                r1 = b == NONE ? -1 : (int32_t)b;
+            } else {
+                r0 = (flags & RF_W1) ? (int32_t)a : -1;
+                r1 = (flags & RF_W2) ? (int32_t)b : -1;
+            }
+            if (degs != DEGS_WIDE) {
+                t0 = k * (degs & 0xffffu) + rk0;
+                t1 = k * (degs >> 16) + rk1;
+            } else {
+                t0 = r0 >= 0 ? k * st.deg[r0] + rk0 : 0u;
+                t1 = r1 >= 0 ? k * st.deg[r1] + rk1 : 0u;
+            }
+            have = true;
+        }
+        // one batch of LDS reads: both counters (a lane without an ordering
+        // body reads body 0's and ignores it) and, for a contact, both bodies'
+        // state (LDS serves one workgroup's operations in order, so state read
+        // with a counter that shows the item ready is what its predecessor
+        // released; values read for an item that is not ready are discarded)
+        const bool contact = !(flags & RF_JOINT);
+        const uint32_t q0 = r0 >= 0 ? (uint32_t)r0 : 0u, q1 = r1 >= 0 ? (uint32_t)r1 : 0u;
+        const uint32_t c0 = __hip_atomic_load(&st.cnt[q0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t c1 = __hip_atomic_load(&st.cnt[q1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        lds_order();
+        const BodyState b1 = body_state(st, contact ? a : 0u);
+        const BodyState b2 = body_state(st, contact ? b : 0u);
+        const bool ready = (r0 < 0 || c0 == t0) && (r1 < 0 || c1 == t1);
+        if (ready) {
+            // a wave with a ready link issues ahead of the waves that poll
+            __builtin_amdgcn_s_setprio(2);
+            Rec cur;
+            cur.flags = flags; cur.a = a; cur.b = b;
+            lane_load(cur, slot);
+            if (!contact) {
+                rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
+            } else {
+                contact_impulses(cur, a, b, b1, b2, st);
+                const uint4 acc = make_uint4(__float_as_uint(cur.acc[0]), __float_as_uint(cur.acc[1]),
+                                             __float_as_uint(cur.acc[2]), __float_as_uint(cur.acc[3]));
+                recs[(size_t)pos * 8 + 6] = acc;
+                slot[5] = acc;
+            }
+            // the state writes above reach LDS before the counters (in order)
+            lds_order();
+            if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_s_setprio(0);
+            pos += p_step;
+            if (pos >= nitems) { pos = p_begin; ++k; }
+            have = false;
+            spins = 0;
+        } else {
+            // every item's predecessors precede it, so some item is always
+            // ready; a stall this long means corrupt ordering data: stop
+            ++spins;
+            if ((spins & 255u) == 0u) {
+                if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                if (spins > SPIN_LIMIT) {
+                    __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    S.w_status[w] = 5;
+                    break;
+                }
+            }
+        }
+        // a wave with no lane ready yields its SIMD
+        if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // the records' impulses, effective masses, lever arms and depths -> the
 // step's manifold arrays (what r2d_get_manifolds reports)
 __device__ __forceinline__ void item_writeback(const StepDev& S, uint4* recs, uint32_t pos)
@@ -1765,9 +1918,13 @@ __device__ __forceinline__ void item_writeback(const StepDev& S, uint4* recs, ui
 // so the earliest unfinished item always has a worker.  Body state lives in
 // LDS (lds: solve_lds_bytes(n) bytes); records at the world's item range
 // [ibase, ibase + nitems) of S.recs; degs: the world's item counts per body.
+// LANES: the chain keeps each lane's current record in its slot of `lanes`
+// (LDS, blockDim.x * LANE_WORDS words, 16-byte aligned, disjoint from lds).
+template <bool LANES>
 __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
                                           uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
-                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t& s_abort)
+                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint4* lanes,
+                                          uint32_t& s_abort)
 {
     if (threadIdx.x == 0) s_abort = 0u;
     const int n = D.nbodies;
@@ -1791,7 +1948,8 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
         __syncthreads();
         for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
         __syncthreads();
-        item_chain<true, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        if constexpr (LANES) item_chain_lanes(S, st, recs, lanes, nitems, iters, sub_dt, w, s_abort);
+        else item_chain<true, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
         __syncthreads();
@@ -1808,7 +1966,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap, uint32_t slot_nmax,
              uint32_t lds_nmax)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = S.w_order[blockIdx.x];
@@ -1817,7 +1975,8 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const WorldDev& D = S.worlds[w];
     if (solve_route((uint32_t)D.nbodies, nitems, slot_cap, slot_nmax, lds_nmax) != 1) return;
-    rec_world(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, smem, s_abort);
+    rec_world<false>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, smem,
+                     nullptr, s_abort);
 }
 
 // ------------------------------------- worlds beyond LDS: split sub-steps
@@ -1948,7 +2107,7 @@ constexpr size_t COMPACT_LDS = 160 * 1024 - 1024;
 template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax, uint32_t iters, float sub_dt)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = S.w_order[blockIdx.x];
@@ -2384,7 +2543,7 @@ template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP = false>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_nmax)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_abort;
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = S.w_order[blockIdx.x];
@@ -2439,12 +2598,20 @@ struct FusedLds {
     uint32_t* doff;    // [n+1] -> list offsets
     uint32_t* dfill;   // [n]
     uint32_t* solve;   // the solve path's body state
+    uint4* lanes;      // the record chain's lane slots (over the front part)
 };
 
-__device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint32_t tb)
+// front part of a world's fused LDS, in words
+__host__ __device__ constexpr uint32_t fused_front_words(uint32_t n, uint32_t tb) { return 6u * n + 2u * tb + 3u; }
+
+// The front part is dead during the solve's chains (the item counts per body
+// are read before them), so the record chain's lane slots (lane_words, at
+// offset 0) overlay it; the solve's part starts after the larger of the two.
+__device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint32_t tb, uint32_t lane_words)
 {
     FusedLds L;
     uint32_t* p = lds;
+    L.lanes = reinterpret_cast<uint4*>(lds);
     L.coff = p; p += n + 1;
     L.cc = (float*)p; p += n;
     L.cs = (float*)p; p += n;
@@ -2453,7 +2620,7 @@ __device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint3
     L.dcnt = p; p += n;
     L.doff = p; p += n + 1;
     L.dfill = p; p += n;
-    L.solve = p;
+    L.solve = lds + umax(fused_front_words(n, tb), lane_words);
     return L;
 }
 
@@ -2462,9 +2629,9 @@ __device__ __forceinline__ FusedLds fused_carve(uint32_t* lds, uint32_t n, uint3
 // the bucket arrays until the solve
 __host__ __device__ constexpr uint32_t fused_solve_words(bool slots) { return slots ? 13u : 7u; }
 
-size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots)
+size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots, uint32_t threads)
 {
-    const size_t front = 6u * (size_t)n + 2u * (size_t)tb + 3u;
+    const size_t front = std::max<size_t>(fused_front_words(n, tb), (size_t)threads * LANE_WORDS);
     return (front + (size_t)fused_solve_words(slots) * n) * 4u;
 }
 
@@ -2544,7 +2711,8 @@ __device__ __forceinline__ void fused_overflow(const StepDev& S, uint32_t w, uin
 template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
 __device__ __forceinline__ void fused_solve(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
                                          uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
-                                         uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint32_t* s_abort)
+                                         uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint4* lanes,
+                                         uint32_t* s_abort)
 {
     if constexpr (NSLOT > 0) {
         if (nitems <= (uint32_t)NSLOT * MAXT) {
@@ -2553,7 +2721,7 @@ __device__ __forceinline__ void fused_solve(const StepDev& S, const WorldDev& D,
             return;
         }
     }
-    rec_world(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, *s_abort);
+    rec_world<true>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, lanes, *s_abort);
 }
 
 // One workgroup per world (S.w_order), up to nsteps steps per world counted
@@ -2564,7 +2732,7 @@ template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t nsteps)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_sm[16];
     __shared__ uint32_t s_err, s_abort, s_slot;
     const uint32_t w = S.w_order[blockIdx.x];
@@ -2573,7 +2741,7 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
     const uint32_t n = (uint32_t)D.nbodies, tb = D.table, base = D.body_base, nj = (uint32_t)D.njoints;
     const uint32_t t = threadIdx.x;
     constexpr uint32_t T = MAXT;
-    const FusedLds L = fused_carve(smem, n, tb);
+    const FusedLds L = fused_carve(smem, n, tb, T * LANE_WORDS);
     uint32_t* cnt = S.w_cnt + 8 * (size_t)w;
     uint64_t* ts = S.w_tstamp ? S.w_tstamp + (size_t)w * (S.tcap + 1) : nullptr;
     uint32_t step = S.w_done[w];
@@ -2793,6 +2961,7 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         mark(FP_SCHED);
         // ---- solve
         fused_solve<NSLOT, MAXT, WAVES, LOCKSTEP>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
+                                                  L.lanes,
                                                   &s_abort);
         __syncthreads();
         mark(FP_SOLVE);
@@ -2882,6 +3051,12 @@ bool fused_slots(int v)
     return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].slot_fn != nullptr;
 }
 
+uint32_t fused_threads(int v)
+{
+    const SolveVariant& sv = kSolveVariants[v];
+    return sv.slot_fn ? (uint32_t)sv.slot_threads : (uint32_t)sv.threads;
+}
+
 bool fused_available(int v)
 {
     return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].fused_fn != nullptr;
@@ -2896,7 +3071,7 @@ void launch_fused(const StepDev& S, int v, uint32_t nworlds, float dt, uint32_t 
                   uint32_t nsteps, size_t lds, hipStream_t st)
 {
     const SolveVariant& sv = kSolveVariants[v];
-    const unsigned threads = sv.slot_fn ? sv.slot_threads : sv.threads;
+    const unsigned threads = fused_threads(v);
     hipLaunchKernelGGL(sv.fused_fn, dim3(nworlds), dim3(threads), lds, st, S, dt, sub_steps, iters, nsteps);
 }
 

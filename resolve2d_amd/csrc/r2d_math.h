@@ -49,18 +49,10 @@ R2D_HD v2 rotcs(v2 a, float c, float s) { return mk(a.x * c - a.y * s, a.x * s +
 
 // @max / @min: LLVM maxnum/minnum; NaN-suppressing; equal operands return the first.
 R2D_HD bool isnanf_(float a) { return a != a; }
-R2D_HD float maxn(float a, float b)
-{
-    if (isnanf_(a)) return b;
-    if (isnanf_(b)) return a;
-    return (b > a) ? b : a;
-}
-R2D_HD float minn(float a, float b)
-{
-    if (isnanf_(a)) return b;
-    if (isnanf_(b)) return a;
-    return (b < a) ? b : a;
-}
+// One select each: a NaN gives b, a NaN b fails the comparison and gives a,
+// and equal operands give a.
+R2D_HD float maxn(float a, float b) { return (b > a || isnanf_(a)) ? b : a; }
+R2D_HD float minn(float a, float b) { return (b < a || isnanf_(a)) ? b : a; }
 // std.math.clamp
 R2D_HD float clampn(float v, float lo, float hi) { return maxn(lo, minn(v, hi)); }
 R2D_HD float absf(float a) { return __builtin_fabsf(a); }
