@@ -1003,10 +1003,11 @@ constexpr uint32_t NONE = 0xffffffffu;
 
 // words: 0 flags, 1 body a (manifold ref / joint r0), 2 body b (inc / r1),
 // 3 index (manifold in the step arrays / world-local joint), 4-5 ranks at the
-// ordering bodies, 6-8 normal + friction, 9-15 point 0 (r1x r1y r2x r2y depth
-// mass_n mass_t), 16-22 point 1, 24-27 impulses (pn0 pt0 pn1 pt1), 28 the
-// ordering bodies' item counts, 29-30 the points' Baumgarte biases, 31 the
-// compact chain indices (worlds beyond LDS)
+// ordering bodies, 6 the ordering bodies' item counts, 7 the compact chain
+// indices (worlds beyond LDS); then the six rows a chain link reads: 8-10
+// normal + friction, 11-17 point 0 (r1x r1y r2x r2y depth mass_n mass_t),
+// 18-24 point 1, 25-26 the points' Baumgarte biases, 28-31 impulses (pn0 pt0
+// pn1 pt1)
 struct Rec {
     uint32_t flags, a, b, idx, rk0, rk1;
     float nx, ny, fric;
@@ -1018,25 +1019,25 @@ struct Rec {
 };
 constexpr uint32_t DEGS_WIDE = 0xffffffffu;
 
+constexpr uint32_t REC_LINK = 2;    // first of the six rows a chain link reads
+constexpr uint32_t REC_ACC = 7;     // the impulses' row
+
 __device__ __forceinline__ void rec_load(Rec& r, const uint4* q)
 {
-    uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3], v4 = q[4], v5 = q[5], v6 = q[6];
+    const uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3], v4 = q[4], v5 = q[5], v6 = q[6], v7 = q[7];
     r.flags = v0.x; r.a = v0.y; r.b = v0.z; r.idx = v0.w;
-    r.rk0 = v1.x; r.rk1 = v1.y;
-    r.nx = __uint_as_float(v1.z); r.ny = __uint_as_float(v1.w);
-    r.fric = __uint_as_float(v2.x);
-    r.p[0][0] = __uint_as_float(v2.y); r.p[0][1] = __uint_as_float(v2.z); r.p[0][2] = __uint_as_float(v2.w);
-    r.p[0][3] = __uint_as_float(v3.x); r.p[0][4] = __uint_as_float(v3.y); r.p[0][5] = __uint_as_float(v3.z);
-    r.p[0][6] = __uint_as_float(v3.w);
-    r.p[1][0] = __uint_as_float(v4.x); r.p[1][1] = __uint_as_float(v4.y); r.p[1][2] = __uint_as_float(v4.z);
-    r.p[1][3] = __uint_as_float(v4.w); r.p[1][4] = __uint_as_float(v5.x); r.p[1][5] = __uint_as_float(v5.y);
-    r.p[1][6] = __uint_as_float(v5.z);
-    r.acc[0] = __uint_as_float(v6.x); r.acc[1] = __uint_as_float(v6.y);
-    r.acc[2] = __uint_as_float(v6.z); r.acc[3] = __uint_as_float(v6.w);
-    uint4 v7 = q[7];
-    r.degs = v7.x;
-    r.bias[0] = __uint_as_float(v7.y); r.bias[1] = __uint_as_float(v7.z);
-    r.cpair = v7.w;
+    r.rk0 = v1.x; r.rk1 = v1.y; r.degs = v1.z; r.cpair = v1.w;
+    r.nx = __uint_as_float(v2.x); r.ny = __uint_as_float(v2.y); r.fric = __uint_as_float(v2.z);
+    r.p[0][0] = __uint_as_float(v2.w);
+    r.p[0][1] = __uint_as_float(v3.x); r.p[0][2] = __uint_as_float(v3.y); r.p[0][3] = __uint_as_float(v3.z);
+    r.p[0][4] = __uint_as_float(v3.w);
+    r.p[0][5] = __uint_as_float(v4.x); r.p[0][6] = __uint_as_float(v4.y); r.p[1][0] = __uint_as_float(v4.z);
+    r.p[1][1] = __uint_as_float(v4.w);
+    r.p[1][2] = __uint_as_float(v5.x); r.p[1][3] = __uint_as_float(v5.y); r.p[1][4] = __uint_as_float(v5.z);
+    r.p[1][5] = __uint_as_float(v5.w);
+    r.p[1][6] = __uint_as_float(v6.x); r.bias[0] = __uint_as_float(v6.y); r.bias[1] = __uint_as_float(v6.z);
+    r.acc[0] = __uint_as_float(v7.x); r.acc[1] = __uint_as_float(v7.y);
+    r.acc[2] = __uint_as_float(v7.z); r.acc[3] = __uint_as_float(v7.w);
 }
 
 // a record read by the compact chain: its bodies as compact indices
@@ -1051,17 +1052,18 @@ __device__ __forceinline__ void rec_to_compact(Rec& r)
 __device__ __forceinline__ void rec_store(const Rec& r, uint4* q)
 {
     q[0] = make_uint4(r.flags, r.a, r.b, r.idx);
-    q[1] = make_uint4(r.rk0, r.rk1, __float_as_uint(r.nx), __float_as_uint(r.ny));
-    q[2] = make_uint4(__float_as_uint(r.fric), __float_as_uint(r.p[0][0]), __float_as_uint(r.p[0][1]),
-                      __float_as_uint(r.p[0][2]));
-    q[3] = make_uint4(__float_as_uint(r.p[0][3]), __float_as_uint(r.p[0][4]), __float_as_uint(r.p[0][5]),
-                      __float_as_uint(r.p[0][6]));
-    q[4] = make_uint4(__float_as_uint(r.p[1][0]), __float_as_uint(r.p[1][1]), __float_as_uint(r.p[1][2]),
-                      __float_as_uint(r.p[1][3]));
-    q[5] = make_uint4(__float_as_uint(r.p[1][4]), __float_as_uint(r.p[1][5]), __float_as_uint(r.p[1][6]), 0u);
-    q[6] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
+    q[1] = make_uint4(r.rk0, r.rk1, r.degs, r.cpair);
+    q[2] = make_uint4(__float_as_uint(r.nx), __float_as_uint(r.ny), __float_as_uint(r.fric),
+                      __float_as_uint(r.p[0][0]));
+    q[3] = make_uint4(__float_as_uint(r.p[0][1]), __float_as_uint(r.p[0][2]), __float_as_uint(r.p[0][3]),
+                      __float_as_uint(r.p[0][4]));
+    q[4] = make_uint4(__float_as_uint(r.p[0][5]), __float_as_uint(r.p[0][6]), __float_as_uint(r.p[1][0]),
+                      __float_as_uint(r.p[1][1]));
+    q[5] = make_uint4(__float_as_uint(r.p[1][2]), __float_as_uint(r.p[1][3]), __float_as_uint(r.p[1][4]),
+                      __float_as_uint(r.p[1][5]));
+    q[6] = make_uint4(__float_as_uint(r.p[1][6]), __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), 0u);
+    q[7] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
                       __float_as_uint(r.acc[3]));
-    q[7] = make_uint4(r.degs, __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), r.cpair);
 }
 
 __device__ __forceinline__ int32_t rec_r0(const Rec& r)
@@ -1720,10 +1722,10 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
                     rec_solve_joint(cur, st, ua, ub, sub_dt);
                 } else {
                     contact_impulses(cur, cur.a, cur.b, b1, b2, st);
-                    recs[(size_t)cur_pos * 8 + 6] = make_uint4(__float_as_uint(cur.acc[0]),
-                                                               __float_as_uint(cur.acc[1]),
-                                                               __float_as_uint(cur.acc[2]),
-                                                               __float_as_uint(cur.acc[3]));
+                    recs[(size_t)cur_pos * 8 + REC_ACC] = make_uint4(__float_as_uint(cur.acc[0]),
+                                                                     __float_as_uint(cur.acc[1]),
+                                                                     __float_as_uint(cur.acc[2]),
+                                                                     __float_as_uint(cur.acc[3]));
                 }
                 if (LDS) {
                     // the state writes above reach LDS before the counters (in order)
@@ -1759,27 +1761,12 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
 // issued, so program order is all the dataflow needs.
 __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 
-// A lane's slot: the fields of its current record that a chain link reads
-// besides the header (flags, bodies), as six 16-byte rows in LDS:
+// A lane's slot: the six rows of its current record that a chain link reads
+// besides the header (rows REC_LINK.., copied as they are):
 // (nx ny fric p00) (p01 p02 p03 p04) (p05 p06 p10 p11) (p12 p13 p14 p15)
 // (p16 bias0 bias1 -) (acc0 acc1 acc2 acc3).
-constexpr uint32_t LANE_ROWS = 6;
+constexpr uint32_t LANE_ROWS = 8 - REC_LINK;
 constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
-
-__device__ __forceinline__ void lane_store(uint4* q, const Rec& r)
-{
-    q[0] = make_uint4(__float_as_uint(r.nx), __float_as_uint(r.ny), __float_as_uint(r.fric),
-                      __float_as_uint(r.p[0][0]));
-    q[1] = make_uint4(__float_as_uint(r.p[0][1]), __float_as_uint(r.p[0][2]), __float_as_uint(r.p[0][3]),
-                      __float_as_uint(r.p[0][4]));
-    q[2] = make_uint4(__float_as_uint(r.p[0][5]), __float_as_uint(r.p[0][6]), __float_as_uint(r.p[1][0]),
-                      __float_as_uint(r.p[1][1]));
-    q[3] = make_uint4(__float_as_uint(r.p[1][2]), __float_as_uint(r.p[1][3]), __float_as_uint(r.p[1][4]),
-                      __float_as_uint(r.p[1][5]));
-    q[4] = make_uint4(__float_as_uint(r.p[1][6]), __float_as_uint(r.bias[0]), __float_as_uint(r.bias[1]), 0u);
-    q[5] = make_uint4(__float_as_uint(r.acc[0]), __float_as_uint(r.acc[1]), __float_as_uint(r.acc[2]),
-                      __float_as_uint(r.acc[3]));
-}
 
 __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 {
@@ -1801,10 +1788,10 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 // lane's current record kept in its LDS slot (lanes: blockDim.x slots of
 // LANE_WORDS, 16-byte aligned) instead of registers: the loop carries only
 // the header (flags, bodies, ranks, targets), so an iteration moves no
-// record between registers, and a poll reads both counters and both bodies'
-// state in one batch of LDS reads.  The record is read from the slot once the
-// item is ready; a lane whose next position is its current one (one item per
-// iteration) keeps the slot, whose impulses are the ones just written.
+// record between registers, and a poll reads just the two counters.  The
+// slot and the bodies' state are read in one batch once the item is ready; a
+// lane whose next position is its current one (one item per iteration) keeps
+// the slot, whose impulses are the ones just written.
 template <class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
@@ -1816,17 +1803,19 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
     uint32_t held = NONE;                    // position whose record the slot holds
     uint32_t flags = 0, a = 0, b = 0, rk0 = 0, rk1 = 0, degs = 0;
     int32_t r0 = -1, r1 = -1;
-    uint32_t t0 = 0, t1 = 0;
+    uint32_t t0 = 0, t1 = 0, n0 = 0, n1 = 0;  // the counters' targets and released values
+    uint32_t *p0 = st.cnt, *p1 = st.cnt;     // the counters polled (body 0's when none)
     bool have = false;
     uint32_t spins = 0;
     while (true) {
         if (!have) {
             if (k >= iters) break;
             if (pos != held) {
-                Rec r;
-                rec_load(r, recs + (size_t)pos * 8);
-                lane_store(slot, r);
-                flags = r.flags; a = r.a; b = r.b; rk0 = r.rk0; rk1 = r.rk1; degs = r.degs;
+                const uint4* q = recs + (size_t)pos * 8;
+                const uint4 h0 = q[0], h1 = q[1];
+#pragma unroll
+                for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row] = q[REC_LINK + row];
+                flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
                 held = pos;
             }
             if (flags & RF_JOINT) {
@@ -1843,40 +1832,43 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 t0 = r0 >= 0 ? k * st.deg[r0] + rk0 : 0u;
                 t1 = r1 >= 0 ? k * st.deg[r1] + rk1 : 0u;
             }
+            n0 = t0 + 1u;
+            n1 = t1 + 1u;
+            p0 = st.cnt + (r0 >= 0 ? r0 : 0);
+            p1 = st.cnt + (r1 >= 0 ? r1 : 0);
             have = true;
         }
-        // one batch of LDS reads: both counters (a lane without an ordering
-        // body reads body 0's and ignores it) and, for a contact, both bodies'
-        // state (LDS serves one workgroup's operations in order, so state read
-        // with a counter that shows the item ready is what its predecessor
-        // released; values read for an item that is not ready are discarded)
+        // a poll reads the two counters only (a lane without an ordering
+        // body reads body 0's and ignores it); the bodies' state is read once
+        // the item is ready, in the same batch as the slot (LDS serves one
+        // workgroup's operations in order, so it is what the predecessors
+        // released)
         const bool contact = !(flags & RF_JOINT);
-        const uint32_t q0 = r0 >= 0 ? (uint32_t)r0 : 0u, q1 = r1 >= 0 ? (uint32_t)r1 : 0u;
-        const uint32_t c0 = __hip_atomic_load(&st.cnt[q0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t c1 = __hip_atomic_load(&st.cnt[q1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        lds_order();
-        const BodyState b1 = body_state(st, contact ? a : 0u);
-        const BodyState b2 = body_state(st, contact ? b : 0u);
+        const uint32_t c0 = __hip_atomic_load(p0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t c1 = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const bool ready = (r0 < 0 || c0 == t0) && (r1 < 0 || c1 == t1);
         if (ready) {
             // a wave with a ready link issues ahead of the waves that poll
             __builtin_amdgcn_s_setprio(2);
+            lds_order();
             Rec cur;
             cur.flags = flags; cur.a = a; cur.b = b;
             lane_load(cur, slot);
+            const BodyState b1 = body_state(st, contact ? a : 0u);
+            const BodyState b2 = body_state(st, contact ? b : 0u);
             if (!contact) {
                 rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
             } else {
                 contact_impulses(cur, a, b, b1, b2, st);
                 const uint4 acc = make_uint4(__float_as_uint(cur.acc[0]), __float_as_uint(cur.acc[1]),
                                              __float_as_uint(cur.acc[2]), __float_as_uint(cur.acc[3]));
-                recs[(size_t)pos * 8 + 6] = acc;
+                recs[(size_t)pos * 8 + REC_ACC] = acc;
                 slot[5] = acc;
             }
             // the state writes above reach LDS before the counters (in order)
             lds_order();
-            if (r0 >= 0) __hip_atomic_store(&st.cnt[r0], t0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (r1 >= 0) __hip_atomic_store(&st.cnt[r1], t1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (r0 >= 0) __hip_atomic_store(p0, n0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (r1 >= 0) __hip_atomic_store(p1, n1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_s_setprio(0);
             pos += p_step;
             if (pos >= nitems) { pos = p_begin; ++k; }
@@ -1895,8 +1887,9 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 }
             }
         }
-        // a wave with no lane ready yields its SIMD
-        if (__ballot(ready) == 0ull) __builtin_amdgcn_s_sleep(1);
+        // no sleep between polls: a sleeping wave notices a released
+        // counter later, and the ready waves of other worlds on the SIMD
+        // issue ahead of it anyway
     }
 }
 
@@ -2092,7 +2085,7 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
             c0 = S.cmap[base + h.y];
             c1 = S.cmap[base + h.z];
         }
-        q[7].w = c0 | (c1 << 16);
+        q[1].w = c0 | (c1 << 16);
     }
 }
 
