@@ -47,7 +47,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 # fused kernel shape of each solve variant: <slots, workgroup size, waves per SIMD>
 FUSED_SHAPES = {1: "4,256,3", 2: "2,512,3", 3: "3,256,3", 4: "4,256,2", 5: "8,128,2", 7: "4,256,3",
-                9: "0,256,3", 10: "0,256,4", 11: "0,192,3"}
+                9: "0,256,3", 10: "0,256,4", 11: "0,192,3", 12: "0,128,2"}
 
 
 def traffic_for(kernel, variant, args):

@@ -232,9 +232,9 @@ int r2d_ctx_fused_counts(r2d_ctx* ctx, uint64_t* out);
  * counterpart in the reference (Solver.process is one world, one step). */
 int r2d_ctx_set_fused(r2d_ctx* ctx, int on);
 /* Launch configuration of the solve kernel: -1 (default) is variant 1;
- * 0..11 select the register-slot kernel's slots x workgroup size, the record
+ * 0..12 select the register-slot kernel's slots x workgroup size, the record
  * kernel's workgroup size, and the fused kernel's shape (table in
- * r2d_kernels.hip; 9-11 solve with records only).
+ * r2d_kernels.hip; 9-12 solve with records only).
  * Every variant gives the same bits; the autotuner (resolve2d_amd/autotune.py)
  * picks the fastest for a batch shape. */
 int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);

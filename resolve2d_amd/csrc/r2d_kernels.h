@@ -24,7 +24,7 @@ enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
 // solve-kernel variants (register-slot kernel shape + record-kernel workgroup
 // size; table in r2d_kernels.hip)
-constexpr int SOLVE_VARIANTS = 12;
+constexpr int SOLVE_VARIANTS = 13;
 unsigned solve_variant_threads(int v);
 int solve_variant_slots(int v);
 

@@ -1767,6 +1767,13 @@ __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 // (p16 bias0 bias1 -) (acc0 acc1 acc2 acc3).
 constexpr uint32_t LANE_ROWS = 8 - REC_LINK;
 constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
+// the record kernel takes the lane slots up to this workgroup size (larger
+// ones keep the chain's records in registers)
+constexpr uint32_t LANES_MAXT = 256;
+__host__ __device__ constexpr size_t rec_lane_bytes(uint32_t threads)
+{
+    return threads <= LANES_MAXT ? (size_t)threads * LANE_WORDS * 4u : 0u;
+}
 
 __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 {
@@ -1968,8 +1975,13 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const WorldDev& D = S.worlds[w];
     if (solve_route((uint32_t)D.nbodies, nitems, slot_cap, slot_nmax, lds_nmax) != 1) return;
-    rec_world<false>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, smem,
-                     nullptr, s_abort);
+    // LDS: the chain's lane slots (workgroups of at most LANES_MAXT threads),
+    // then the body state
+    constexpr bool LANES = MAXT <= LANES_MAXT;
+    uint4* lanes = LANES ? reinterpret_cast<uint4*>(smem) : nullptr;
+    uint32_t* body = smem + (LANES ? MAXT * LANE_WORDS : 0u);
+    rec_world<LANES>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, body,
+                     lanes, s_abort);
 }
 
 // ------------------------------------- worlds beyond LDS: split sub-steps
@@ -3037,6 +3049,9 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     // records only, 192-thread workgroups: with the records' smaller LDS
     // state, four worlds of 500 bodies are resident per CU
     {nullptr, 0, 0, k_solve<192, 3>, 192, k_fused<0, 192, 3, false>},
+    // records only, 128-thread workgroups at 2 waves per SIMD: the same four
+    // worlds per CU with the full register file (no spills)
+    {nullptr, 0, 0, k_solve<128, 2>, 128, k_fused<0, 128, 2, false>},
 };
 
 bool fused_slots(int v)
@@ -3138,7 +3153,7 @@ SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t l
     const SolveVariant& v = kSolveVariants[p.variant];
     p.slot_cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
     p.slot_nmax = v.slot_fn ? SLOT_NMAX : 0u;
-    p.lds_nmax = (uint32_t)((160u * 1024u - 1024u) / solve_lds_bytes(1));
+    p.lds_nmax = (uint32_t)((160u * 1024u - 1024u - rec_lane_bytes(v.threads)) / solve_lds_bytes(1));
     if (lds_limit) p.lds_nmax = std::min(p.lds_nmax, lds_limit);
     for (size_t i = 0; i < nw; ++i) {
         uint32_t n = nbodies[i];
@@ -3257,7 +3272,8 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             tmark(ss, "k_big (global state)", sb);
         }
         if (run_lds) {
-            hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads), solve_lds_bytes((int)sp.lds_maxn),
+            hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads),
+                               solve_lds_bytes((int)sp.lds_maxn) + rec_lane_bytes(v.threads),
                                ss.aux[0], S, dt, sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
             tmark(ss, "k_solve (record)", ss.aux[0]);
         }

@@ -91,7 +91,7 @@ def test_batched_equals_single(gpu_ctx, oracle):
 
 
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "phases"])
-@pytest.mark.parametrize("variant", list(range(12)))
+@pytest.mark.parametrize("variant", list(range(13)))
 def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, fused):
     """Every launch configuration of the solve kernel (workgroup size per
     world), in the fused kernel where the variant has one and in the phase
