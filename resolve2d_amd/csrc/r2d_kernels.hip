@@ -1761,11 +1761,11 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
 // issued, so program order is all the dataflow needs.
 __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 
-// A lane's slot: the six rows of its current record that a chain link reads
-// besides the header (rows REC_LINK.., copied as they are):
+// A lane's slot: the five rows of its current record that a chain link reads
+// besides the header and the impulses (rows REC_LINK.., copied as they are):
 // (nx ny fric p00) (p01 p02 p03 p04) (p05 p06 p10 p11) (p12 p13 p14 p15)
-// (p16 bias0 bias1 -) (acc0 acc1 acc2 acc3).
-constexpr uint32_t LANE_ROWS = 8 - REC_LINK;
+// (p16 bias0 bias1 -).
+constexpr uint32_t LANE_ROWS = REC_ACC - REC_LINK;
 constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
 // the record kernel takes the lane slots up to this workgroup size (larger
 // ones keep the chain's records in registers)
@@ -1775,9 +1775,16 @@ __host__ __device__ constexpr size_t rec_lane_bytes(uint32_t threads)
     return threads <= LANES_MAXT ? (size_t)threads * LANE_WORDS * 4u : 0u;
 }
 
+// a contact's impulses back to its record
+__device__ __forceinline__ void rec_put_acc(uint4* recs, uint32_t pos, const float* acc)
+{
+    recs[(size_t)pos * 8 + REC_ACC] = make_uint4(__float_as_uint(acc[0]), __float_as_uint(acc[1]),
+                                                 __float_as_uint(acc[2]), __float_as_uint(acc[3]));
+}
+
 __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 {
-    const uint4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+    const uint4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
     r.nx = __uint_as_float(a.x); r.ny = __uint_as_float(a.y); r.fric = __uint_as_float(a.z);
     r.p[0][0] = __uint_as_float(a.w);
     r.p[0][1] = __uint_as_float(b.x); r.p[0][2] = __uint_as_float(b.y); r.p[0][3] = __uint_as_float(b.z);
@@ -1787,18 +1794,20 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
     r.p[1][2] = __uint_as_float(d.x); r.p[1][3] = __uint_as_float(d.y); r.p[1][4] = __uint_as_float(d.z);
     r.p[1][5] = __uint_as_float(d.w);
     r.p[1][6] = __uint_as_float(e.x); r.bias[0] = __uint_as_float(e.y); r.bias[1] = __uint_as_float(e.z);
-    r.acc[0] = __uint_as_float(f.x); r.acc[1] = __uint_as_float(f.y); r.acc[2] = __uint_as_float(f.z);
-    r.acc[3] = __uint_as_float(f.w);
 }
 
 // The same ordered dataflow as item_chain (body state in LDS), with a
 // lane's current record kept in its LDS slot (lanes: blockDim.x slots of
 // LANE_WORDS, 16-byte aligned) instead of registers: the loop carries only
-// the header (flags, bodies, ranks, targets), so an iteration moves no
-// record between registers, and a poll reads just the two counters.  The
-// slot and the bodies' state are read in one batch once the item is ready; a
-// lane whose next position is its current one (one item per iteration) keeps
-// the slot, whose impulses are the ones just written.
+// the header (flags, bodies, ranks, targets) and the impulses, so an
+// iteration moves no record between registers, and a poll reads just the two
+// counters.  The slot and the bodies' state are read in one batch once the
+// item is ready.  A link issues no global memory operation: a contact's
+// impulses go back to its record when the lane moves to another position
+// and when the chain ends (a wave's memory counter is waited on as a whole,
+// so a store per link would make every later wait of the wave wait for it);
+// a lane whose next position is its current one (one item per iteration)
+// keeps slot and impulses.
 template <class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
@@ -1812,17 +1821,21 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
     int32_t r0 = -1, r1 = -1;
     uint32_t t0 = 0, t1 = 0, n0 = 0, n1 = 0;  // the counters' targets and released values
     uint32_t *p0 = st.cnt, *p1 = st.cnt;     // the counters polled (body 0's when none)
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};     // the held record's impulses
     bool have = false;
     uint32_t spins = 0;
     while (true) {
         if (!have) {
             if (k >= iters) break;
             if (pos != held) {
+                if (held != NONE && !(flags & RF_JOINT)) rec_put_acc(recs, held, acc);
                 const uint4* q = recs + (size_t)pos * 8;
-                const uint4 h0 = q[0], h1 = q[1];
+                const uint4 h0 = q[0], h1 = q[1], ac = q[REC_ACC];
 #pragma unroll
                 for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row] = q[REC_LINK + row];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
+                acc[0] = __uint_as_float(ac.x); acc[1] = __uint_as_float(ac.y);
+                acc[2] = __uint_as_float(ac.z); acc[3] = __uint_as_float(ac.w);
                 held = pos;
             }
             if (flags & RF_JOINT) {
@@ -1861,16 +1874,16 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             Rec cur;
             cur.flags = flags; cur.a = a; cur.b = b;
             lane_load(cur, slot);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) cur.acc[f] = acc[f];
             const BodyState b1 = body_state(st, contact ? a : 0u);
             const BodyState b2 = body_state(st, contact ? b : 0u);
             if (!contact) {
                 rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
             } else {
                 contact_impulses(cur, a, b, b1, b2, st);
-                const uint4 acc = make_uint4(__float_as_uint(cur.acc[0]), __float_as_uint(cur.acc[1]),
-                                             __float_as_uint(cur.acc[2]), __float_as_uint(cur.acc[3]));
-                recs[(size_t)pos * 8 + REC_ACC] = acc;
-                slot[5] = acc;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) acc[f] = cur.acc[f];
             }
             // the state writes above reach LDS before the counters (in order)
             lds_order();
@@ -1898,6 +1911,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
         // counter later, and the ready waves of other worlds on the SIMD
         // issue ahead of it anyway
     }
+    if (held != NONE && !(flags & RF_JOINT)) rec_put_acc(recs, held, acc);
 }
 
 // the records' impulses, effective masses, lever arms and depths -> the
