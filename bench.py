@@ -40,7 +40,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # LDS: 256 CUs x 128 B per clock x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
-TUNE_AT, TUNE_STEPS = 60, 30   # autotuner: samples of 30 steps from step 60 of a copy of the batch
+# autotuner: samples of 30 steps from step 60 of a copy of the batch; the 3
+# fastest timed again over 120 steps from the same state
+TUNE_AT, TUNE_STEPS, TUNE_REFINE, TUNE_REFINE_STEPS = 60, 30, 3, 120
 G = 9.81
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -415,7 +417,8 @@ def main():
 
         cands = autotune.candidates(lds_limits=(0,))
         tuned = autotune.tune(ctx, None, dt, S, K, rounds=1, cands=cands, cache_path=None,
-                              steps_per_sample=TUNE_STEPS, make_batch=fresh_copy)
+                              steps_per_sample=TUNE_STEPS, make_batch=fresh_copy, refine=TUNE_REFINE,
+                              refine_steps=TUNE_REFINE_STEPS)
         log(f"autotune: variant {tuned['variant']} fused {tuned['fused']} ({tuned['step_ms']:.3f} ms per step)")
     ctx.sync()
 
@@ -580,7 +583,9 @@ def main():
                    "schedule": "reference order (bit-exact)", "parallelism": "1 GPU",
                    "solve_launch": ({"autotuned": True, "variant": tuned["variant"], "fused": tuned["fused"],
                                      "tuned_at_step": TUNE_AT, "sample_steps": TUNE_STEPS,
-                                     "candidates_step_ms (variant/lds_limit/fused)": tuned["all_ms"]}
+                                     "candidates_step_ms (variant/lds_limit/fused)": tuned["all_ms"],
+                                     "refine_steps": TUNE_REFINE_STEPS,
+                                     "refined_step_ms": tuned.get("refined_ms")}
                                     if tuned else {"autotuned": False, "variant": args.variant})},
         "p50_step_ms": float(np.percentile(lat_ms, 50)) if len(lat_ms) else None,
         "p99_step_ms": float(np.percentile(lat_ms, 99)) if len(lat_ms) else None,

@@ -117,15 +117,20 @@ def pick(times: Dict[tuple, list]) -> tuple:
 
 def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: int = 2,
          cands=None, cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True,
-         steps_per_sample: int = 30, make_batch: Optional[Callable[[], list]] = None) -> dict:
+         steps_per_sample: int = 30, make_batch: Optional[Callable[[], list]] = None,
+         refine: int = 0, refine_steps: int = 120) -> dict:
     """Pick the launch shape for stepping `handles` together and apply it to
     the context.  Spends len(cands) * rounds * steps_per_sample real steps of
     the batch (untimed by the caller's measure; the results are the same as
     with any other shape), each sample one call of steps_per_sample steps.
     With make_batch (returns fresh worlds in the same state each time, with
     handle and close()), every sample starts from that same state instead of
-    where the previous sample left the batch.  Returns {"variant",
-    "lds_limit", "fused", "step_ms", "key", "cached", "all_ms"}."""
+    where the previous sample left the batch.  refine > 1: the `refine`
+    fastest candidates are timed again on one call of refine_steps steps
+    (a short sample can rank shapes differently from a long run, whose later
+    steps hold more contacts), and that time decides.  Returns {"variant",
+    "lds_limit", "fused", "step_ms", "key", "cached", "all_ms"} (plus
+    "refined_ms" after a refinement)."""
     probe = None
     if handles is None:
         probe = make_batch()
@@ -141,27 +146,39 @@ def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: 
     cands = cands or candidates()
     times = {c: [] for c in cands}
     was = ctx.profiling
+
+    def sample(c, steps):
+        ctx.set_solve_variant(c[0])
+        ctx.set_lds_limit(c[1])
+        ctx.set_fused(c[2] if len(c) > 2 else True)
+        fresh = make_batch() if make_batch else None
+        hs = [w.handle for w in fresh] if fresh else handles
+        ctx.reset_phase_times()
+        ctx.process_batch_handles(hs, dt, sub_steps, iters, steps)
+        t = step_ms(ctx, steps)
+        for w in fresh or ():
+            w.close()
+        return t
+
     ctx.set_profiling(True)
+    refined = {}
     try:
         for _ in range(rounds):
             for c in cands:
-                ctx.set_solve_variant(c[0])
-                ctx.set_lds_limit(c[1])
-                ctx.set_fused(c[2] if len(c) > 2 else True)
-                fresh = make_batch() if make_batch else None
-                hs = [w.handle for w in fresh] if fresh else handles
-                ctx.reset_phase_times()
-                ctx.process_batch_handles(hs, dt, sub_steps, iters, steps_per_sample)
-                times[c].append(step_ms(ctx, steps_per_sample))
-                for w in fresh or ():
-                    w.close()
+                times[c].append(sample(c, steps_per_sample))
+        if refine > 1:
+            ranked = sorted(times, key=lambda c: float(np.median(times[c])))[:refine]
+            refined = {c: [sample(c, refine_steps)] for c in ranked}
     finally:
         ctx.set_profiling(was)
-    best = pick(times)
+    best = pick(refined) if refined else pick(times)
     v, lim = best[0], best[1]
     fused = bool(best[2]) if len(best) > 2 else True
-    choice = {"variant": v, "lds_limit": lim, "fused": fused, "step_ms": float(np.median(times[best])),
+    choice = {"variant": v, "lds_limit": lim, "fused": fused,
+              "step_ms": float(np.median((refined or times)[best])),
               "all_ms": {"/".join(str(int(x)) for x in c): float(np.median(t)) for c, t in times.items()}}
+    if refined:
+        choice["refined_ms"] = {"/".join(str(int(x)) for x in c): float(t[0]) for c, t in refined.items()}
     apply(ctx, choice)
     if cache_path:
         cache = load_cache(cache_path)

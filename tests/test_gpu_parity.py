@@ -124,7 +124,8 @@ def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, fused):
 
 def test_autotune_picks_and_caches(gpu_ctx, oracle, tmp_path):
     """The tuner steps the batch with each candidate (results unchanged: the
-    batch still equals the oracle) and reuses its choice for the same shape."""
+    batch still equals the oracle), re-times its two fastest on longer calls,
+    and reuses its choice for the same shape."""
     import os
     from resolve2d_amd import autotune, scenes
     from resolve2d_amd.engine import GpuWorld, apply_scene
@@ -136,11 +137,13 @@ def test_autotune_picks_and_caches(gpu_ctx, oracle, tmp_path):
         apply_scene(sc, o)
     path = os.path.join(str(tmp_path), "tune.json")
     cands = [(1, 0, True), (6, 0, False), (7, 1024, False)]   # fused; phase kernels
-    per = 2                                 # steps per timed sample
+    per, refine, longer = 2, 2, 3           # steps per timed sample; refinement
     try:
         r = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, rounds=1, cands=cands, cache_path=path,
-                          steps_per_sample=per)
+                          steps_per_sample=per, refine=refine, refine_steps=longer)
         assert (r["variant"], r["lds_limit"], r["fused"]) in cands and not r["cached"]
+        assert len(r["refined_ms"]) == refine and r["step_ms"] == r["refined_ms"]["/".join(
+            str(int(x)) for x in (r["variant"], r["lds_limit"], r["fused"]))]
         r2 = autotune.tune(gpu_ctx, [g.handle for g in gw], scenes.DT, 4, 4, cands=cands, cache_path=path)
         assert r2["cached"] and r2["variant"] == r["variant"]
     finally:
@@ -149,7 +152,7 @@ def test_autotune_picks_and_caches(gpu_ctx, oracle, tmp_path):
         gpu_ctx.set_fused(True)
     assert r["step_ms"] > 0.0 and all(v > 0.0 for v in r["all_ms"].values())
     for o in ow:
-        for _ in range(len(cands) * per):
+        for _ in range(len(cands) * per + refine * longer):
             assert o.process(scenes.DT, 4, 4) == 0
     for g, o in zip(gw, ow):
         assert bits_equal(g.bodies(), o.bodies())
