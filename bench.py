@@ -188,6 +188,27 @@ def accuracy(ctx, specs, sample, steps, snaps):
     return res, bit_equal, pen
 
 
+def cpu_single(ops, steps, budget_s):
+    """The f32 oracle (the reference's algorithm restated in C, single
+    threaded like the reference) stepping one world on one host thread, from
+    the start, for `steps` steps or until budget_s has passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import apply_scene
+    o = pyoracle.OracleWorld("f32")
+    apply_scene(ops, o)
+    n = o.bodies().shape[0]
+    done, t0 = 0, time.perf_counter()
+    while done < steps and time.perf_counter() - t0 < budget_s:
+        o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+        done += 1
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": n * done / t, "unit": "body-steps/s", "ms_per_step": t / done * 1e3, "cores": 1,
+            "kind": "port", "sample": f"oracle f32 restatement, 1 thread, the first {done} of the {steps} steps"}
+
+
 def pyramid_energy(ctx, steps):
     """Energy drift of the resting 20-row pyramid (configuration 1) over the
     horizon: GPU f32 against the f64 oracle."""
@@ -200,16 +221,19 @@ def pyramid_energy(ctx, steps):
     apply_scene(scenes.pyramid(20), g)
     apply_scene(scenes.pyramid(20), o)
     e0 = energy(g.bodies())
-    worst = 0.0
+    worst, gpu_s = 0.0, 0.0
     for s in range(1, steps + 1):
+        a = time.perf_counter()
         checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "pyramid step")
+        gpu_s += time.perf_counter() - a
         o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
         if s % 20 == 0 or s == steps:
             worst = max(worst, abs(energy(g.bodies()) - e0) / abs(e0))
     b, r = g.bodies(), o.bodies()
     out = dict(steps=steps, drift=(energy(b) - e0) / abs(e0), max_abs_drift=worst,
                drift_f64=(energy(r) - e0) / abs(e0), max_penetration_m=max_penetration(g.manifolds()),
-               vs_f64=state_error(b, r))
+               vs_f64=state_error(b, r), gpu_ms_per_step=gpu_s / steps * 1e3,
+               cpu_single=cpu_single(scenes.pyramid(20), steps, 5.0))
     g.close()
     o.close()
     return out
@@ -252,6 +276,7 @@ def large_world(ctx, steps, snaps, with_f64):
                 errs[f"step_{s}"] = state_error(shots[s], o.bodies())
         o.close()
         out["vs_f64_oracle"] = errs
+    out["cpu_single"] = cpu_single(ops, steps, 8.0)
     return out
 
 
@@ -528,6 +553,8 @@ def main():
                              "p50_step_ms": float(np.percentile(np.array(lat1) * 1e3, 50)),
                              "p99_step_ms": float(np.percentile(np.array(lat1) * 1e3, 99))}
             w.close()
+        single["cpu_single"] = cpu_single(scenes.mixed(4096, seed=1, hash_cell=2.0 if args.tuned_hash else None),
+                                          args.single_steps, 6.0)
 
     large = None
     if not args.no_large:
