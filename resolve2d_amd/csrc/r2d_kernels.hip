@@ -1767,6 +1767,7 @@ __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 // (p16 bias0 bias1 -).
 constexpr uint32_t LANE_ROWS = REC_ACC - REC_LINK;
 constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
+static_assert(REC_LINK == 2 && REC_ACC == 7 && LANE_ROWS == 5, "lane_load reads rows REC_LINK..REC_ACC-1 in this order");
 // the record kernel takes the lane slots up to this workgroup size (larger
 // ones keep the chain's records in registers)
 constexpr uint32_t LANES_MAXT = 256;
