@@ -1808,8 +1808,9 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 // and when the chain ends (a wave's memory counter is waited on as a whole,
 // so a store per link would make every later wait of the wave wait for it);
 // a lane whose next position is its current one (one item per iteration)
-// keeps slot and impulses.
-template <class ST>
+// keeps slot and impulses.  COMPACT: body state under the compact chain
+// indices of worlds beyond LDS (record row 1 .w), as in rec_to_compact.
+template <bool COMPACT = false, class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                                  uint32_t& s_abort)
@@ -1835,6 +1836,11 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
 #pragma unroll
                 for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row] = q[REC_LINK + row];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
+                if (COMPACT) {
+                    const uint32_t c0 = h1.w & 0xffffu, c1 = h1.w >> 16;
+                    a = c0;
+                    b = ((flags & RF_JOINT) && b == NONE) ? NONE : c1;
+                }
                 acc[0] = __uint_as_float(ac.x); acc[1] = __uint_as_float(ac.y);
                 acc[2] = __uint_as_float(ac.z); acc[3] = __uint_as_float(ac.w);
                 held = pos;
@@ -2140,9 +2146,12 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
     const uint32_t base = D.body_base;
     uint4* recs = S.recs + (size_t)ibase * 8;
     const uint32_t nc = S.w_nc[w];
+    // the lane-slot chain when the slots fit beside the compact state
+    constexpr size_t LANE_BYTES = (size_t)MAXT * LANE_WORDS * 4u;
+    const bool lanes = (size_t)nc * COMPACT_BYTES + LANE_BYTES <= COMPACT_LDS;
     if ((size_t)nc * COMPACT_BYTES <= COMPACT_LDS) {
         SolverState<true> st;
-        st.mx = (float*)smem;
+        st.mx = (float*)(smem + (lanes ? MAXT * LANE_WORDS : 0u));
         st.my = st.mx + nc;
         st.L = st.my + nc;
         st.im = st.L + nc;
@@ -2157,7 +2166,8 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
             st.deg[c] = S.deg[base + i];
         }
         __syncthreads();
-        item_chain<true, true>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        if (lanes) item_chain_lanes<true>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w, s_abort);
+        else item_chain<true, true>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
             const uint32_t i = S.cinv[base + c];
