@@ -1810,7 +1810,10 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 // a lane whose next position is its current one (one item per iteration)
 // keeps slot and impulses.  COMPACT: body state under the compact chain
 // indices of worlds beyond LDS (record row 1 .w), as in rec_to_compact.
-template <bool COMPACT = false, class ST>
+// GLOBAL: body state and counters in global memory (worlds whose touched
+// bodies do not fit LDS): counters are polled with acquire loads and
+// released with release stores, and the state is read after the acquire.
+template <bool COMPACT = false, bool GLOBAL = false, class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                                  uint32_t& s_abort)
@@ -1871,9 +1874,15 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
         // workgroup's operations in order, so it is what the predecessors
         // released)
         const bool contact = !(flags & RF_JOINT);
-        const uint32_t c0 = __hip_atomic_load(p0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t c1 = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const bool ready = (r0 < 0 || c0 == t0) && (r1 < 0 || c1 == t1);
+        bool ready;
+        if (GLOBAL) {
+            ready = r0 < 0 || ld_acq(p0) == t0;
+            if (ready && r1 >= 0) ready = ld_acq(p1) == t1;
+        } else {
+            const uint32_t c0 = __hip_atomic_load(p0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t c1 = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ready = (r0 < 0 || c0 == t0) && (r1 < 0 || c1 == t1);
+        }
         if (ready) {
             // a wave with a ready link issues ahead of the waves that poll
             __builtin_amdgcn_s_setprio(2);
@@ -1883,8 +1892,8 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             lane_load(cur, slot);
 #pragma unroll
             for (int f = 0; f < 4; ++f) cur.acc[f] = acc[f];
-            const BodyState b1 = body_state(st, contact ? a : 0u);
-            const BodyState b2 = body_state(st, contact ? b : 0u);
+            const BodyState b1 = GLOBAL ? body_state_ordered(st, contact ? a : 0u) : body_state(st, contact ? a : 0u);
+            const BodyState b2 = GLOBAL ? body_state_ordered(st, contact ? b : 0u) : body_state(st, contact ? b : 0u);
             if (!contact) {
                 rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
             } else {
@@ -1894,8 +1903,13 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             }
             // the state writes above reach LDS before the counters (in order)
             lds_order();
-            if (r0 >= 0) __hip_atomic_store(p0, n0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (r1 >= 0) __hip_atomic_store(p1, n1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (GLOBAL) {
+                if (r0 >= 0) st_rel(p0, n0);
+                if (r1 >= 0) st_rel(p1, n1);
+            } else {
+                if (r0 >= 0) __hip_atomic_store(p0, n0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (r1 >= 0) __hip_atomic_store(p1, n1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
             __builtin_amdgcn_s_setprio(0);
             pos += p_step;
             if (pos >= nitems) { pos = p_begin; ++k; }
@@ -2174,8 +2188,11 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
             D.mx[i] = st.mx[c]; D.my[i] = st.my[c]; D.L[i] = st.L[c];
         }
     } else {
+        // body state and counters in global memory; the lane slots take the
+        // start of LDS, which this path does not otherwise use
         const SolverState<false> st = global_state(S, D);
-        item_chain<false, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        item_chain_lanes<false, true>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+                                      s_abort);
     }
 }
 
