@@ -197,8 +197,24 @@ int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t cap, size_t*
 int r2d_set_hash_params(r2d_world* w, int mode, float cell_width, size_t table_mult);
 /* Solver schedule.  0 (default): the reference's sequential Gauss-Seidel
  * order, reproduced exactly.  1: graph-coloured batches (same equations,
- * another order; results agree within tolerance, not bit for bit). */
+ * another order; results agree within tolerance, not bit for bit).
+ * Schedule 1 is refused (R2D_E_INVALID_ARG) while warm starting is on. */
 int r2d_set_schedule(r2d_world* w, int mode);
+/* Warm starting (opt-in; the roadmap's "persist manifolds across frames by
+ * keying with stable body IDs", README.md:60-61 -- the reference rebuilds
+ * every manifold with zero impulses, lib.zig:261 and collision.zig:38-51).
+ * factor 0 (default): off, the reference's results.  factor in (0, 1]:
+ * after each step the world keeps every manifold (body ids, points, normal,
+ * accumulated impulses) under the lower slot of its two bodies, at most 8
+ * per body in discovery order.  A manifold of the next step with the same
+ * (reference id, incident id), the same point slots, a normal within
+ * cos = 0.99 of the kept one and no point that updateTGSDepth moves starts
+ * every sub-step with factor * impulse / sub_steps per point: added to its
+ * accumulators before the iterations and applied to its bodies with the
+ * forces (each body's share summed in item order).  Results then differ
+ * from the reference's by design.  Removing a body, clearing the world or
+ * calling this function empties the kept manifolds.  Needs schedule 0. */
+int r2d_set_warm_start(r2d_world* w, float factor);
 
 /* --------------------------------------------------------- profiling */
 /* When on, every step records HIP events around its phases on the

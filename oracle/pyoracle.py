@@ -48,6 +48,7 @@ def load():
             "set_trace": (None, [P, I]), "num_trace": (I64, [P]), "set_schedule": (None, [P, I]),
             "get_trace": (None, [P, ctypes.POINTER(U32)]),
             "narrow_pair": (I, [P, U32, U32, ctypes.POINTER(D)]),
+            "set_warm": (None, [P, D]), "num_warm": (I, [P]), "set_warm_factor": (None, [P, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, pre + name)
@@ -121,6 +122,18 @@ class OracleWorld:
 
     def set_schedule(self, mode):
         self._f("set_schedule")(self.h, mode)
+
+    def set_warm_start(self, factor):
+        """Warm starting (an extension, not the reference's): 0 off."""
+        self._f("set_warm")(self.h, factor)
+
+    def set_warm_factor(self, factor):
+        """A new factor, the kept manifolds left in place (tests only)."""
+        self._f("set_warm_factor")(self.h, factor)
+
+    def num_warm(self):
+        """Manifolds the last step warm started."""
+        return self._f("num_warm")(self.h)
 
     def process(self, dt, sub_steps, iters):
         return self._f("process_")(self.h, dt, sub_steps, iters)

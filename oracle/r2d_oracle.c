@@ -102,6 +102,7 @@ EXPORT float or_cosf(float x) { return r2dref_cosf(x); }
         IP##world *w = (IP##world *)vw;                                                                   \
         for (uint32_t i = 0; i < w->capid; ++i) w->id2slot[i] = -1;                                     \
         w->nbodies = 0; w->ncons = 0; w->ngens = 0; w->nmans = 0; w->nexcl = 0;                         \
+        if (w->wcnt) memset(w->wcnt, 0, sizeof(int32_t) * (size_t)w->wcap);                             \
     }                                                                                                   \
     EXPORT uint32_t EP##make_disc_(void *w, double px, double py, double vx, double vy, double ang,     \
                                   double om, double mu, int mk, double mv, double r)                    \
@@ -208,6 +209,16 @@ EXPORT float or_cosf(float x) { return r2dref_cosf(x); }
     }                                                                                                   \
     EXPORT void EP##set_trace(void *w, int on) { ((IP##world *)w)->trace_on = on; }                       \
     EXPORT void EP##set_schedule(void *w, int mode) { ((IP##world *)w)->sched = mode; }                   \
+    /* warm starting (extension): factor, kept manifolds emptied */                                     \
+    EXPORT void EP##set_warm(void *vw, double f)                                                         \
+    {                                                                                                   \
+        IP##world *w = (IP##world *)vw;                                                                   \
+        w->warm = (R)f;                                                                                 \
+        if (w->wcnt) memset(w->wcnt, 0, sizeof(int32_t) * (size_t)w->wcap);                             \
+    }                                                                                                   \
+    EXPORT int EP##num_warm(void *w) { return ((IP##world *)w)->nwarm; }                                   \
+    /* tests: a new factor with the kept manifolds left in place */                                     \
+    EXPORT void EP##set_warm_factor(void *w, double f) { ((IP##world *)w)->warm = (R)f; }                 \
     EXPORT int64_t EP##num_trace(void *w) { return ((IP##world *)w)->ntrace / 2; }                        \
     /* (id_a, id_b) pairs that reached SAT, in visiting order */                                        \
     EXPORT void EP##get_trace(void *vw, uint32_t *o)                                                     \

@@ -149,7 +149,20 @@ typedef struct {
     F(vec2) app_lin1, app_lin2;
     R2D_REAL app_rot1, app_rot2;
     R2D_REAL friction;
+    /* warm starting (an extension of this build): matched last step's manifold,
+     * impulse per sub-step carried per point (pn0, pt0, pn1, pt1) */
+    int32_t warm;
+    R2D_REAL wg[4];
 } F(manifold);
+
+/* warm starting: a kept manifold (body ids, point mask, normal, impulses) */
+#define WARM_K_ 8
+typedef struct {
+    uint32_t ida, idb;
+    int32_t mask;
+    F(vec2) n;
+    R2D_REAL acc[4];
+} F(wentry);
 
 typedef struct {
     uint32_t current_body_id;
@@ -165,6 +178,11 @@ typedef struct {
     /* solver schedule: 0 reference order; 1 graph-coloured (an extension of
      * this build, not of the reference: items run colour by colour) */
     int32_t sched;
+    /* warm starting (an extension of this build, off when warm == 0): kept
+     * manifolds under the lower slot of their bodies, WARM_K_ per slot */
+    R2D_REAL warm;
+    F(wentry) *wtab; int32_t *wcnt; int32_t wcap;
+    int32_t nwarm;                              /* manifolds matched by the last step */
     /* broadphase trace for tests: candidate pairs visited in discovery order */
     int32_t trace_on;
     int32_t *trace; int64_t ntrace, captrace;
@@ -476,6 +494,7 @@ static void F(world_free)(F(world) *w)
 {
     if (!w) return;
     free(w->bodies); free(w->id2slot); free(w->cons); free(w->gens); free(w->excl); free(w->mans); free(w->trace);
+    free(w->wtab); free(w->wcnt);
     free(w);
 }
 
@@ -556,6 +575,7 @@ static int F(remove_body)(F(world) *w, uint32_t id)
     }
     w->id2slot[id] = -1;
     w->nbodies -= 1;
+    if (w->wcnt) memset(w->wcnt, 0, sizeof(int32_t) * (size_t)w->wcap);   /* tables are kept under slots */
     return 0;
 }
 
@@ -765,7 +785,7 @@ static void F(calc_impulses)(F(manifold) *m, F(body) *b1, F(body) *b2, R2D_REAL 
         R2D_REAL bias = K_BAUMGARTE * F(maxn)(0, (-p->depth) - K_BAUMGARTE_SLOP) / dt;
         R2D_REAL num = F(dot2)(dv, m->normal) + bias;
         R2D_REAL pn = num * p->mass_n;
-        if (pn < K_MIN_MANIFOLD_IMPULSE) continue;
+        if (pn < K_MIN_MANIFOLD_IMPULSE && !m->warm) continue;
         num = F(dot2)(dv, m->tangent);
         R2D_REAL pt = num * p->mass_t;
         R2D_REAL new_pn = F(maxn)(0, p->acc_pn + pn);
@@ -995,6 +1015,69 @@ static void F(color_items)(F(world) *w, int32_t *perm)
 
 /* Solver.process (lib.zig:189-251).  Returns 0, or -2 for
  * error.InvalidRigidBodyId (state is left as the reference leaves it). */
+/* ------------------------------------------------------ warm starting
+ * Not in the reference (its roadmap: "persist manifolds across frames by
+ * keying with stable body IDs", README.md:60-61).  The rule, step by step:
+ * match (after updateManifolds), per-body sums in manifold order, and at
+ * every sub-step the sums with the forces and the carried impulses into
+ * the accumulators before the iterations; after the step the manifolds are
+ * kept for the next one. */
+static void F(warm_grow)(F(world) *w)
+{
+    if (w->nbodies <= w->wcap) return;
+    int32_t nc = w->wcap ? w->wcap : 16;
+    while (nc < w->nbodies) nc *= 2;
+    w->wtab = (F(wentry) *)realloc(w->wtab, sizeof(F(wentry)) * (size_t)nc * WARM_K_);
+    w->wcnt = (int32_t *)realloc(w->wcnt, sizeof(int32_t) * (size_t)nc);
+    for (int32_t i = w->wcap; i < nc; ++i) w->wcnt[i] = 0;
+    w->wcap = nc;
+}
+
+static int32_t F(point_mask)(const F(manifold) *M) { return (M->pts[0].valid ? 1 : 0) | (M->pts[1].valid ? 2 : 0); }
+
+static void F(warm_match)(F(world) *w, F(manifold) *M, R2D_REAL f_sub)
+{
+    M->warm = 0;
+    for (int i = 0; i < 4; ++i) M->wg[i] = 0;
+    int live = 0;
+    for (int p = 0; p < 2; ++p)
+        if (M->pts[p].valid) live |= !F(approx)(M->pts[p].depth, M->pts[p].orig_depth, (R2D_REAL)1e-4f);
+    if (live) return;
+    int32_t lo = M->ref < M->inc ? M->ref : M->inc;
+    int32_t c = w->wcnt[lo] < WARM_K_ ? w->wcnt[lo] : WARM_K_;
+    uint32_t ida = w->bodies[M->ref].id, idb = w->bodies[M->inc].id;
+    for (int32_t k = 0; k < c; ++k) {
+        const F(wentry) *e = &w->wtab[(size_t)lo * WARM_K_ + k];
+        if (e->ida != ida || e->idb != idb) continue;
+        R2D_REAL d = F(dot2)(M->normal, e->n);
+        if (e->mask == F(point_mask)(M) && d >= (R2D_REAL)0.99f) {
+            M->warm = 1;
+            for (int p = 0; p < 2; ++p) {
+                if (!M->pts[p].valid || !(M->pts[p].depth < 0)) continue;
+                M->wg[2 * p] = (w->warm * e->acc[2 * p]) / f_sub;
+                M->wg[2 * p + 1] = (w->warm * e->acc[2 * p + 1]) / f_sub;
+            }
+        }
+        break;
+    }
+}
+
+static void F(warm_save)(F(world) *w)
+{
+    for (int32_t b = 0; b < w->nbodies; ++b) w->wcnt[b] = 0;
+    for (int32_t m = 0; m < w->nmans; ++m) {
+        const F(manifold) *M = &w->mans[m];
+        int32_t lo = M->ref < M->inc ? M->ref : M->inc;
+        if (w->wcnt[lo] >= WARM_K_) continue;
+        F(wentry) *e = &w->wtab[(size_t)lo * WARM_K_ + w->wcnt[lo]++];
+        e->ida = w->bodies[M->ref].id;
+        e->idb = w->bodies[M->inc].id;
+        e->mask = F(point_mask)(M);
+        e->n = M->normal;
+        for (int p = 0; p < 2; ++p) { e->acc[2 * p] = M->pts[p].acc_pn; e->acc[2 * p + 1] = M->pts[p].acc_pt; }
+    }
+}
+
 static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t iters)
 {
     R2D_REAL f_sub = (R2D_REAL)(float)sub_steps;
@@ -1012,6 +1095,30 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
         perm = (int32_t *)malloc(sizeof(int32_t) * (size_t)(w->ncons + w->nmans + 1));
         F(color_items)(w, perm);
     }
+    /* warm starting: matches, then each body's carried impulse per sub-step,
+     * summed in manifold order as calculateImpulses applies impulses */
+    R2D_REAL *wsum = NULL;
+    const int warm = w->warm > 0;
+    if (warm) {
+        F(warm_grow)(w);
+        wsum = (R2D_REAL *)calloc((size_t)w->nbodies * 3 + 1, sizeof(R2D_REAL));
+        w->nwarm = 0;
+        for (int32_t m = 0; m < w->nmans; ++m) {
+            F(manifold) *M = &w->mans[m];
+            F(warm_match)(w, M, f_sub);
+            if (!M->warm) continue;
+            w->nwarm += 1;
+            F(vec2) t = F(rot90cw)(M->normal);
+            int s1 = w->bodies[M->ref].is_static, s2 = w->bodies[M->inc].is_static;
+            for (int p = 0; p < 2; ++p) {
+                if (!M->pts[p].valid) continue;
+                F(vec2) dp = F(add2)(F(scale2)(M->normal, M->wg[2 * p]), F(scale2)(t, M->wg[2 * p + 1]));
+                R2D_REAL *a = wsum + 3 * (size_t)M->ref, *b = wsum + 3 * (size_t)M->inc;
+                if (!s1) { a[0] -= dp.x; a[1] -= dp.y; a[2] -= F(cross2)(M->pts[p].ref_r, dp); }
+                if (!s2) { b[0] += dp.x; b[1] += dp.y; b[2] += F(cross2)(M->pts[p].inc_r, dp); }
+            }
+        }
+    }
     for (uint64_t s = 0; s < sub_steps; ++s) {
         for (int32_t gi = 0; gi < w->ngens; ++gi)
             for (int32_t b = 0; b < w->nbodies; ++b) F(apply_gen)(&w->gens[gi], &w->bodies[b]);
@@ -1022,11 +1129,22 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
             B->mom.x += B->force.x * sub_dt;
             B->mom.y += B->force.y * sub_dt;
             B->L += B->torque * sub_dt;
+            if (warm) {
+                B->mom.x += wsum[3 * (size_t)b];
+                B->mom.y += wsum[3 * (size_t)b + 1];
+                B->L += wsum[3 * (size_t)b + 2];
+            }
         }
         for (int32_t m = 0; m < w->nmans; ++m) {
             F(manifold) *M = &w->mans[m];
             F(update_tgs_depth)(M, &w->bodies[M->ref], &w->bodies[M->inc]);
             F(pre_step)(M, &w->bodies[M->ref], &w->bodies[M->inc]);
+            if (M->warm)
+                for (int p = 0; p < 2; ++p) {
+                    if (!M->pts[p].valid) continue;
+                    M->pts[p].acc_pn += M->wg[2 * p];
+                    M->pts[p].acc_pt += M->wg[2 * p + 1];
+                }
         }
         for (uint64_t k = 0; k < iters; ++k) {
             if (perm) {
@@ -1043,7 +1161,7 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
             }
             for (int32_t c = 0; c < w->ncons; ++c) {
                 int rc = F(solve_constraint)(w, &w->cons[c], sub_dt);
-                if (rc) return rc;
+                if (rc) { free(wsum); return rc; }
             }
             for (int32_t m = 0; m < w->nmans; ++m) {
                 F(manifold) *M = &w->mans[m];
@@ -1061,6 +1179,8 @@ static int F(process)(F(world) *w, R2D_REAL dt, uint64_t sub_steps, uint64_t ite
             B->torque = 0;
         }
     }
+    if (warm) F(warm_save)(w);
+    free(wsum);
     free(perm);
     return 0;
 }

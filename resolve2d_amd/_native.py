@@ -132,6 +132,7 @@ SIGNATURES = {
     "r2d_get_manifolds": [P, ctypes.POINTER(ManifoldState), SZ, SZP],
     "r2d_set_hash_params": [P, I32, F, SZ],
     "r2d_set_schedule": [P, I32],
+    "r2d_set_warm_start": [P, F],
     "r2d_ctx_set_profiling": [P, I32],
     "r2d_ctx_phase_times": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
     "r2d_ctx_reset_phase_times": [P],

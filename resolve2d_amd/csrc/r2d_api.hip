@@ -86,6 +86,8 @@ struct r2d_world {
     float hash_cell = 4.0f;
     size_t hash_mult = 2;
     int sched = 0;
+    float warm = 0.0f;        // warm starting: fraction of the previous step's impulses carried (0: off)
+    bool warm_clear = true;   // the warm tables are to be emptied before the next step
     // device
     uint32_t* dblock = nullptr;
     size_t dcap = 0;
@@ -95,6 +97,10 @@ struct r2d_world {
     size_t dgcap = 0;
     uint64_t* dexcl = nullptr;
     size_t dxcap = 0;
+    uint4* dwtab = nullptr;   // warm tables (WARM_K entries of 3 uint4 per body), counts, per-body sums
+    uint32_t* dwcnt = nullptr;
+    float4* dwsum = nullptr;
+    size_t dwcap = 0;
     bool host_valid = true;   // host mirror is current
     bool dev_valid = false;   // device body block is current
     bool topo_dirty = true;   // joints / generators / exclusions need upload
@@ -203,6 +209,39 @@ int ensure_world_device(r2d_world* w)
         w->dev_valid = false;
         w->version += 1;
     }
+    return R2D_OK;
+}
+
+// warm-start tables of a warm-started world, as large as its device body
+// block; a larger table keeps the old one's entries
+int ensure_warm(r2d_world* w)
+{
+    if (!(w->warm > 0.0f)) return R2D_OK;
+    hipStream_t st = w->ctx->stream;
+    if (w->dwcap < w->dcap) {
+        const size_t c = w->dcap;
+        uint4* tab = nullptr;
+        uint32_t* cnt = nullptr;
+        float4* sum = nullptr;
+        if (hipMalloc(&tab, c * WARM_K * 3 * sizeof(uint4)) != hipSuccess) return R2D_E_OOM;
+        if (hipMalloc(&cnt, c * 4) != hipSuccess) { (void)hipFree(tab); return R2D_E_OOM; }
+        if (hipMalloc(&sum, c * sizeof(float4)) != hipSuccess) { (void)hipFree(tab); (void)hipFree(cnt); return R2D_E_OOM; }
+        HIPCHK(hipMemsetAsync(cnt, 0, c * 4, st));
+        if (w->dwcap && !w->warm_clear) {
+            HIPCHK(hipMemcpyAsync(tab, w->dwtab, w->dwcap * WARM_K * 3 * sizeof(uint4), hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(cnt, w->dwcnt, w->dwcap * 4, hipMemcpyDeviceToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        if (w->dwtab) (void)hipFree(w->dwtab);
+        if (w->dwcnt) (void)hipFree(w->dwcnt);
+        if (w->dwsum) (void)hipFree(w->dwsum);
+        w->dwtab = tab; w->dwcnt = cnt; w->dwsum = sum;
+        w->dwcap = c;
+        w->version += 1;
+    } else if (w->warm_clear) {
+        HIPCHK(hipMemsetAsync(w->dwcnt, 0, w->dwcap * 4, st));
+    }
+    w->warm_clear = false;
     return R2D_OK;
 }
 
@@ -356,6 +395,10 @@ WorldDev make_desc(r2d_world* w)
     d.cell = hash_cell(w);
     d.table = table_size(w);
     d.sched = w->sched;
+    if (w->warm > 0.0f) {
+        d.warm = w->warm;
+        d.wtab = w->dwtab; d.wcnt = w->dwcnt; d.wsum = w->dwsum;
+    }
     return d;
 }
 
@@ -741,6 +784,7 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         int rc;
         if (!w->dev_valid && (rc = upload_bodies(w))) return rc;
         if ((rc = upload_topology(w))) return rc;
+        if ((rc = ensure_warm(w))) return rc;
         if (w->desc_version != w->version) {
             w->desc = make_desc(w);
             w->desc_version = w->version;
@@ -766,9 +810,14 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
         int v = std::atoi(e);
         if (v >= 0 && v < SOLVE_VARIANTS) variant = v;
     }
-    bool colored = false;
-    for (size_t i = 0; i < nw; ++i) colored = colored || worlds[i]->sched != 0;
-    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored);
+    bool colored = false, warm = false;
+    for (size_t i = 0; i < nw; ++i) {
+        colored = colored || worlds[i]->sched != 0;
+        warm = warm || worlds[i]->warm > 0.0f;
+    }
+    // warm starting is compiled into one variant's kernels only
+    if (warm) variant = WARM_VARIANT;
+    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored, warm);
     b.variant = variant;
     // whole steps per workgroup: worlds of at most FUSED_NMAX bodies in the
     // reference order whose broadphase and solve arrays fit LDS
@@ -1453,6 +1502,9 @@ R2D_EXPORT int r2d_world_destroy(r2d_world* w)
     if (w->djoints) (void)hipFree(w->djoints);
     if (w->dgens) (void)hipFree(w->dgens);
     if (w->dexcl) (void)hipFree(w->dexcl);
+    if (w->dwtab) (void)hipFree(w->dwtab);
+    if (w->dwcnt) (void)hipFree(w->dwcnt);
+    if (w->dwsum) (void)hipFree(w->dwsum);
     delete w;
     return R2D_OK;
 }
@@ -1464,6 +1516,7 @@ R2D_EXPORT int r2d_world_clear(r2d_world* w)
     if (rc) return rc;
     std::fill(w->id2slot.begin(), w->id2slot.end(), -1);
     w->n = 0;
+    w->warm_clear = true;
     w->joints.clear();
     w->gens.clear();
     w->excl.clear();
@@ -1559,6 +1612,7 @@ R2D_EXPORT int r2d_remove_body(r2d_world* w, uint32_t id)
     }
     w->id2slot[id] = -1;
     w->n -= 1;
+    w->warm_clear = true;   // the warm tables are kept under slots
     w->topo_dirty = true;   // joint / spring slots move
     w->version += 1;
     w->epoch = 0;
@@ -1796,7 +1850,19 @@ R2D_EXPORT int r2d_set_schedule(r2d_world* w, int mode)
 {
     if (!w || (mode != 0 && mode != 1)) return R2D_E_INVALID_ARG;
     if (w->in_service) return R2D_E_BUSY;
+    if (mode == 1 && w->warm > 0.0f) return R2D_E_INVALID_ARG;   // warm starting runs in the reference order
     w->sched = mode;
+    w->version += 1;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_set_warm_start(r2d_world* w, float factor)
+{
+    if (!w || !(factor >= 0.0f && factor <= 1.0f)) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
+    if (factor > 0.0f && w->sched != 0) return R2D_E_INVALID_ARG;
+    w->warm = factor;
+    w->warm_clear = true;
     w->version += 1;
     return R2D_OK;
 }

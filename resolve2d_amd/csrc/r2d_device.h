@@ -53,7 +53,22 @@ struct WorldDev {
     uint32_t table;                  // spatial-hash table size (buckets)
     uint32_t body_base, bucket_base, joint_base;
     int32_t sched;                   // 0 reference order, 1 coloured
+    // warm starting (opt-in; warm == 0: the reference's cold manifolds)
+    float warm;                      // fraction of last step's impulses carried over, (0, 1]
+    uint4* wtab;                     // [WARM_K x 3 per body] last step's manifolds under their lower slot
+    uint32_t* wcnt;                  // [per body] entries of wtab in use
+    float4* wsum;                    // [per body] warm impulse per sub-step: momentum x, y, angular
 };
+
+// Warm starting keeps, after each step, every manifold's ids, point mask,
+// normal and accumulated impulses under the lower slot of its two bodies (at
+// most WARM_K per body, in discovery order).  A manifold of the next step
+// with the same (ref id, inc id), the same points and a normal within
+// WARM_DOT of the old one starts each sub-step with warm * acc / sub_steps.
+constexpr uint32_t WARM_K = 8;
+constexpr float WARM_DOT = 0.99f;
+// m_valid bit: the manifold matched last step's (warm starting)
+enum : uint32_t { MV_WARM = 4u };
 
 // device counters
 enum : int {
@@ -155,7 +170,9 @@ enum : int { FP_PREP = 0, FP_BUCKETS, FP_CANDS, FP_NARROW, FP_CONTACTS, FP_SCHED
 // manifold point fields: m_pt[(field * 2 + point) * cap_m + m]
 enum : int {
     MP_ACC_PN = 0, MP_ACC_PT, MP_R1X, MP_R1Y, MP_R2X, MP_R2Y, MP_POSX, MP_POSY,
-    MP_DEPTH, MP_ODEPTH, MP_MASS_N, MP_MASS_T, MP_NFIELDS
+    MP_DEPTH, MP_ODEPTH, MP_MASS_N, MP_MASS_T,
+    MP_WARM_N, MP_WARM_T,      // warm starting: impulse per sub-step carried from the previous step
+    MP_NFIELDS
 };
 
 }  // namespace r2d

@@ -25,6 +25,9 @@ enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 // solve-kernel variants (register-slot kernel shape + record-kernel workgroup
 // size; table in r2d_kernels.hip)
 constexpr int SOLVE_VARIANTS = 13;
+// the variant every batch with a warm-started world takes (not tunable)
+constexpr int WARM_VARIANT = SOLVE_VARIANTS;
+constexpr int SOLVE_TABLE = SOLVE_VARIANTS + 1;
 unsigned solve_variant_threads(int v);
 int solve_variant_slots(int v);
 
@@ -41,10 +44,12 @@ struct SolvePlan {
     uint32_t lds_maxn = 0;
     uint32_t any_slot = 0, any_lds = 0, any_big = 0;
     uint32_t any_colored = 0;  // some world uses the coloured schedule
+    uint32_t any_warm = 0;     // some world is warm started
 };
 // lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
 // many as LDS holds); larger worlds use the global-memory record kernel
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored);
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored,
+                     bool any_warm);
 
 // Optional per-kernel timing (a context created with R2D_TRACE_KERNELS set):
 // an event after every launch, summed per kernel name; the steps run eagerly.

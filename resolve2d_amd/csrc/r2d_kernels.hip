@@ -992,12 +992,14 @@ struct SolverState {
 // position, built in parallel at the first sub-step: the ordering bodies and
 // ranks, and for a manifold its normal, friction, both points' lever arms,
 // depths, effective masses (identical every sub-step unless updateTGSDepth
-// moves the point) and accumulated impulses.  A worker reads its next record
-// while it still waits on the current one, so a link of a dependency chain
-// touches only LDS body state.
+// moves the point) and accumulated impulses.  A worker holds a record before
+// the item's dependencies are met (item_chain reads its next record while it
+// still waits on the current one; item_chain_lanes copies it into the lane's
+// LDS slot), so a link of a dependency chain touches only LDS body state.
 enum : uint32_t {
     RF_JOINT = 1u, RF_P0 = 2u, RF_P1 = 4u, RF_W1 = 8u, RF_W2 = 16u, RF_S1 = 32u, RF_S2 = 64u,
-    RF_TGS = 128u            // a point's depth is not its original depth: updateTGSDepth is live
+    RF_TGS = 128u,           // a point's depth is not its original depth: updateTGSDepth is live
+    RF_WARM = 256u           // warm started: each sub-step adds MP_WARM_N / MP_WARM_T to the impulses
 };
 constexpr uint32_t NONE = 0xffffffffu;
 
@@ -1114,7 +1116,7 @@ __device__ __forceinline__ void rec_masses(Rec& r, const ST& st, int p)
 }
 
 // record of the item at execution position `pos` (first sub-step)
-template <class ST>
+template <bool WARM = false, class ST>
 __device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldDev& D, const ST& st, uint32_t pos,
                                           uint32_t ibase, uint32_t nj, uint32_t mbeg, uint32_t base)
 {
@@ -1155,6 +1157,8 @@ __device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldD
     r.ny = S.m_ny[m];
     r.fric = sqrt_rn(D.mu[r.a] * D.mu[r.b]);   // preStep (collision.zig:114)
     uint32_t valid = S.m_valid[m];
+    if constexpr (WARM)
+        if (valid & MV_WARM) r.flags |= RF_WARM;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         if (!(valid & (1u << p))) continue;
@@ -1224,7 +1228,7 @@ __device__ __forceinline__ void rec_impulses(Rec& r, const ST& st, float dt)
         float bias = BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
         float num = dot2(dv, n) + bias;
         float pn = num * r.p[p][5];
-        if (pn < MIN_MANIFOLD_IMPULSE) continue;
+        if (pn < MIN_MANIFOLD_IMPULSE && !(r.flags & RF_WARM)) continue;   // warm started: clamped only
         num = dot2(dv, t);
         float pt = num * r.p[p][6];
         float acc_pn = r.acc[2 * p], acc_pt = r.acc[2 * p + 1];
@@ -1417,7 +1421,7 @@ __device__ __forceinline__ BodyState body_state_ordered(const ST& st, uint32_t i
 // record, with both bodies' state already read (the link's only reads of
 // them) and the Baumgarte bias of the sub-step precomputed
 // ((0.02 max(0, -depth - 0.005)) / dt, collision.zig:172; -1 marks depth >= 0)
-template <class Item, class ST>
+template <bool WARM = false, class Item, class ST>
 __device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t l2, const BodyState& b1,
                                                  const BodyState& b2, const ST& st)
 {
@@ -1431,7 +1435,9 @@ __device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t 
     float rot1 = 0.f, rot2 = 0.f;
     // straight-line code with selects: a point that is absent, separated
     // (bias < 0: its impulses reset) or below the minimum impulse changes
-    // nothing else, exactly as skipping it would
+    // nothing else, exactly as skipping it would.  A warm-started manifold
+    // has no minimum: its accumulated impulse may shrink (clamped at 0), so a
+    // carried impulse that is too large can be taken back.
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         const bool on = (z.flags & (RF_P0 << p)) != 0u;
@@ -1442,7 +1448,7 @@ __device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t 
         v2 dv = sub2(v1, v2v);
         float num = dot2(dv, n) + z.bias[p];
         float pn = num * z.p[p][5];
-        const bool go = on && !sep && !(pn < MIN_MANIFOLD_IMPULSE);
+        const bool go = on && !sep && (!(pn < MIN_MANIFOLD_IMPULSE) || (WARM && (z.flags & RF_WARM) != 0u));
         num = dot2(dv, t);
         float pt = num * z.p[p][6];
         float acc_pn = z.acc[2 * p], acc_pt = z.acc[2 * p + 1];
@@ -1497,9 +1503,9 @@ __host__ __device__ inline size_t slot_lds_bytes(int nbodies) { return (size_t)n
 // global memory.  Every kernel evaluates the same rule, so each world is
 // stepped by exactly one of them.
 __device__ __forceinline__ int solve_route(uint32_t n, uint32_t nitems, uint32_t slot_cap, uint32_t slot_nmax,
-                                           uint32_t lds_nmax)
+                                           uint32_t lds_nmax, bool warm)
 {
-    if (slot_cap && n <= slot_nmax && nitems <= slot_cap) return 0;
+    if (slot_cap && !warm && n <= slot_nmax && nitems <= slot_cap) return 0;
     if (n <= lds_nmax) return 1;
     return 2;
 }
@@ -1524,7 +1530,7 @@ __device__ __forceinline__ void body_setup(const WorldDev& D, const ST& st, int 
 
 // forces (lib.zig:200-205), AABB refresh + velocity integration
 // (lib.zig:207-216) for body i; clears its solver counter
-template <class ST>
+template <bool WARM = false, class ST>
 __device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int i, bool last, float sub_dt)
 {
     uint32_t f = D.flags[i];
@@ -1568,6 +1574,15 @@ __device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int
         st.mx[i] += fx * sub_dt;
         st.my[i] += fy * sub_dt;
         st.L[i] += D.tq[i] * sub_dt;
+        if constexpr (WARM) {
+            if (D.warm > 0.0f) {
+                // warm starting: the body's share of the matched manifolds' carried impulses
+                const float4 ws = D.wsum[i];
+                st.mx[i] += ws.x;
+                st.my[i] += ws.y;
+                st.L[i] += ws.z;
+            }
+        }
     }
     D.fx[i] = fx; D.fy[i] = fy;
     st.cnt[i] = 0u;
@@ -1586,13 +1601,128 @@ __device__ __forceinline__ void body_positions(const WorldDev& D, const ST& st, 
     D.tq[i] = 0.0f;
 }
 
+// ------------------------------------------------------------ warm starting
+// Opt-in, not in the reference (its roadmap item "persist manifolds across
+// frames by keying with stable body IDs", README.md:60-61).  After a step
+// every manifold is kept under the lower slot of its two bodies (WARM_K
+// entries per body, discovery order); at the next step a manifold with the
+// same (ref id, inc id), the same points, a normal within WARM_DOT of the old
+// one and no live updateTGSDepth point carries warm * acc / sub_steps of each
+// point's impulses into every sub-step: added to its accumulators before the
+// iterations, and applied to its bodies with the forces.
+
+// manifold m (step index): match against its lower body's table; the points'
+// impulses per sub-step to MP_WARM_N / MP_WARM_T, MV_WARM on a match
+__device__ __forceinline__ void warm_match(const StepDev& S, const WorldDev& D, uint32_t m, uint32_t base,
+                                           uint32_t sub_steps)
+{
+    const size_t cm = S.cap_m;
+    float* P = S.m_pt;
+    const uint32_t valid = S.m_valid[m] & 3u;
+    const uint32_t a = S.m_ref[m] - base, b = S.m_inc[m] - base;
+    const uint32_t lo = umin(a, b);
+    const v2 n = mk(S.m_nx[m], S.m_ny[m]);
+    bool live = false;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+        if (valid & (1u << p)) live |= !approx(P[(MP_DEPTH * 2 + p) * cm + m], P[(MP_ODEPTH * 2 + p) * cm + m], 1e-4f);
+    float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool hit = false;
+    if (!live) {
+        const uint32_t ida = D.id[a], idb = D.id[b];
+        const uint32_t c = umin(D.wcnt[lo], WARM_K);
+        const uint4* e = D.wtab + (size_t)lo * WARM_K * 3;
+        for (uint32_t k = 0; k < c; ++k, e += 3) {
+            const uint4 h = e[0];
+            if (h.x != ida || h.y != idb) continue;
+            const uint4 q = e[1], r = e[2];
+            const float d = dot2(n, mk(__uint_as_float(h.w), __uint_as_float(q.x)));
+            if (h.z == valid && d >= WARM_DOT) {
+                hit = true;
+                const float acc[4] = {__uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w),
+                                      __uint_as_float(r.x)};
+                const float ss = (float)sub_steps;
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    if (!(valid & (1u << p)) || !(P[(MP_DEPTH * 2 + p) * cm + m] < 0.0f)) continue;
+                    g[2 * p] = (D.warm * acc[2 * p]) / ss;
+                    g[2 * p + 1] = (D.warm * acc[2 * p + 1]) / ss;
+                }
+            }
+            break;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        P[(MP_WARM_N * 2 + p) * cm + m] = g[2 * p];
+        P[(MP_WARM_T * 2 + p) * cm + m] = g[2 * p + 1];
+    }
+    if (hit) S.m_valid[m] = valid | MV_WARM;
+}
+
+// body x (world slot) from its item list (entries item * 2 + side, items
+// counted from ibase, in item order): the sum, in that order, of its matched
+// manifolds' carried impulses, as calculateImpulses applies them
+// (collision.zig:200-217): the reference body gets -dp, -r1 x dp, the
+// incident body +dp, +r2 x dp
+__device__ __forceinline__ void warm_body_sum(const StepDev& S, const WorldDev& D, const uint32_t* list, uint32_t cnt,
+                                              uint32_t x, uint32_t ibase, uint32_t nj, uint32_t mbeg)
+{
+    const size_t cm = S.cap_m;
+    const float* P = S.m_pt;
+    float sx = 0.0f, sy = 0.0f, sl = 0.0f;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        const uint32_t v = list[i], j = (v >> 1) - ibase;
+        if (j < nj) continue;
+        const uint32_t m = mbeg + (j - nj);
+        const uint32_t valid = S.m_valid[m];
+        if (!(valid & MV_WARM)) continue;
+        const v2 n = mk(S.m_nx[m], S.m_ny[m]), t = rot90cw(n);
+        for (int p = 0; p < 2; ++p) {
+            if (!(valid & (1u << p))) continue;
+            const v2 dp = add2(scale2(n, P[(MP_WARM_N * 2 + p) * cm + m]), scale2(t, P[(MP_WARM_T * 2 + p) * cm + m]));
+            if (v & 1u) {
+                const v2 r2 = mk(P[(MP_R2X * 2 + p) * cm + m], P[(MP_R2Y * 2 + p) * cm + m]);
+                sx += dp.x; sy += dp.y; sl += cross2(r2, dp);
+            } else {
+                const v2 r1 = mk(P[(MP_R1X * 2 + p) * cm + m], P[(MP_R1Y * 2 + p) * cm + m]);
+                sx -= dp.x; sy -= dp.y; sl -= cross2(r1, dp);
+            }
+        }
+    }
+    D.wsum[x] = make_float4(sx, sy, sl, 0.0f);
+}
+
+// after the step: manifold m of the world's [mbeg, mend) into its lower
+// body's table at its rank among that body's manifolds (contiguous in
+// discovery order: a pair is found from its lower slot); the last of them
+// sets the count.  The world's counts are zero beforehand.
+__device__ __forceinline__ void warm_save(const StepDev& S, const WorldDev& D, uint32_t m, uint32_t mbeg,
+                                          uint32_t mend, uint32_t base)
+{
+    auto lo_of = [&](uint32_t k) { return umin(S.m_ref[k], S.m_inc[k]) - base; };
+    const uint32_t lo = lo_of(m);
+    uint32_t k = 0;
+    while (k < WARM_K && m - k > mbeg && lo_of(m - k - 1) == lo) ++k;
+    if (m + 1 == mend || lo_of(m + 1) != lo) D.wcnt[lo] = umin(k + 1, WARM_K);
+    if (k >= WARM_K) return;
+    const size_t cm = S.cap_m;
+    const float* P = S.m_pt;
+    uint4* e = D.wtab + ((size_t)lo * WARM_K + k) * 3;
+    const uint32_t a = S.m_ref[m] - base, b = S.m_inc[m] - base;
+    e[0] = make_uint4(D.id[a], D.id[b], S.m_valid[m] & 3u, __float_as_uint(S.m_nx[m]));
+    e[1] = make_uint4(__float_as_uint(S.m_ny[m]), __float_as_uint(P[(MP_ACC_PN * 2 + 0) * cm + m]),
+                      __float_as_uint(P[(MP_ACC_PT * 2 + 0) * cm + m]), __float_as_uint(P[(MP_ACC_PN * 2 + 1) * cm + m]));
+    e[2] = make_uint4(__float_as_uint(P[(MP_ACC_PT * 2 + 1) * cm + m]), 0u, 0u, 0u);
+}
+
 // the record of execution position pos (first sub-step's preStep)
-template <class ST>
+template <bool WARM = false, class ST>
 __device__ __forceinline__ void item_build(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs, uint32_t pos,
                                            uint32_t ibase, uint32_t mbeg, float sub_dt)
 {
     Rec r;
-    rec_build(r, S, D, st, pos, ibase, (uint32_t)D.njoints, mbeg, D.body_base);
+    rec_build<WARM>(r, S, D, st, pos, ibase, (uint32_t)D.njoints, mbeg, D.body_base);
     r.degs = rec_degs(r, st);
     if (r.flags & RF_JOINT) {
         const Joint& J = D.joints[r.idx];
@@ -1606,7 +1736,7 @@ __device__ __forceinline__ void item_build(const StepDev& S, const WorldDev& D, 
 
 // updateTGSDepth (collision.zig:73-100): live only for records flagged at
 // build time; every manifold's previous angles end as the last sub-step's
-template <class ST>
+template <bool WARM = false, class ST>
 __device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs, uint32_t pos,
                                          bool last, float sub_dt)
 {
@@ -1618,6 +1748,21 @@ __device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, co
         rec_joint_pose(r, D.joints[h.w], D);
         rec_store(r, q);
         return;
+    }
+    if (WARM && (h.x & RF_WARM)) {
+        // warm starting: the carried impulse per sub-step joins the accumulators
+        const size_t cm = S.cap_m;
+        const float* P = S.m_pt;
+        uint4 a = q[REC_ACC];
+        if (h.x & RF_P0) {
+            a.x = __float_as_uint(__uint_as_float(a.x) + P[(MP_WARM_N * 2 + 0) * cm + h.w]);
+            a.y = __float_as_uint(__uint_as_float(a.y) + P[(MP_WARM_T * 2 + 0) * cm + h.w]);
+        }
+        if (h.x & RF_P1) {
+            a.z = __float_as_uint(__uint_as_float(a.z) + P[(MP_WARM_N * 2 + 1) * cm + h.w]);
+            a.w = __float_as_uint(__uint_as_float(a.w) + P[(MP_WARM_T * 2 + 1) * cm + h.w]);
+        }
+        q[REC_ACC] = a;
     }
     if (h.x & RF_TGS) {
         Rec r;
@@ -1633,7 +1778,7 @@ __device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, co
 
 // K iterations of one sub-step's (constraints, manifolds) for the world of
 // this workgroup, as an ordered dataflow
-template <bool LDS, bool COMPACT, class ST>
+template <bool LDS, bool COMPACT, bool WARM = false, class ST>
 __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs,
                                            uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                            uint32_t& s_abort)
@@ -1721,7 +1866,7 @@ __device__ __forceinline__ void item_chain(const StepDev& S, const WorldDev& D, 
                     const uint32_t ub = COMPACT ? (cur.cpair >> 16) : (cur.b == NONE ? cur.a : cur.b);
                     rec_solve_joint(cur, st, ua, ub, sub_dt);
                 } else {
-                    contact_impulses(cur, cur.a, cur.b, b1, b2, st);
+                    contact_impulses<WARM>(cur, cur.a, cur.b, b1, b2, st);
                     recs[(size_t)cur_pos * 8 + REC_ACC] = make_uint4(__float_as_uint(cur.acc[0]),
                                                                      __float_as_uint(cur.acc[1]),
                                                                      __float_as_uint(cur.acc[2]),
@@ -1813,7 +1958,7 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 // GLOBAL: body state and counters in global memory (worlds whose touched
 // bodies do not fit LDS): counters are polled with acquire loads and
 // released with release stores, and the state is read after the acquire.
-template <bool COMPACT = false, bool GLOBAL = false, class ST>
+template <bool COMPACT = false, bool GLOBAL = false, bool WARM = false, class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                                  uint32_t& s_abort)
@@ -1897,7 +2042,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             if (!contact) {
                 rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
             } else {
-                contact_impulses(cur, a, b, b1, b2, st);
+                contact_impulses<WARM>(cur, a, b, b1, b2, st);
 #pragma unroll
                 for (int f = 0; f < 4; ++f) acc[f] = cur.acc[f];
             }
@@ -1955,7 +2100,7 @@ __device__ __forceinline__ void item_writeback(const StepDev& S, uint4* recs, ui
 // [ibase, ibase + nitems) of S.recs; degs: the world's item counts per body.
 // LANES: the chain keeps each lane's current record in its slot of `lanes`
 // (LDS, blockDim.x * LANE_WORDS words, 16-byte aligned, disjoint from lds).
-template <bool LANES>
+template <bool LANES, bool WARM = false>
 __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
                                           uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
                                           uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint4* lanes,
@@ -1976,15 +2121,17 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
     st.deg = st.cnt + n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) body_setup(D, st, i, true, degs);
     __syncthreads();
-    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_build(S, D, st, recs, pos, ibase, mbeg, sub_dt);
+    for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x)
+        item_build<WARM>(S, D, st, recs, pos, ibase, mbeg, sub_dt);
     for (uint32_t s = 0; s < sub_steps; ++s) {
         const bool last = s + 1 == sub_steps;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces(D, st, i, last, sub_dt);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces<WARM>(D, st, i, last, sub_dt);
         __syncthreads();
-        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_tgs(S, D, st, recs, pos, last, sub_dt);
+        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x)
+            item_tgs<WARM>(S, D, st, recs, pos, last, sub_dt);
         __syncthreads();
-        if constexpr (LANES) item_chain_lanes(S, st, recs, lanes, nitems, iters, sub_dt, w, s_abort);
-        else item_chain<true, false>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        if constexpr (LANES) item_chain_lanes<false, false, WARM>(S, st, recs, lanes, nitems, iters, sub_dt, w, s_abort);
+        else item_chain<true, false, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
         __syncthreads();
@@ -1996,7 +2143,7 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
 }
 
 // the record kernel: the worlds solve_route gives it, one workgroup each
-template <int MAXT, int WAVES = 1>
+template <int MAXT, int WAVES = 1, bool WARM = false>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t slot_cap, uint32_t slot_nmax,
              uint32_t lds_nmax)
@@ -2009,14 +2156,66 @@ void k_solve(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t s
     const uint32_t ibase = S.w_item_base[w];
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const WorldDev& D = S.worlds[w];
-    if (solve_route((uint32_t)D.nbodies, nitems, slot_cap, slot_nmax, lds_nmax) != 1) return;
+    if (solve_route((uint32_t)D.nbodies, nitems, slot_cap, slot_nmax, lds_nmax, D.warm > 0.0f) != 1) return;
     // LDS: the chain's lane slots (workgroups of at most LANES_MAXT threads),
     // then the body state
     constexpr bool LANES = MAXT <= LANES_MAXT;
     uint4* lanes = LANES ? reinterpret_cast<uint4*>(smem) : nullptr;
     uint32_t* body = smem + (LANES ? MAXT * LANE_WORDS : 0u);
-    rec_world<LANES>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, body,
+    rec_world<LANES, WARM>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps, iters, body,
                      lanes, s_abort);
+}
+
+// ---------------------------------- warm starting in the phase kernels
+// grid-wide over the step's manifolds / bodies; worlds without warm starting
+// (and stopped worlds) are skipped
+
+// after the per-body item lists: the matches, then each body's sum
+__global__ void k_warm_match(StepDev S, uint32_t sub_steps)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t M = dev_count(&S.counters[C_MANIFOLDS], S.cap_m);
+    for (uint32_t m = grid_tid(); m < M; m += grid_stride()) {
+        const uint32_t w = S.bworld[S.m_ref[m]];
+        const WorldDev& D = S.worlds[w];
+        if (D.warm > 0.0f && S.w_status[w] == 0) warm_match(S, D, m, D.body_base, sub_steps);
+    }
+}
+
+__global__ void k_warm_sums(StepDev S)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    for (uint32_t x = grid_tid(); x < S.nb; x += grid_stride()) {
+        const uint32_t w = S.bworld[x];
+        const WorldDev& D = S.worlds[w];
+        if (!(D.warm > 0.0f) || S.w_status[w] != 0) continue;
+        const uint32_t o = S.deg_off[x];
+        warm_body_sum(S, D, S.lists + o, S.deg_off[x + 1] - o, x - D.body_base, S.w_item_base[w],
+                      (uint32_t)D.njoints, S.w_man_begin[w]);
+    }
+}
+
+// after the solve: clear the tables' counts, then store the manifolds
+__global__ void k_warm_clear(StepDev S)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    for (uint32_t x = grid_tid(); x < S.nb; x += grid_stride()) {
+        const uint32_t w = S.bworld[x];
+        const WorldDev& D = S.worlds[w];
+        if (D.warm > 0.0f && S.w_status[w] == 0) D.wcnt[x - D.body_base] = 0u;
+    }
+}
+
+__global__ void k_warm_save(StepDev S)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t M = dev_count(&S.counters[C_MANIFOLDS], S.cap_m);
+    for (uint32_t m = grid_tid(); m < M; m += grid_stride()) {
+        const uint32_t w = S.bworld[S.m_ref[m]];
+        const WorldDev& D = S.worlds[w];
+        if (D.warm > 0.0f && S.w_status[w] == 0)
+            warm_save(S, D, m, S.w_man_begin[w], S.w_man_begin[w + 1], D.body_base);
+    }
 }
 
 // ------------------------------------- worlds beyond LDS: split sub-steps
@@ -2050,7 +2249,7 @@ __global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s
         const SolverState<false> st = global_state(S, D);
         const int i = (int)(x - D.body_base);
         if (phase == 0) body_setup(D, st, i, false, S.deg + D.body_base);
-        else if (phase == 1) body_forces(D, st, i, s + 1 == sub_steps, sub_dt);
+        else if (phase == 1) body_forces<true>(D, st, i, s + 1 == sub_steps, sub_dt);
         else body_positions(D, st, i, sub_dt);
     }
 }
@@ -2074,8 +2273,8 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
     const int n = D.nbodies;
     uint4* recs = S.recs + (size_t)ibase * 8;
     for (uint32_t pos = t; pos < nitems; pos += T) {
-        if (phase == 0) item_build(S, D, st, recs, pos, ibase, S.w_man_begin[w], sub_dt);
-        else if (phase == 1) item_tgs(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
+        if (phase == 0) item_build<true>(S, D, st, recs, pos, ibase, S.w_man_begin[w], sub_dt);
+        else if (phase == 1) item_tgs<true>(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
         else item_writeback(S, recs, pos);
     }
     if (phase != 0) return;
@@ -2144,7 +2343,7 @@ constexpr size_t COMPACT_LDS = 160 * 1024 - 1024;
 // items touch fit LDS, their state is gathered there under compact indices,
 // the chain runs on LDS, and the state is scattered back; otherwise the
 // chain runs on global memory.
-template <int MAXT>
+template <int MAXT, bool WARM = false>
 __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax, uint32_t iters, float sub_dt)
 {
     extern __shared__ __align__(16) uint32_t smem[];
@@ -2180,8 +2379,10 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
             st.deg[c] = S.deg[base + i];
         }
         __syncthreads();
-        if (lanes) item_chain_lanes<true>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w, s_abort);
-        else item_chain<true, true>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
+        if (lanes)
+            item_chain_lanes<true, false, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+                                                s_abort);
+        else item_chain<true, true, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
             const uint32_t i = S.cinv[base + c];
@@ -2191,7 +2392,7 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
         // body state and counters in global memory; the lane slots take the
         // start of LDS, which this path does not otherwise use
         const SolverState<false> st = global_state(S, D);
-        item_chain_lanes<false, true>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+        item_chain_lanes<false, true, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
                                       s_abort);
     }
 }
@@ -2598,7 +2799,7 @@ void k_solve_slots(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint
     const uint32_t ibase = S.w_item_base[w];
     const uint32_t nitems = S.w_item_base[w + 1] - ibase;
     const WorldDev& D = S.worlds[w];
-    if (solve_route((uint32_t)D.nbodies, nitems, (uint32_t)NSLOT * MAXT, slot_nmax, 0u) != 0) return;
+    if (solve_route((uint32_t)D.nbodies, nitems, (uint32_t)NSLOT * MAXT, slot_nmax, 0u, D.warm > 0.0f) != 0) return;
     slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, S.w_man_begin[w], S.deg + D.body_base, dt, sub_steps,
                                        iters, smem, s_abort);
 }
@@ -2755,27 +2956,27 @@ __device__ __forceinline__ void fused_overflow(const StepDev& S, uint32_t w, uin
 // fit the slots, else the record path.  Inlined: a call would save and
 // restore the step's registers around it every step (scratch traffic of the
 // order of the step's own); WAVES sets the register budget of the shape.
-template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP, bool WARM>
 __device__ __forceinline__ void fused_solve(const StepDev& S, const WorldDev& D, uint32_t w, uint32_t ibase,
                                          uint32_t nitems, uint32_t mbeg, const uint32_t* degs, float dt,
                                          uint32_t sub_steps, uint32_t iters, uint32_t* lds, uint4* lanes,
                                          uint32_t* s_abort)
 {
     if constexpr (NSLOT > 0) {
-        if (nitems <= (uint32_t)NSLOT * MAXT) {
+        if (nitems <= (uint32_t)NSLOT * MAXT && !(WARM && D.warm > 0.0f)) {
             slots_world<NSLOT, MAXT, LOCKSTEP>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds,
                                                *s_abort);
             return;
         }
     }
-    rec_world<true>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, lanes, *s_abort);
+    rec_world<true, WARM>(S, D, w, ibase, nitems, mbeg, degs, dt, sub_steps, iters, lds, lanes, *s_abort);
 }
 
 // One workgroup per world (S.w_order), up to nsteps steps per world counted
 // in S.w_done, so a relaunch after an overflow resumes each world where it
 // stopped.  The solve takes the register-slot path when the step's items fit
 // NSLOT * MAXT slots, else the record path; both keep body state in LDS.
-template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP>
+template <int NSLOT, int MAXT, int WAVES, bool LOCKSTEP, bool WARM = false>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t nsteps)
 {
@@ -3005,14 +3206,28 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
             }
         }
         __syncthreads();
+        if (WARM && D.warm > 0.0f) {
+            // warm starting: match last step's manifolds, then each body's carried impulse
+            for (uint32_t q = t; q < M; q += T) warm_match(S, D, X.m0 + q, base, sub_steps);
+            __syncthreads();
+            for (uint32_t x = t; x < n; x += T) warm_body_sum(S, D, lists + L.doff[x], L.dcnt[x], x, 0u, nj, X.m0);
+            __syncthreads();
+        }
         mark(FP_SCHED);
         // ---- solve
-        fused_solve<NSLOT, MAXT, WAVES, LOCKSTEP>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
+        fused_solve<NSLOT, MAXT, WAVES, LOCKSTEP, WARM>(S, D, w, X.i0, nit, X.m0, L.dcnt, dt, sub_steps, iters, L.solve,
                                                   L.lanes,
                                                   &s_abort);
         __syncthreads();
         mark(FP_SOLVE);
         if (s_abort) break;                     // the solve stalled: status set
+        if (WARM && D.warm > 0.0f) {
+            // warm starting: this step's manifolds become the next step's tables
+            for (uint32_t x = t; x < n; x += T) D.wcnt[x] = 0u;
+            __syncthreads();
+            for (uint32_t q = t; q < M; q += T) warm_save(S, D, X.m0 + q, X.m0, X.m0 + M, base);
+            __syncthreads();
+        }
         if (t == 0) {
             cnt[0] = E; cnt[1] = R; cnt[2] = M; cnt[3] = nit;
             unsigned long long* sums = reinterpret_cast<unsigned long long*>(S.counters + C_SUMS);
@@ -3072,7 +3287,7 @@ struct SolveVariant {
 // (fewer waves per world keep a dependency chain's wave from sharing its
 // SIMD with the world's other waves: 256 threads beat 1,024 by about 3x on a
 // 4,096-body world)
-static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
+static const SolveVariant kSolveVariants[SOLVE_TABLE] = {
     {nullptr, 0, 0, k_solve<1024>, 1024, nullptr},
     {k_solve_slots<4, 256, 3>, 256, 4, k_solve<256, 2>, 256, k_fused<4, 256, 3, false>},
     {k_solve_slots<2, 512, 3>, 512, 2, k_solve<256, 2>, 256, k_fused<2, 512, 3, false>},
@@ -3094,11 +3309,14 @@ static const SolveVariant kSolveVariants[SOLVE_VARIANTS] = {
     // records only, 128-thread workgroups at 2 waves per SIMD: the same four
     // worlds per CU with the full register file (no spills)
     {nullptr, 0, 0, k_solve<128, 2>, 128, k_fused<0, 128, 2, false>},
+    // WARM_VARIANT: batches with a warm-started world (the shape of variant
+    // 11 with the warm-start passes compiled in; the kernels above carry none)
+    {nullptr, 0, 0, k_solve<192, 3, true>, 192, k_fused<0, 192, 3, false, true>},
 };
 
 bool fused_slots(int v)
 {
-    return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].slot_fn != nullptr;
+    return v >= 0 && v < SOLVE_TABLE && kSolveVariants[v].slot_fn != nullptr;
 }
 
 uint32_t fused_threads(int v)
@@ -3109,7 +3327,7 @@ uint32_t fused_threads(int v)
 
 bool fused_available(int v)
 {
-    return v >= 0 && v < SOLVE_VARIANTS && kSolveVariants[v].fused_fn != nullptr;
+    return v >= 0 && v < SOLVE_TABLE && kSolveVariants[v].fused_fn != nullptr;
 }
 
 hipError_t fused_attributes(int v, hipFuncAttributes* out)
@@ -3125,8 +3343,8 @@ void launch_fused(const StepDev& S, int v, uint32_t nworlds, float dt, uint32_t 
     hipLaunchKernelGGL(sv.fused_fn, dim3(nworlds), dim3(threads), lds, st, S, dt, sub_steps, iters, nsteps);
 }
 
-unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].threads; }
-int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_VARIANTS && v >= 0 ? v : 0].slots; }
+unsigned solve_variant_threads(int v) { return kSolveVariants[v < SOLVE_TABLE && v >= 0 ? v : 0].threads; }
+int solve_variant_slots(int v) { return kSolveVariants[v < SOLVE_TABLE && v >= 0 ? v : 0].slots; }
 
 void KernelTrace::mark(const char* name, hipStream_t st)
 {
@@ -3187,11 +3405,13 @@ static inline void tmark(const Streams& ss, const char* name, hipStream_t st)
     if (ss.trace) ss.trace->mark(name, st);
 }
 
-SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored)
+SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored,
+                     bool any_warm)
 {
     SolvePlan p;
     p.any_colored = any_colored ? 1u : 0u;
-    p.variant = (variant >= 0 && variant < SOLVE_VARIANTS) ? variant : 0;
+    p.any_warm = any_warm ? 1u : 0u;
+    p.variant = (variant >= 0 && variant < SOLVE_TABLE) ? variant : 0;
     const SolveVariant& v = kSolveVariants[p.variant];
     p.slot_cap = v.slot_fn ? (uint32_t)v.slots * v.slot_threads : 0u;
     p.slot_nmax = v.slot_fn ? SLOT_NMAX : 0u;
@@ -3281,6 +3501,12 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             hipLaunchKernelGGL(k_list_rank<true>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
             tmark(ss, "k_list_rank<true>", st);
         }
+        if (sp.any_warm) {
+            hipLaunchKernelGGL(k_warm_match, dim3(GRID), dim3(BS), 0, st, S, sub_steps);
+            tmark(ss, "k_warm_match", st);
+            hipLaunchKernelGGL(k_warm_sums, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+            tmark(ss, "k_warm_sums", st);
+        }
         break;
     default: {
         // the three solve kernels take disjoint sets of worlds (solve_route);
@@ -3305,8 +3531,12 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps, sub_dt);
                 hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s,
                                    sub_steps, sub_dt);
-                hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S, sp.lds_nmax, iters,
-                                   sub_dt);
+                if (sp.any_warm)
+                    hipLaunchKernelGGL((k_big_chain<256, true>), dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S,
+                                       sp.lds_nmax, iters, sub_dt);
+                else
+                    hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S, sp.lds_nmax,
+                                       iters, sub_dt);
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 2, s, sub_steps, sub_dt);
             }
             hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps,
@@ -3332,6 +3562,12 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             (void)hipEventRecord(ss.join[1], ss.aux[1]);
             (void)hipStreamWaitEvent(st, ss.join[1], 0);
         }
+        if (sp.any_warm) {
+            hipLaunchKernelGGL(k_warm_clear, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
+            tmark(ss, "k_warm_clear", st);
+            hipLaunchKernelGGL(k_warm_save, dim3(GRID), dim3(BS), 0, st, S);
+            tmark(ss, "k_warm_save", st);
+        }
         hipLaunchKernelGGL(k_commit_step, dim3(1), dim3(64), 0, st, S.counters);
         tmark(ss, "k_commit_step", st);
         break;
@@ -3342,7 +3578,9 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
 void set_solve_smem_limit()
 {
     (void)hipFuncSetAttribute((const void*)k_big_chain<256>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    for (int v = 0; v < SOLVE_VARIANTS; ++v) {
+    (void)hipFuncSetAttribute((const void*)k_big_chain<256, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    for (int v = 0; v < SOLVE_TABLE; ++v) {
         if (kSolveVariants[v].slot_fn)
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

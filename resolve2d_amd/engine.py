@@ -101,6 +101,9 @@ class GpuWorld:
     def set_schedule(self, mode):
         N.check(self.lib.r2d_set_schedule(self.handle, mode), "r2d_set_schedule")
 
+    def set_warm_start(self, factor):
+        N.check(self.lib.r2d_set_warm_start(self.handle, factor), "r2d_set_warm_start")
+
     def process(self, dt, sub_steps, iters):
         return self.lib.r2d_process(self.handle, dt, sub_steps, iters)
 

@@ -412,3 +412,8 @@ class Solver:
 
     def set_schedule(self, mode: int):
         N.check(self.lib.r2d_set_schedule(self.handle, mode), "r2d_set_schedule")
+
+    def set_warm_start(self, factor: float):
+        """Opt-in warm starting (r2d_set_warm_start): 0 off (the reference's
+        results), (0, 1] the fraction of the previous step's impulses carried."""
+        N.check(self.lib.r2d_set_warm_start(self.handle, factor), "r2d_set_warm_start")
