@@ -21,8 +21,10 @@ request latency); accuracy (bit equality with the f32 oracle, error against
 an f64 run of the same method, maximum penetration, energy drift of the
 resting pyramid); a roofline entry for the dominant kernel (its launch timed
 with HIP events on the library's stream; HBM traffic from the committed
-rocprofv3 counter measurement; an LDS figure); and a CPU baseline (the
-oracle restatement on a bounded sample, worlds in parallel over host cores).
+rocprofv3 counter measurement; an LDS figure); a CPU baseline (the
+oracle restatement on a bounded sample, worlds in parallel over host cores);
+and the opt-in warm starting (the batch warm started: its throughput against
+the cold run; the resting pyramid warm against cold).
 """
 import argparse
 import json
@@ -239,6 +241,76 @@ def pyramid_energy(ctx, steps):
     return out
 
 
+def warm_start_runs(ctx, specs, args, cold_value, factor=0.8):
+    """Opt-in warm starting (r2d_set_warm_start): the batched workload with
+    every world warm started (same warm-up and timed steps as the cold run
+    above, so the ratio is the feature's cost), and the resting 20-row
+    pyramid over 600 steps warm against cold: deepest contact, energy drift,
+    contacts kept, GPU time per step and bit equality with the f32 oracle's
+    warm starting."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    dt, S, K = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
+    worlds = []
+    for sc in specs:
+        w = GpuWorld(ctx)
+        apply_scene(sc, w)
+        w.set_warm_start(factor)
+        worlds.append(w)
+    n_bodies = sum(count_bodies(sc) for sc in specs)
+    batch = ctx.prepare_batch([w.handle for w in worlds])
+    for _ in range(args.warmup):
+        batch.process(dt, S, K, 1)
+    ctx.set_profiling(True)
+    ctx.reset_phase_times()
+    ctx.sync()
+    t0 = time.perf_counter()
+    batch.process(dt, S, K, args.steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    kernel_ms, launches, _ = ctx.fused_times()
+    ctx.set_profiling(False)
+    value = n_bodies * args.steps / elapsed
+    for w in worlds:
+        w.close()
+    out = {"factor": factor, "batched": {"value": value, "unit": "body-steps/s", "vs_cold": value / cold_value,
+                                         "fused_launches": launches, "kernel_ms": kernel_ms,
+                                         "note": "every world warm started; same warm-up and timed steps as the "
+                                                 "cold headline run"}}
+    pyr = {}
+    for f in (0.0, factor):
+        g = GpuWorld(ctx)
+        o = pyoracle.OracleWorld("f32")
+        for e in (g, o):
+            apply_scene(scenes.pyramid(20), e)
+            if f:
+                e.set_warm_start(f)
+        e0 = energy(g.bodies())
+        pen, worst, gpu_s = 0.0, 0.0, 0.0
+        for s in range(1, 601):
+            a = time.perf_counter()
+            checked(g.process(dt, S, K), "pyramid step")
+            gpu_s += time.perf_counter() - a
+            checked(o.process(dt, S, K), "oracle pyramid step")
+            if s % 20 == 0:
+                pen = max(pen, max_penetration(g.manifolds()))
+                worst = max(worst, abs(energy(g.bodies()) - e0) / abs(e0))
+        b = g.bodies()
+        pyr["warm" if f else "cold"] = {
+            "max_penetration_m": pen, "drift": (energy(b) - e0) / abs(e0), "max_abs_drift": worst,
+            "manifolds_at_600": int(g.manifolds().shape[0]), "gpu_ms_per_step": gpu_s / 600 * 1e3,
+            "bit_equal_to_f32_oracle": bool(np.array_equal(b.astype(np.float32).view(np.uint32),
+                                                           o.bodies().astype(np.float32).view(np.uint32)))}
+        g.close()
+        o.close()
+    out["pyramid_600"] = pyr
+    out["pyramid_600"]["note"] = ("20-row pyramid (210 boxes, ground), 600 steps; penetration and drift sampled "
+                                  "every 20 steps; 590 contacts at rest")
+    return out
+
+
 def large_world(ctx, steps, snaps, with_f64):
     """The large constrained world (65,536 bodies, ropes, motors, walls): GPU
     step time over the horizon and the error against an f64 run of the same
@@ -365,6 +437,7 @@ def main():
                     help="steps a serving lane's launch takes (fewer when a running scene has fewer left)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-starting runs")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the timed region")
     ap.add_argument("--tuned-hash", action="store_true")
@@ -557,6 +630,12 @@ def main():
                                           args.single_steps, 6.0)
 
     large = None
+    warm = None
+    if not args.no_warm:
+        log("warm starting")
+        warm = warm_start_runs(ctx, specs, args, value)
+        log(f"warm starting: batched {warm['batched']['value'] / 1e6:.1f} M body-steps/s "
+            f"({warm['batched']['vs_cold']:.2f}x cold)")
     if not args.no_large:
         log("large constrained world")
         large = large_world(ctx, 120, {1, 10, 120}, not args.no_accuracy)
@@ -631,6 +710,7 @@ def main():
         "single_world_4096": single,
         "large_world_65536": large,
         "accuracy": acc,
+        "warm_start": warm,
         "mode": ("fused: one kernel runs every step of every world, one workgroup per world" if fused_launches else
                  "phase kernels: one launch sequence (hipGraph) per step"),
         "phase_ms": per_step,
