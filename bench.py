@@ -695,6 +695,8 @@ def main():
                                     if tuned else {"autotuned": False, "variant": args.variant})},
         "p50_step_ms": float(np.percentile(lat_ms, 50)) if len(lat_ms) else None,
         "p99_step_ms": float(np.percentile(lat_ms, 99)) if len(lat_ms) else None,
+        "p999_step_ms": float(np.percentile(lat_ms, 99.9)) if len(lat_ms) else None,
+        "max_step_ms": float(lat_ms.max()) if len(lat_ms) else None,
         "step_latency": ("each world's step, wall clock of its workgroup in the fused kernel (all world-steps of the "
                          "timed region)" if fused_launches else
                          "device time per step from HIP events (the timed region's steps)"),

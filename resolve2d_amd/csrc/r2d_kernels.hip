@@ -3085,13 +3085,20 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         __syncthreads();
         mark(FP_BUCKETS);
         // raw candidates per entry: distinct bucket members above the entry's
-        // body (k_ent_cnt) that pass the both-static and AABB filters
+        // body (k_ent_cnt) that pass the both-static and AABB filters; the
+        // passing members among the first 32 as a mask (in `items`, whose
+        // bucket lists are no longer read)
         for (uint32_t e = t; e < E; e += T) {
             uint32_t a = ent_body[e], b = ent_bkt[e];
-            uint32_t o = L.boff[b], c = L.boff[b + 1] - o, q = 0;
+            uint32_t o = L.boff[b], c = L.boff[b + 1] - o, q = 0, mask = 0;
             for (uint32_t j = 0; j < c && sorted[o + j] > a; ++j)
-                if (j == 0 || sorted[o + j] != sorted[o + j - 1]) q += pair_passes(P, a, sorted[o + j]);
+                if (j == 0 || sorted[o + j] != sorted[o + j - 1]) {
+                    const bool ok = pair_passes(P, a, sorted[o + j]);
+                    q += ok;
+                    if (ok && j < 32) mask |= 1u << j;
+                }
             ecnt[e] = q;
+            items[e] = mask;
         }
         __syncthreads();
         const uint32_t R = group_scan(ecnt, E, s_sm);
@@ -3104,9 +3111,12 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
             uint32_t o = L.boff[b];
             uint32_t ro = ecnt[e], c = ecnt[e + 1] - ro;
             uint32_t k = e - L.coff[a];
-            for (uint32_t j = 0, q = 0; q < c; ++j) {
+            uint32_t q = 0;
+            for (uint32_t mask = items[e]; mask; mask &= mask - 1u, ++q)
+                cand[ro + q] = make_uint4(a, sorted[o + (uint32_t)__builtin_ctz(mask)], k, 0u);
+            for (uint32_t j = 32; q < c; ++j) {
                 const uint32_t bj = sorted[o + j];
-                if ((j > 0 && bj == sorted[o + j - 1]) || !pair_passes(P, a, bj)) continue;
+                if (bj == sorted[o + j - 1] || !pair_passes(P, a, bj)) continue;
                 cand[ro + q] = make_uint4(a, bj, k, 0u);
                 ++q;
             }

@@ -168,3 +168,40 @@ def test_finished_worlds_may_be_destroyed_before_the_server(gpu_ctx):
     assert ws[3].process(1.0 / 60.0, 4, 4) == 0
     assert all(x > 0 for x in t + [late])
     ws[3].close()
+
+
+def test_served_warm_worlds_equal_alone(gpu_ctx):
+    """Warm-started worlds (r2d_set_warm_start) served beside cold ones: the
+    lanes' batches take the warm kernels, and every world ends bit-identical
+    to the same scene stepped alone with the same factor."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.serve import Server
+    specs = [("served", 21, 300), ("ops", scenes.pyramid(rows=10)), ("served", 22, 2000),
+             ("ops", scenes.batched_world(41)), ("served", 23, 90)]
+    warms = [0.8, 1.0, 0.8, 0.0, 0.5]
+    mix = []
+    for sp, f in zip(specs, warms):
+        w = _served_world(gpu_ctx, sp[1], sp[2]) if sp[0] == "served" else _scene_world(gpu_ctx, sp[1])
+        w.set_warm_start(f)
+        mix.append(w)
+    sv = Server(gpu_ctx, sum(w.footprint() for w in mix), max_worlds=3, steps_per_tick=3)
+    try:
+        tickets = [sv.submit(w, STEPS) for w in mix]
+        for _ in range(10 * STEPS * len(mix)):
+            sv.tick()
+            if sv.stats()["completed"] == len(mix):
+                break
+        for t in tickets:
+            p = sv.poll(t)
+            assert p["state"] == 2 and p["status"] == 0 and p["steps_done"] == STEPS, p
+    finally:
+        sv.close()
+    for i, (sp, f, w) in enumerate(zip(specs, warms, mix)):
+        ref = _served_world(gpu_ctx, sp[1], sp[2]) if sp[0] == "served" else _scene_world(gpu_ctx, sp[1])
+        ref.set_warm_start(f)
+        for _ in range(STEPS):
+            assert ref.process(1.0 / 60.0, 4, 4) == 0
+        a, b = w.bodies(), ref.bodies()
+        assert bits_equal(a, b), (i, f, first_diff(a, b))
+        w.close()
+        ref.close()
