@@ -2338,12 +2338,17 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
 // LDS bytes of the compact chain per body (momentum, inverse masses, counter, item count)
 constexpr size_t COMPACT_BYTES = 7 * 4;
 constexpr size_t COMPACT_LDS = 160 * 1024 - 1024;
+// lanes of the global-memory chain's workgroup
+constexpr int BIG_GLOBAL_THREADS = 1024;
 
 // one sub-step's K iterations; one workgroup per world.  When the bodies the
 // items touch fit LDS, their state is gathered there under compact indices,
 // the chain runs on LDS, and the state is scattered back; otherwise the
-// chain runs on global memory.
-template <int MAXT, bool WARM = false>
+// chain runs on global memory.  PART 0 takes only the worlds of the first
+// kind, 1 only those of the second (launched with more lanes: a lane owns
+// fewer positions, so fewer of its items wait behind one whose operands are
+// not ready, and the chain's loads and polls, all to L2, overlap more).
+template <int MAXT, bool WARM = false, int PART = 0>
 __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax, uint32_t iters, float sub_dt)
 {
     extern __shared__ __align__(16) uint32_t smem[];
@@ -2359,10 +2364,12 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
     const uint32_t base = D.body_base;
     uint4* recs = S.recs + (size_t)ibase * 8;
     const uint32_t nc = S.w_nc[w];
+    const bool compact = (size_t)nc * COMPACT_BYTES <= COMPACT_LDS;
+    if (compact != (PART == 0)) return;   // uniform: the other launch takes this world
     // the lane-slot chain when the slots fit beside the compact state
     constexpr size_t LANE_BYTES = (size_t)MAXT * LANE_WORDS * 4u;
     const bool lanes = (size_t)nc * COMPACT_BYTES + LANE_BYTES <= COMPACT_LDS;
-    if ((size_t)nc * COMPACT_BYTES <= COMPACT_LDS) {
+    if constexpr (PART == 0) {
         SolverState<true> st;
         st.mx = (float*)(smem + (lanes ? MAXT * LANE_WORDS : 0u));
         st.my = st.mx + nc;
@@ -2389,6 +2396,7 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
             D.mx[i] = st.mx[c]; D.my[i] = st.my[c]; D.L[i] = st.L[c];
         }
     } else {
+        (void)lanes;
         // body state and counters in global memory; the lane slots take the
         // start of LDS, which this path does not otherwise use
         const SolverState<false> st = global_state(S, D);
@@ -3541,12 +3549,19 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps, sub_dt);
                 hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s,
                                    sub_steps, sub_dt);
-                if (sp.any_warm)
-                    hipLaunchKernelGGL((k_big_chain<256, true>), dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S,
+                // worlds whose touched bodies fit LDS, then those that do not
+                constexpr size_t GLOBAL_LDS = (size_t)BIG_GLOBAL_THREADS * LANE_WORDS * 4u;
+                if (sp.any_warm) {
+                    hipLaunchKernelGGL((k_big_chain<256, true, 0>), dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S,
                                        sp.lds_nmax, iters, sub_dt);
-                else
-                    hipLaunchKernelGGL(k_big_chain<256>, dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S, sp.lds_nmax,
-                                       iters, sub_dt);
+                    hipLaunchKernelGGL((k_big_chain<BIG_GLOBAL_THREADS, true, 1>), dim3(S.nworlds),
+                                       dim3(BIG_GLOBAL_THREADS), GLOBAL_LDS, sb, S, sp.lds_nmax, iters, sub_dt);
+                } else {
+                    hipLaunchKernelGGL((k_big_chain<256, false, 0>), dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S,
+                                       sp.lds_nmax, iters, sub_dt);
+                    hipLaunchKernelGGL((k_big_chain<BIG_GLOBAL_THREADS, false, 1>), dim3(S.nworlds),
+                                       dim3(BIG_GLOBAL_THREADS), GLOBAL_LDS, sb, S, sp.lds_nmax, iters, sub_dt);
+                }
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 2, s, sub_steps, sub_dt);
             }
             hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps,
@@ -3587,9 +3602,14 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
 
 void set_solve_smem_limit()
 {
-    (void)hipFuncSetAttribute((const void*)k_big_chain<256>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_big_chain<256, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)k_big_chain<256, false, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_big_chain<256, true, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_big_chain<BIG_GLOBAL_THREADS, false, 1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_big_chain<BIG_GLOBAL_THREADS, true, 1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     for (int v = 0; v < SOLVE_TABLE; ++v) {
         if (kSolveVariants[v].slot_fn)
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].slot_fn,

@@ -106,3 +106,33 @@ def test_warm_is_refused_with_the_coloured_schedule(gpu_ctx):
         with pytest.raises(R2DError):
             w.set_warm_start(bad)
     w.close()
+
+
+def test_warm_tables_survive_growth(gpu_ctx, oracle):
+    """Bodies added between steps grow the device body block past its
+    capacity: the warm tables move to larger ones and keep what they held,
+    so the next step still carries impulses, as in the oracle."""
+    from resolve2d_amd.scenes import _Builder
+    base = _Builder()
+    base.gravity(9.81)
+    base.static(base.rect(0.0, -0.5, 40.0, 1.0))
+    for i in range(6):
+        for j in range(3):
+            base.rect(-6.0 + 2.0 * i, 0.5 + 1.0 * j, 1.0, 1.0)
+    more = _Builder()
+    for i in range(40):
+        more.disc(-15.0 + 0.8 * i, 8.0 + (i % 3), 0.35)
+    g, o = _pair(gpu_ctx, oracle, base.ops, 0.8)
+    print("warm world: 19 bodies, 40 added after 8 steps", flush=True)
+    for _ in range(8):
+        assert g.process(DT, 4, 4) == 0 and o.process(DT, 4, 4) == 0
+    _same(g, o, "before growth")
+    for e in (g, o):
+        apply_scene(more.ops, e)
+    assert g.process(DT, 4, 4) == 0 and o.process(DT, 4, 4) == 0
+    assert o.num_warm() > 0
+    for _ in range(20):
+        assert g.process(DT, 4, 4) == 0 and o.process(DT, 4, 4) == 0
+    _same(g, o, "after growth")
+    g.close()
+    o.close()
