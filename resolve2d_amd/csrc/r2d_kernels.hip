@@ -344,8 +344,10 @@ __device__ __forceinline__ bool excluded(const WorldDev& D, uint32_t ida, uint32
 // ------------------------------------------------------------ broadphase
 __global__ void k_fill_bworld(StepDev S)
 {
-    const WorldDev& D = S.worlds[blockIdx.x];
-    for (int i = threadIdx.x; i < D.nbodies; i += blockDim.x) S.bworld[D.body_base + i] = blockIdx.x;
+    // the descriptor's fields read once: the stores below could alias them,
+    // so reading them in the loop would reload them every iteration
+    const uint32_t n = (uint32_t)S.worlds[blockIdx.x].nbodies, base = S.worlds[blockIdx.x].body_base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) S.bworld[base + i] = blockIdx.x;
 }
 
 // One body's broadphase preparation: cos / sin of the angle, the scheduling
@@ -2254,11 +2256,30 @@ __global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s
     }
 }
 
-// phase 0: build records, then the compact chain indices; 1: updateTGSDepth
-// (sub-step s); 2: write back.  One workgroup (BIG_THREADS) per world.
+// the per-item parts, BIG_SPLIT workgroups per world (gridDim.y): phase 0
+// builds the records, 1 is updateTGSDepth (sub-step s), 2 writes back
 constexpr unsigned BIG_THREADS = 256;
-__global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t lds_nmax, int phase, uint32_t s,
-                                                        uint32_t sub_steps, float sub_dt)
+constexpr unsigned BIG_SPLIT = 32;
+__global__ __launch_bounds__(BIG_THREADS) void k_big_item_pass(StepDev S, uint32_t lds_nmax, int phase, uint32_t s,
+                                                            uint32_t sub_steps, float sub_dt)
+{
+    if (S.counters[C_OVERFLOW]) return;
+    const uint32_t w = blockIdx.x;
+    if (S.w_status[w] != 0) return;
+    const WorldDev& D = S.worlds[w];
+    if (!big_world(D, lds_nmax)) return;
+    const SolverState<false> st = global_state(S, D);
+    const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase, mbeg = S.w_man_begin[w];
+    uint4* recs = S.recs + (size_t)ibase * 8;
+    for (uint32_t pos = blockIdx.y * blockDim.x + threadIdx.x; pos < nitems; pos += gridDim.y * blockDim.x) {
+        if (phase == 0) item_build<true>(S, D, st, recs, pos, ibase, mbeg, sub_dt);
+        else if (phase == 1) item_tgs<true>(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
+        else item_writeback(S, recs, pos);
+    }
+}
+
+// after the records: the compact chain indices, one workgroup per world
+__global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t lds_nmax)
 {
     __shared__ uint32_t s_scan[BIG_THREADS];
     __shared__ uint32_t s_total;
@@ -2267,17 +2288,10 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
     if (S.w_status[w] != 0) return;
     const WorldDev& D = S.worlds[w];
     if (!big_world(D, lds_nmax)) return;
-    const SolverState<false> st = global_state(S, D);
     const uint32_t ibase = S.w_item_base[w], nitems = S.w_item_base[w + 1] - ibase;
     const uint32_t base = D.body_base, T = blockDim.x, t = threadIdx.x;
     const int n = D.nbodies;
     uint4* recs = S.recs + (size_t)ibase * 8;
-    for (uint32_t pos = t; pos < nitems; pos += T) {
-        if (phase == 0) item_build<true>(S, D, st, recs, pos, ibase, S.w_man_begin[w], sub_dt);
-        else if (phase == 1) item_tgs<true>(S, D, st, recs, pos, s + 1 == sub_steps, sub_dt);
-        else item_writeback(S, recs, pos);
-    }
-    if (phase != 0) return;
     // compact chain indices: the bodies the items touch, numbered in slot order
     const uint32_t UNUSED = 0xffffffffu;
     for (int i = t; i < n; i += T) S.cmap[base + i] = UNUSED;
@@ -2294,27 +2308,37 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
         }
     }
     __syncthreads();
-    // each thread numbers a contiguous range of slots: count, block scan, assign
-    const int chunk = (n + (int)T - 1) / (int)T;
-    const int lo = min((int)t * chunk, n), hi = min(lo + chunk, n);
+    // each wave numbers a contiguous range of slots, 64 at a time (coalesced
+    // loads, ranks from the wave's ballot): count, the waves' offsets, assign
+    constexpr uint32_t NW = BIG_THREADS / 64;
+    const uint32_t lane = t & 63u, wv = t >> 6;
+    const uint32_t span = (((uint32_t)n + NW - 1) / NW + 63u) & ~63u;
+    const uint32_t lo = umin(wv * span, (uint32_t)n), hi = umin(lo + span, (uint32_t)n);
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t cnt = 0;
-    for (int i = lo; i < hi; ++i) cnt += S.cmap[base + i] == 0u;
-    s_scan[t] = cnt;
+    for (uint32_t i0 = lo; i0 < hi; i0 += 64u) {
+        const uint32_t i = i0 + lane;
+        cnt += (uint32_t)__popcll(__ballot(i < hi && S.cmap[base + i] == 0u));
+    }
+    if (lane == 0) s_scan[wv] = cnt;
     __syncthreads();
-    for (uint32_t d = 1; d < T; d <<= 1) {
-        uint32_t v = t >= d ? s_scan[t - d] : 0u;
-        __syncthreads();
-        s_scan[t] += v;
-        __syncthreads();
+    uint32_t c = 0, total = 0;
+    for (uint32_t k = 0; k < NW; ++k) {
+        c += k < wv ? s_scan[k] : 0u;
+        total += s_scan[k];
     }
-    uint32_t c = s_scan[t] - cnt;
-    for (int i = lo; i < hi; ++i) {
-        if (S.cmap[base + i] != 0u) continue;
-        S.cmap[base + i] = c;
-        S.cinv[base + c] = (uint32_t)i;
-        ++c;
+    for (uint32_t i0 = lo; i0 < hi; i0 += 64u) {
+        const uint32_t i = i0 + lane;
+        const bool hit = i < hi && S.cmap[base + i] == 0u;
+        const uint64_t bal = __ballot(hit);
+        if (hit) {
+            const uint32_t ci = c + (uint32_t)__popcll(bal & below);
+            S.cmap[base + i] = ci;
+            S.cinv[base + ci] = i;
+        }
+        c += (uint32_t)__popcll(bal);
     }
-    if (t == T - 1) s_total = s_scan[T - 1];
+    if (t == 0) s_total = total;
     __syncthreads();
     const uint32_t nc = s_total;
     if (t == 0) S.w_nc[w] = nc;
@@ -3542,13 +3566,14 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             const hipStream_t sb = ss.aux[1];
             const float sub_dt = dt / (float)sub_steps;
             const unsigned gb = blocks_for(S.nb, BS);
+            const dim3 gi(S.nworlds, BIG_SPLIT);
             hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
-            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps,
-                               sub_dt);
+            hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
+            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax);
             for (uint32_t s = 0; s < sub_steps; ++s) {
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps, sub_dt);
-                hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s,
-                                   sub_steps, sub_dt);
+                hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps,
+                                   sub_dt);
                 // worlds whose touched bodies fit LDS, then those that do not
                 constexpr size_t GLOBAL_LDS = (size_t)BIG_GLOBAL_THREADS * LANE_WORDS * 4u;
                 if (sp.any_warm) {
@@ -3564,8 +3589,7 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
                 }
                 hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 2, s, sub_steps, sub_dt);
             }
-            hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps,
-                               sub_dt);
+            hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps, sub_dt);
             tmark(ss, "k_big (global state)", sb);
         }
         if (run_lds) {
