@@ -172,6 +172,7 @@ struct r2d_ctx {
     std::vector<WorldScratch> fscr;               // current slices (host copy)
     std::vector<WorldScratch> fscr_uploaded;
     std::vector<uint32_t> forder, forder_uploaded;
+    int num_cus = 0;                              // compute units of the device
     uint32_t* hdone = nullptr;                    // pinned: steps done per world, last-step / needed counts
     uint32_t* hcnt = nullptr;
     WorldScratch* hscr = nullptr;                 // pinned: the slice table as the last launch left it
@@ -706,6 +707,14 @@ int fused_layout(r2d_ctx* c, Batch& b)
     for (size_t i = 0; i < nw; ++i) c->forder[i] = (uint32_t)i;
     std::stable_sort(c->forder.begin(), c->forder.end(),
                      [&](uint32_t x, uint32_t y) { return b.worlds[x]->n > b.worlds[y]->n; });
+    // Workgroups b, b + P, b + 2P, ... (P = compute units) land on one CU when
+    // the dispatcher hands them out round-robin and they are resident at once,
+    // so with plain largest-first order one CU holds the largest world of every
+    // run of P.  Every other run is reversed: each CU mixes large and small
+    // worlds, and the call, which lasts as long as its slowest CU, ends sooner.
+    const size_t P = c->num_cus > 0 ? (size_t)c->num_cus : 0;
+    for (size_t q = 1; P && (q + 1) * P <= nw; q += 2)
+        std::reverse(c->forder.begin() + q * P, c->forder.begin() + (q + 1) * P);
     return fused_upload_tables(c);
 }
 
@@ -1419,6 +1428,8 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
     r2d_ctx* c = new (std::nothrow) r2d_ctx();
     if (!c) return R2D_E_OOM;
     c->device = device;
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        c->num_cus = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return R2D_E_DEVICE; }
     for (int i = 0; i < 2; ++i) HIPCHK(hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
