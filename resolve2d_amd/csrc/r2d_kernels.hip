@@ -1204,52 +1204,6 @@ __device__ __forceinline__ void rec_tgs(Rec& r, const StepDev& S, const WorldDev
     S.m_prev2[m] = a2;
 }
 
-// CollisionManifold.calculateImpulses (collision.zig:135-218) on a record
-template <class ST>
-__device__ __forceinline__ void rec_impulses(Rec& r, const ST& st, float dt)
-{
-    const uint32_t l1 = r.a, l2 = r.b;
-    float im1 = st.im[l1], ii1 = st.ii[l1], im2 = st.im[l2], ii2 = st.ii[l2];
-    v2 vl1 = scale2(mk(st.mx[l1], st.my[l1]), im1);
-    float om1 = st.L[l1] * ii1;
-    v2 vl2 = scale2(mk(st.mx[l2], st.my[l2]), im2);
-    float om2 = st.L[l2] * ii2;
-    v2 n = mk(r.nx, r.ny);
-    v2 t = rot90cw(n);
-    v2 app1 = mk(0.f, 0.f), app2 = mk(0.f, 0.f);
-    float rot1 = 0.f, rot2 = 0.f;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        if (!(r.flags & (RF_P0 << p))) continue;
-        float depth = r.p[p][4];
-        if (depth >= 0.f) { r.acc[2 * p] = 0.f; r.acc[2 * p + 1] = 0.f; continue; }
-        v2 r1 = mk(r.p[p][0], r.p[p][1]), r2 = mk(r.p[p][2], r.p[p][3]);
-        v2 v1 = add2(vl1, mk(-r1.y * om1, r1.x * om1));
-        v2 v2v = add2(vl2, mk(-r2.y * om2, r2.x * om2));
-        v2 dv = sub2(v1, v2v);
-        float bias = BAUMGARTE * maxn(0.0f, (-depth) - BAUMGARTE_SLOP) / dt;
-        float num = dot2(dv, n) + bias;
-        float pn = num * r.p[p][5];
-        if (pn < MIN_MANIFOLD_IMPULSE && !(r.flags & RF_WARM)) continue;   // warm started: clamped only
-        num = dot2(dv, t);
-        float pt = num * r.p[p][6];
-        float acc_pn = r.acc[2 * p], acc_pt = r.acc[2 * p + 1];
-        float new_pn = maxn(0.0f, acc_pn + pn);
-        float applied_pn = new_pn - acc_pn;
-        acc_pn = new_pn;
-        float max_pt = r.fric * absf(acc_pn);
-        float new_pt = clampn(acc_pt + pt, -max_pt, max_pt);
-        float applied_pt = new_pt - acc_pt;
-        r.acc[2 * p] = acc_pn;
-        r.acc[2 * p + 1] = new_pt;
-        v2 dp = add2(scale2(n, applied_pn), scale2(t, applied_pt));
-        if (!(r.flags & RF_S1)) { app1.x -= dp.x; app1.y -= dp.y; rot1 -= cross2(r1, dp); }
-        if (!(r.flags & RF_S2)) { app2.x += dp.x; app2.y += dp.y; rot2 += cross2(r2, dp); }
-    }
-    if (r.flags & RF_W1) { st.mx[l1] += app1.x; st.my[l1] += app1.y; st.L[l1] += rot1; }
-    if (r.flags & RF_W2) { st.mx[l2] += app2.x; st.my[l2] += app2.y; st.L[l2] += rot2; }
-}
-
 // What a joint link reads besides the bodies' momentum and inverse masses:
 // parameters and the pose of its bodies, constant within a sub-step.
 struct JointIn {
