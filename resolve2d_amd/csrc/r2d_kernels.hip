@@ -1914,11 +1914,15 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
 // GLOBAL: body state and counters in global memory (worlds whose touched
 // bodies do not fit LDS): counters are polled with acquire loads and
 // released with release stores, and the state is read after the acquire.
-template <bool COMPACT = false, bool GLOBAL = false, bool WARM = false, class ST>
+// HYBRID (with GLOBAL): the state stays in global memory but the counters
+// are in LDS (lcnt) under the compact chain indices, so a poll is an LDS
+// read; acquire / release at workgroup scope order the state around them.
+template <bool COMPACT = false, bool GLOBAL = false, bool WARM = false, bool HYBRID = false, class ST>
 __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
-                                                 uint32_t& s_abort)
+                                                 uint32_t& s_abort, uint32_t* lcnt = nullptr)
 {
+    static_assert(!HYBRID || (GLOBAL && !COMPACT), "the LDS counters go with global-memory state");
     const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
     uint4* slot = lanes + (size_t)threadIdx.x * LANE_ROWS;
     uint32_t pos = p_begin, k = (p_begin < nitems) ? 0u : iters;
@@ -1926,7 +1930,9 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
     uint32_t flags = 0, a = 0, b = 0, rk0 = 0, rk1 = 0, degs = 0;
     int32_t r0 = -1, r1 = -1;
     uint32_t t0 = 0, t1 = 0, n0 = 0, n1 = 0;  // the counters' targets and released values
-    uint32_t *p0 = st.cnt, *p1 = st.cnt;     // the counters polled (body 0's when none)
+    uint32_t* const cbase = HYBRID ? lcnt : st.cnt;
+    uint32_t *p0 = cbase, *p1 = cbase;       // the counters polled (body 0's when none)
+    uint32_t cpair = 0;                      // HYBRID: the compact indices of a and b
     float acc[4] = {0.f, 0.f, 0.f, 0.f};     // the held record's impulses
     bool have = false;
     uint32_t spins = 0;
@@ -1940,6 +1946,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
 #pragma unroll
                 for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row] = q[REC_LINK + row];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
+                if (HYBRID) cpair = h1.w;
                 if (COMPACT) {
                     const uint32_t c0 = h1.w & 0xffffu, c1 = h1.w >> 16;
                     a = c0;
@@ -1965,8 +1972,13 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             }
             n0 = t0 + 1u;
             n1 = t1 + 1u;
-            p0 = st.cnt + (r0 >= 0 ? r0 : 0);
-            p1 = st.cnt + (r1 >= 0 ? r1 : 0);
+            if (HYBRID) {
+                p0 = cbase + (r0 >= 0 ? (cpair & 0xffffu) : 0u);
+                p1 = cbase + (r1 >= 0 ? (cpair >> 16) : 0u);
+            } else {
+                p0 = cbase + (r0 >= 0 ? r0 : 0);
+                p1 = cbase + (r1 >= 0 ? r1 : 0);
+            }
             have = true;
         }
         // a poll reads the two counters only (a lane without an ordering
@@ -1976,7 +1988,10 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
         // released)
         const bool contact = !(flags & RF_JOINT);
         bool ready;
-        if (GLOBAL) {
+        if (HYBRID) {
+            ready = r0 < 0 || __hip_atomic_load(p0, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == t0;
+            if (ready && r1 >= 0) ready = __hip_atomic_load(p1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == t1;
+        } else if (GLOBAL) {
             ready = r0 < 0 || ld_acq(p0) == t0;
             if (ready && r1 >= 0) ready = ld_acq(p1) == t1;
         } else {
@@ -2004,7 +2019,10 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             }
             // the state writes above reach LDS before the counters (in order)
             lds_order();
-            if (GLOBAL) {
+            if (HYBRID) {
+                if (r0 >= 0) __hip_atomic_store(p0, n0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (r1 >= 0) __hip_atomic_store(p1, n1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else if (GLOBAL) {
                 if (r0 >= 0) st_rel(p0, n0);
                 if (r1 >= 0) st_rel(p1, n1);
             } else {
@@ -2375,11 +2393,20 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
         }
     } else {
         (void)lanes;
-        // body state and counters in global memory; the lane slots take the
-        // start of LDS, which this path does not otherwise use
+        // body state in global memory; the lane slots take the start of LDS,
+        // then the counters under the compact indices when they fit (the
+        // compact indices exist below 65,535 touched bodies), else the
+        // counters stay in global memory too
         const SolverState<false> st = global_state(S, D);
-        item_chain_lanes<false, true, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
-                                      s_abort);
+        uint4* slots = reinterpret_cast<uint4*>(smem);
+        if (nc < 0xffffu && (size_t)nc * 4u + LANE_BYTES <= COMPACT_LDS) {
+            uint32_t* lcnt = smem + (size_t)MAXT * LANE_WORDS;
+            for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) lcnt[c] = 0u;
+            __syncthreads();
+            item_chain_lanes<false, true, WARM, true>(S, st, recs, slots, nitems, iters, sub_dt, w, s_abort, lcnt);
+        } else {
+            item_chain_lanes<false, true, WARM>(S, st, recs, slots, nitems, iters, sub_dt, w, s_abort);
+        }
     }
 }
 
@@ -3529,7 +3556,8 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
                 hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 1, s, sub_steps,
                                    sub_dt);
                 // worlds whose touched bodies fit LDS, then those that do not
-                constexpr size_t GLOBAL_LDS = (size_t)BIG_GLOBAL_THREADS * LANE_WORDS * 4u;
+                // the second launch's LDS: its lane slots, then the counters
+                constexpr size_t GLOBAL_LDS = COMPACT_LDS;
                 if (sp.any_warm) {
                     hipLaunchKernelGGL((k_big_chain<256, true, 0>), dim3(S.nworlds), dim3(256), COMPACT_LDS, sb, S,
                                        sp.lds_nmax, iters, sub_dt);
