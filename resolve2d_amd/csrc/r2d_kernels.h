@@ -45,6 +45,7 @@ struct SolvePlan {
     uint32_t any_slot = 0, any_lds = 0, any_big = 0;
     uint32_t any_colored = 0;  // some world uses the coloured schedule
     uint32_t any_warm = 0;     // some world is warm started
+    uint32_t n_big = 0;        // worlds beyond LDS
 };
 // lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
 // many as LDS holds); larger worlds use the global-memory record kernel

@@ -2228,8 +2228,8 @@ __global__ void k_big_bodies(StepDev S, uint32_t lds_nmax, int phase, uint32_t s
     }
 }
 
-// the per-item parts, BIG_SPLIT workgroups per world (gridDim.y): phase 0
-// builds the records, 1 is updateTGSDepth (sub-step s), 2 writes back
+// the per-item parts, gridDim.y workgroups per world: phase 0 builds the
+// records, 1 is updateTGSDepth (sub-step s), 2 writes back
 constexpr unsigned BIG_THREADS = 256;
 constexpr unsigned BIG_SPLIT = 32;
 __global__ __launch_bounds__(BIG_THREADS) void k_big_item_pass(StepDev S, uint32_t lds_nmax, int phase, uint32_t s,
@@ -3444,7 +3444,7 @@ SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t l
         uint32_t n = nbodies[i];
         if (p.slot_cap && n <= p.slot_nmax) { p.any_slot = 1; p.slot_maxn = std::max(p.slot_maxn, n); }
         if (n <= p.lds_nmax) { p.any_lds = 1; p.lds_maxn = std::max(p.lds_maxn, n); }
-        else p.any_big = 1;
+        else { p.any_big = 1; p.n_big += 1; }
     }
     return p;
 }
@@ -3547,7 +3547,13 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             const hipStream_t sb = ss.aux[1];
             const float sub_dt = dt / (float)sub_steps;
             const unsigned gb = blocks_for(S.nb, BS);
-            const dim3 gi(S.nworlds, BIG_SPLIT);
+            // the per-item passes: up to BIG_SPLIT workgroups per world beyond
+            // LDS, fewer when there are many of them or many other worlds
+            // (every world gets the same count and the others exit at once)
+            const uint32_t split = std::max<uint32_t>(
+                1u, std::min<uint32_t>({BIG_SPLIT, 256u / std::max<uint32_t>(sp.n_big, 1u),
+                                        8192u / std::max<uint32_t>(S.nworlds, 1u)}));
+            const dim3 gi(S.nworlds, split);
             hipLaunchKernelGGL(k_big_bodies, dim3(gb), dim3(BS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
             hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 0, 0u, sub_steps, sub_dt);
             hipLaunchKernelGGL(k_big_items, dim3(S.nworlds), dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax);
