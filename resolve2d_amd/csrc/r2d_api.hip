@@ -112,6 +112,7 @@ struct r2d_world {
     uint64_t epoch = 0;
     uint32_t batch_index = 0;
     bool fused_src = false;   // ... computed by the fused kernel (their range: w_man_begin / w_man_end)
+    bool prestepped = true;   // ... by a step with sub-steps (a step without runs no preStep)
     // capacities of the world's slices of the fused step scratch (cell
     // entries, raw candidates, manifolds, solver items); 0: initial estimate
     uint32_t fz[4] = {0, 0, 0, 0};
@@ -1104,6 +1105,7 @@ int batch_finish(r2d_ctx* c, Batch& b)
         b.worlds[i]->epoch = c->epoch;
         b.worlds[i]->batch_index = (uint32_t)i;
         b.worlds[i]->fused_src = b.fused;
+        b.worlds[i]->prestepped = b.sub > 0;
     }
     return b.first_err;
 }
@@ -1828,8 +1830,13 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
         o.ref_id = w->u(F_ID)[ref[i] - wb[0]];
         o.inc_id = w->u(F_ID)[inc[i] - wb[0]];
         o.normal_x = nx[i]; o.normal_y = ny[i];
-        o.tangent_x = ny[i]; o.tangent_y = -nx[i];
-        o.friction = fr[i];
+        // preStep (collision.zig:102-133) sets the tangent, the friction and
+        // the points' effective masses; after a step without sub-steps they
+        // keep their initial zeros (the kernels precompute them at the
+        // record build regardless)
+        const bool pre = w->prestepped;
+        o.tangent_x = pre ? ny[i] : 0.0f; o.tangent_y = pre ? -nx[i] : 0.0f;
+        o.friction = pre ? fr[i] : 0.0f;
         for (int p = 0; p < 2; ++p) {
             r2d_manifold_point& q = o.points[p];
             q.valid = (valid[i] >> p) & 1u;
@@ -1839,7 +1846,7 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
             q.ref_r_x = F(MP_R1X); q.ref_r_y = F(MP_R1Y); q.inc_r_x = F(MP_R2X); q.inc_r_y = F(MP_R2Y);
             q.pos_x = F(MP_POSX); q.pos_y = F(MP_POSY);
             q.depth = F(MP_DEPTH); q.original_depth = F(MP_ODEPTH);
-            q.mass_n = F(MP_MASS_N); q.mass_t = F(MP_MASS_T);
+            q.mass_n = pre ? F(MP_MASS_N) : 0.0f; q.mass_t = pre ? F(MP_MASS_T) : 0.0f;
         }
     }
     return R2D_OK;

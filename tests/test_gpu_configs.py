@@ -106,7 +106,9 @@ def test_zero_substeps_and_iterations(gpu_ctx, oracle):
     for sub, it in ((0, 4), (4, 0), (1, 1), (3, 2)):
         step_both(g, o, 2, sub=sub, iters=it)
         assert_same(g, o, (sub, it))
-    assert bits_equal(g.manifolds(), o.manifolds())
+        # after a step without sub-steps no preStep ran: tangent, friction and
+        # effective masses read back as their initial zeros
+        assert bits_equal(g.manifolds(), o.manifolds()), (sub, it)
 
 
 def test_invalid_constraint_id_is_an_error(gpu_ctx, oracle):

@@ -136,3 +136,32 @@ def test_warm_tables_survive_growth(gpu_ctx, oracle):
     _same(g, o, "after growth")
     g.close()
     o.close()
+
+
+def test_warm_edge_steps_and_worlds(gpu_ctx, oracle):
+    """Warm-started worlds through the step shapes the reference allows: no
+    sub-steps (nothing carried is applied), no iterations (only the carried
+    impulses act), one sub-step; and worlds with no bodies or only static
+    ones.  Every step is compared with the oracle's warm starting."""
+    from resolve2d_amd.scenes import _Builder
+    shapes = [(4, 4), (4, 4), (0, 4), (4, 0), (1, 1), (4, 4), (2, 3), (4, 4)]
+    g, o = _pair(gpu_ctx, oracle, scenes.mixed(120, seed=33, width=10.0, height=20.0, ground_w=20.0), 0.9)
+    print("warm world: 120 bodies through sub-step / iteration edge cases", flush=True)
+    for _ in range(10):
+        assert g.process(DT, 4, 4) == 0 and o.process(DT, 4, 4) == 0
+    for sub, it in shapes:
+        assert g.process(DT, sub, it) == 0 and o.process(DT, sub, it) == 0
+        _same(g, o, (sub, it))
+    g.close()
+    o.close()
+    static_only = _Builder()
+    static_only.gravity(9.81)
+    static_only.static(static_only.rect(0.0, 0.0, 5.0, 1.0))
+    static_only.static(static_only.rect(0.0, 0.9, 1.0, 1.0))
+    for ops in ([], static_only.ops):
+        g, o = _pair(gpu_ctx, oracle, ops, 0.8)
+        for _ in range(3):
+            assert g.process(DT, 4, 4) == 0 and o.process(DT, 4, 4) == 0
+        _same(g, o, ("static or empty", len(ops)))
+        g.close()
+        o.close()
