@@ -46,6 +46,8 @@ struct SolvePlan {
     uint32_t any_colored = 0;  // some world uses the coloured schedule
     uint32_t any_warm = 0;     // some world is warm started
     uint32_t n_big = 0;        // worlds beyond LDS
+    uint32_t n_lds = 0;        // worlds of the record kernel with LDS body state
+    uint32_t rec_wide = 0;     // they take the 512-lane record kernel (few worlds, each alone on a CU)
 };
 // lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
 // many as LDS holds); larger worlds use the global-memory record kernel

@@ -1871,7 +1871,9 @@ constexpr uint32_t LANE_WORDS = LANE_ROWS * 4;
 static_assert(REC_LINK == 2 && REC_ACC == 7 && LANE_ROWS == 5, "lane_load reads rows REC_LINK..REC_ACC-1 in this order");
 // the record kernel takes the lane slots up to this workgroup size (larger
 // ones keep the chain's records in registers)
-constexpr uint32_t LANES_MAXT = 256;
+constexpr uint32_t LANES_MAXT = 512;
+// the record kernel's workgroup when a batch holds few worlds with LDS body state
+constexpr int REC_WIDE_THREADS = 512;
 __host__ __device__ constexpr size_t rec_lane_bytes(uint32_t threads)
 {
     return threads <= LANES_MAXT ? (size_t)threads * LANE_WORDS * 4u : 0u;
@@ -3443,9 +3445,18 @@ SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t l
     for (size_t i = 0; i < nw; ++i) {
         uint32_t n = nbodies[i];
         if (p.slot_cap && n <= p.slot_nmax) { p.any_slot = 1; p.slot_maxn = std::max(p.slot_maxn, n); }
-        if (n <= p.lds_nmax) { p.any_lds = 1; p.lds_maxn = std::max(p.lds_maxn, n); }
+        if (n <= p.lds_nmax) { p.any_lds = 1; p.lds_maxn = std::max(p.lds_maxn, n); p.n_lds += 1; }
         else { p.any_big = 1; p.n_big += 1; }
     }
+    // A few worlds, each alone on a CU, are latency-bound on their chains:
+    // more lanes hold fewer positions each and wait less (4,096-body world:
+    // 0.71 -> 0.58 ms per step with 512 lanes).  Many worlds share CUs and
+    // are issue-bound: the variant's shape.  The wide kernel's lane slots
+    // leave LDS for fewer bodies.
+    const uint32_t wide_nmax = (uint32_t)((160u * 1024u - 1024u - rec_lane_bytes(REC_WIDE_THREADS)) /
+                                          solve_lds_bytes(1));
+    p.rec_wide = (p.any_lds && !any_warm && p.n_lds <= 64u && p.lds_maxn <= wide_nmax &&
+                  kSolveVariants[p.variant].threads < REC_WIDE_THREADS) ? 1u : 0u;
     return p;
 }
 
@@ -3580,7 +3591,11 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             hipLaunchKernelGGL(k_big_item_pass, gi, dim3(BIG_THREADS), 0, sb, S, sp.lds_nmax, 2, 0u, sub_steps, sub_dt);
             tmark(ss, "k_big (global state)", sb);
         }
-        if (run_lds) {
+        if (run_lds && sp.rec_wide) {
+            hipLaunchKernelGGL((k_solve<REC_WIDE_THREADS, 1>), dim3(S.nworlds), dim3(REC_WIDE_THREADS),
+                               solve_lds_bytes((int)sp.lds_maxn) + rec_lane_bytes(REC_WIDE_THREADS), ss.aux[0], S, dt,
+                               sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
+        } else if (run_lds) {
             hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads),
                                solve_lds_bytes((int)sp.lds_maxn) + rec_lane_bytes(v.threads),
                                ss.aux[0], S, dt, sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
@@ -3615,6 +3630,8 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
 void set_solve_smem_limit()
 {
     (void)hipFuncSetAttribute((const void*)k_big_chain<256, false, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_solve<REC_WIDE_THREADS, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_big_chain<256, true, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
