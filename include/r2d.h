@@ -106,6 +106,9 @@ typedef struct r2d_manifold_state {
 /* --------------------------------------------------------------- context */
 /* Creates a context on HIP device `device` with its own stream. */
 int r2d_ctx_create(int device, r2d_ctx** out);
+/* Destroy order: servers (r2d_serve_destroy), then worlds
+ * (r2d_world_destroy), then the context.  R2D_E_BUSY while a server is
+ * attached or a world created on the context is not yet destroyed. */
 int r2d_ctx_destroy(r2d_ctx* ctx);
 /* Waits for all device work of the context. */
 int r2d_ctx_sync(r2d_ctx* ctx);
@@ -164,10 +167,13 @@ int r2d_exclude_collision_pair(r2d_world* w, uint32_t id1, uint32_t id2);
  * unstepped). */
 int r2d_process(r2d_world* w, float dt, size_t sub_steps, size_t collision_iters);
 
-/* Steps n independent worlds of one context together, `steps` times, as one
- * device launch sequence per step.  Equivalent to calling r2d_process on
- * every world `steps` times.  The first failing world's status is returned;
- * other worlds are stepped regardless. */
+/* Steps n independent worlds of one context together, `steps` times.
+ * Equivalent to calling r2d_process on every world `steps` times.  A batch
+ * whose worlds all have at most 1,024 bodies, the reference order and hash
+ * tables that fit LDS is one device launch for all steps of all worlds (one
+ * workgroup per world, r2d_ctx_set_fused); other batches take one launch
+ * sequence per step (replayed as a hipGraph when it repeats).  The first
+ * failing world's status is returned; other worlds are stepped regardless. */
 int r2d_process_batch(r2d_ctx* ctx, r2d_world* const* worlds, size_t n, float dt, size_t sub_steps,
                       size_t collision_iters, size_t steps);
 
@@ -259,6 +265,12 @@ int r2d_ctx_set_solve_variant(r2d_ctx* ctx, int variant);
  * = as many as LDS holds.  Results are identical either way; this is a
  * launch-shape knob for the autotuner. */
 int r2d_ctx_set_lds_limit(r2d_ctx* ctx, uint32_t max_bodies);
+/* Phase kernels only: a batch of at most 64 worlds whose solver body state
+ * fits LDS takes the record kernel with 512 lanes per world instead of the
+ * variant's shape (on, the default: such worlds are alone on their CUs and
+ * latency-bound).  Off keeps the variant's record kernel.  Same bits either
+ * way; a launch-shape knob with no counterpart in the reference. */
+int r2d_ctx_set_wide_record(r2d_ctx* ctx, int on);
 /* Totals of the last step of the context: entries, raw candidates,
  * manifolds, solver items (out[4]). */
 int r2d_ctx_last_counts(r2d_ctx* ctx, uint64_t* out);

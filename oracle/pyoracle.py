@@ -29,7 +29,26 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    _lib = _bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def build_planted(plant: int, out_dir: str) -> str:
+    """Compile the oracle with the deliberate mistake number `plant`
+    (R2D_PLANT in r2d_oracle_impl.h) into out_dir; tests use it to show
+    that their checks catch that mistake."""
+    path = os.path.join(out_dir, f"libr2d_oracle_plant{plant}.so")
+    subprocess.run(["gcc", "-std=gnu11", "-O1", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-shared",
+                    f"-DR2D_PLANT={plant}", "-o", path, os.path.join(HERE, "r2d_oracle.c"), "-lm"], check=True)
+    return path
+
+
+def load_path(path: str):
+    """A separately built oracle library (see build_planted)."""
+    return _bind(ctypes.CDLL(path))
+
+
+def _bind(lib):
     D, U32, I, P, I64, U64 = ctypes.c_double, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64
     for pre in ("or32_", "or64_"):
         sig = {
@@ -58,15 +77,14 @@ def load():
     lib.or_sinf.argtypes = [ctypes.c_float]
     lib.or_cosf.restype = ctypes.c_float
     lib.or_cosf.argtypes = [ctypes.c_float]
-    _lib = lib
     return lib
 
 
 class OracleWorld:
     """Reference step restated in C; precision "f32" (bit-level reference) or "f64"."""
 
-    def __init__(self, precision="f32", cell_width=4.0, table_mult=2):
-        self.lib = load()
+    def __init__(self, precision="f32", cell_width=4.0, table_mult=2, lib=None):
+        self.lib = lib if lib is not None else load()
         self.p = "or32_" if precision == "f32" else "or64_"
         self.h = self._f("world_new")(cell_width, table_mult)
 

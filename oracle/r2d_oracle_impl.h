@@ -17,7 +17,20 @@
  *   - sin/cos: the musl/msun method in r2d_trig_ref.h (f32); libm in f64.
  *
  * Every function cites the reference file:line it restates.
+ *
+ * R2D_PLANT (default 0) compiles a deliberately wrong variant of one formula.
+ * It exists only so tests/test_oracle_spring_offset.py can show that its
+ * checks fail when the restatement is wrong; the library built by the
+ * Makefile always has R2D_PLANT = 0.
+ *   1  OffsetDistanceJoint: body 1's angular impulse with the wrong sign
+ *   2  OffsetDistanceJoint: lever arms not rotated into the world frame
+ *   3  OffsetDistanceJoint: anchors placed without the body's rotation
+ *   4  anchored spring: damping term dropped
+ *   5  anchored spring: damping on momentum instead of velocity
  */
+#ifndef R2D_PLANT
+#define R2D_PLANT 0
+#endif
 
 #ifndef R2D_REAL
 #error "define R2D_REAL and R2D_P before including"
@@ -842,8 +855,13 @@ static void F(solve_distance)(const F(constraint) *c, F(body) *b1, F(body) *b2)
 static void F(solve_offset)(const F(constraint) *c, F(body) *b1, F(body) *b2)
 {
     R2D_REAL eps = K_GRAD_DIV_LIMIT;
+#if R2D_PLANT == 3
+    F(vec2) a1 = F(add2)(c->r1, b1->pos);
+    F(vec2) a2 = F(add2)(c->r2, b2->pos);
+#else
     F(vec2) a1 = F(add2)(F(rotate2)(c->r1, b1->angle), b1->pos);   /* localToWorld, RigidBody.zig:113-117 */
     F(vec2) a2 = F(add2)(F(rotate2)(c->r2, b2->angle), b2->pos);
+#endif
     F(vec2) n = F(normalize2)(F(sub2)(a2, a1));
     R2D_REAL im1 = b1->is_static ? 0 : ((R2D_REAL)1 / b1->mass);
     R2D_REAL im2 = b2->is_static ? 0 : ((R2D_REAL)1 / b2->mass);
@@ -853,10 +871,18 @@ static void F(solve_offset)(const F(constraint) *c, F(body) *b1, F(body) *b2)
     R2D_REAL w1 = b1->L * ii1;
     F(vec2) vl2 = F(scale2)(b2->mom, im2);
     R2D_REAL w2 = b2->L * ii2;
+#if R2D_PLANT == 2
+    F(vec2) rr1 = c->r1;
+#else
     F(vec2) rr1 = F(rotate2)(c->r1, b1->angle);
+#endif
     F(vec2) va1 = F(v2)(-rr1.y * w1, rr1.x * w1);
     F(vec2) v1 = F(add2)(vl1, va1);
+#if R2D_PLANT == 2
+    F(vec2) rr2 = c->r2;
+#else
     F(vec2) rr2 = F(rotate2)(c->r2, b2->angle);
+#endif
     F(vec2) va2 = F(v2)(-rr2.y * w2, rr2.x * w2);
     F(vec2) v2v = F(add2)(vl2, va2);
     F(vec2) dv = F(sub2)(v2v, v1);
@@ -873,7 +899,11 @@ static void F(solve_offset)(const F(constraint) *c, F(body) *b1, F(body) *b2)
     J = F(clampn)(J, jmin, jmax);
     F(vec2) dp = F(scale2)(n, J);
     b1->mom.x -= dp.x; b1->mom.y -= dp.y;
+#if R2D_PLANT == 1
+    b1->L += J * r1xn;
+#else
     b1->L -= J * r1xn;
+#endif
     b2->mom.x += dp.x; b2->mom.y += dp.y;
     b2->L += J * r2xn;
 }
@@ -949,8 +979,16 @@ static void F(apply_gen)(const F(forcegen) *g, F(body) *b)
     R2D_REAL inv_m = (R2D_REAL)1 / b->mass;
     F(vec2) v = F(scale2)(b->mom, inv_m);
     F(vec2) d = F(sub2)(b->pos, g->anchor);
+#if R2D_PLANT == 4
+    b->force.x += (-g->k) * d.x;
+    b->force.y += (-g->k) * d.y;
+#elif R2D_PLANT == 5
+    b->force.x += (-g->k) * d.x - g->c * b->mom.x;
+    b->force.y += (-g->k) * d.y - g->c * b->mom.y;
+#else
     b->force.x += (-g->k) * d.x - g->c * v.x;
     b->force.y += (-g->k) * d.y - g->c * v.y;
+#endif
 }
 
 /* Coloured schedule (extension, no reference counterpart).  Items are the

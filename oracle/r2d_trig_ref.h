@@ -14,6 +14,20 @@
  * Payne-Hanek reduction (__rem_pio2_large, prec=0).  This restatement uses an
  * exact 128-bit reduction there instead, so results in that range are checked
  * only against FP64 sin/cos (<= 1 ulp) -- parity unpinned for that range.
+ *
+ * The sin/cos kernels, their coefficients and the reduction thresholds below
+ * follow FreeBSD msun (k_sinf.c, k_cosf.c, e_rem_pio2f.c, s_sinf.c, s_cosf.c;
+ * as carried by musl), which carry this notice:
+ *
+ *   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+ *
+ *   Developed at SunPro, a Sun Microsystems, Inc. business.
+ *   Permission to use, copy, modify, and distribute this
+ *   software is freely granted, provided that this notice
+ *   is preserved.
+ *
+ *   (float versions: conversion to float by Ian Lance Taylor, Cygnus
+ *   Support; optimized by Bruce D. Evans.)
  */
 #ifndef R2D_TRIG_REF_H
 #define R2D_TRIG_REF_H

@@ -143,6 +143,7 @@ SIGNATURES = {
     "r2d_ctx_set_solve_variant": [P, I32],
     "r2d_ctx_set_lds_limit": [P, U32],
     "r2d_ctx_set_fused": [P, I32],
+    "r2d_ctx_set_wide_record": [P, I32],
     "r2d_ctx_fused_phase_times": [P, ctypes.POINTER(ctypes.c_double)],
     "r2d_ctx_fused_counts": [P, ctypes.POINTER(ctypes.c_uint64)],
     "r2d_ctx_fused_times": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),

@@ -71,6 +71,7 @@ struct HGen {
 
 struct r2d_world {
     r2d_ctx* ctx = nullptr;
+    r2d_ctx* owner = nullptr;   // the context that created it (ctx moves to a serving lane while served)
     uint32_t current_body_id = 0;
     int32_t n = 0;
     size_t cap = 0;
@@ -130,6 +131,7 @@ struct r2d_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;   // brackets of the last launched batch
     hipEvent_t ev_kend = nullptr;                      // end of the last fused kernel
     r2d_server* server = nullptr;
+    uint64_t live_worlds = 0;   // worlds created on this context and not yet destroyed
     // step scratch
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -159,6 +161,7 @@ struct r2d_ctx {
     std::vector<WorldDev> uploaded_descs;
     int solve_variant = -1;                       // -1: choose by world size
     uint32_t lds_nmax = 0;                        // 0: LDS body state up to the LDS capacity
+    bool wide_record = true;                      // batches of <= 64 LDS worlds may take the 512-lane record kernel
 
     KernelTrace* trace = nullptr;                 // per-kernel timing (R2D_TRACE_KERNELS)
 
@@ -827,7 +830,7 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
     }
     // warm starting is compiled into one variant's kernels only
     if (warm) variant = WARM_VARIANT;
-    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored, warm);
+    b.plan = plan_solve(variant, sizes.data(), nw, c->lds_nmax, colored, warm, c->wide_record);
     b.variant = variant;
     // whole steps per workgroup: worlds of at most FUSED_NMAX bodies in the
     // reference order whose broadphase and solve arrays fit LDS
@@ -1457,6 +1460,8 @@ R2D_EXPORT int r2d_ctx_create(int device, r2d_ctx** out)
 R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
 {
     if (!c) return R2D_E_INVALID_ARG;
+    // a server's lanes and the worlds of this context refer to it
+    if (c->server || c->live_worlds) return R2D_E_BUSY;
     (void)hipStreamSynchronize(c->stream);
     if (c->trace) {
         c->trace->report(stderr);
@@ -1500,6 +1505,8 @@ R2D_EXPORT int r2d_world_create(r2d_ctx* c, float cell, size_t mult, r2d_world**
     r2d_world* w = new (std::nothrow) r2d_world();
     if (!w) return R2D_E_OOM;
     w->ctx = c;
+    w->owner = c;
+    c->live_worlds += 1;
     w->cell_arg = cell;
     w->mult_arg = mult;
     *out = w;
@@ -1518,6 +1525,7 @@ R2D_EXPORT int r2d_world_destroy(r2d_world* w)
     if (w->dwtab) (void)hipFree(w->dwtab);
     if (w->dwcnt) (void)hipFree(w->dwcnt);
     if (w->dwsum) (void)hipFree(w->dwsum);
+    w->owner->live_worlds -= 1;
     delete w;
     return R2D_OK;
 }
@@ -2010,6 +2018,13 @@ R2D_EXPORT int r2d_ctx_set_lds_limit(r2d_ctx* c, uint32_t max_bodies)
 {
     if (!c) return R2D_E_INVALID_ARG;
     c->lds_nmax = max_bodies;
+    return R2D_OK;
+}
+
+R2D_EXPORT int r2d_ctx_set_wide_record(r2d_ctx* c, int on)
+{
+    if (!c) return R2D_E_INVALID_ARG;
+    c->wide_record = on != 0;
     return R2D_OK;
 }
 

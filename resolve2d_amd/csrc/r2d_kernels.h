@@ -52,7 +52,7 @@ struct SolvePlan {
 // lds_limit: largest world (bodies) that keeps its body state in LDS (0: as
 // many as LDS holds); larger worlds use the global-memory record kernel
 SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored,
-                     bool any_warm);
+                     bool any_warm, bool allow_wide = true);
 
 // Optional per-kernel timing (a context created with R2D_TRACE_KERNELS set):
 // an event after every launch, summed per kernel name; the steps run eagerly.

@@ -3431,7 +3431,7 @@ static inline void tmark(const Streams& ss, const char* name, hipStream_t st)
 }
 
 SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t lds_limit, bool any_colored,
-                     bool any_warm)
+                     bool any_warm, bool allow_wide)
 {
     SolvePlan p;
     p.any_colored = any_colored ? 1u : 0u;
@@ -3455,7 +3455,7 @@ SolvePlan plan_solve(int variant, const uint32_t* nbodies, size_t nw, uint32_t l
     // leave LDS for fewer bodies.
     const uint32_t wide_nmax = (uint32_t)((160u * 1024u - 1024u - rec_lane_bytes(REC_WIDE_THREADS)) /
                                           solve_lds_bytes(1));
-    p.rec_wide = (p.any_lds && !any_warm && p.n_lds <= 64u && p.lds_maxn <= wide_nmax &&
+    p.rec_wide = (allow_wide && p.any_lds && !any_warm && p.n_lds <= 64u && p.lds_maxn <= wide_nmax &&
                   kSolveVariants[p.variant].threads < REC_WIDE_THREADS) ? 1u : 0u;
     return p;
 }
@@ -3595,6 +3595,7 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
             hipLaunchKernelGGL((k_solve<REC_WIDE_THREADS, 1>), dim3(S.nworlds), dim3(REC_WIDE_THREADS),
                                solve_lds_bytes((int)sp.lds_maxn) + rec_lane_bytes(REC_WIDE_THREADS), ss.aux[0], S, dt,
                                sub_steps, iters, sp.slot_cap, sp.slot_nmax, sp.lds_nmax);
+            tmark(ss, "k_solve (record, wide)", ss.aux[0]);
         } else if (run_lds) {
             hipLaunchKernelGGL(v.rec_fn, dim3(S.nworlds), dim3(v.threads),
                                solve_lds_bytes((int)sp.lds_maxn) + rec_lane_bytes(v.threads),

@@ -6,6 +6,20 @@
 // correctly rounded ones.  sin/cos follow the musl/msun method that Zig's
 // compiler_rt ports (double-precision reduction + __sindf/__cosdf kernels),
 // so host and GPU produce the same bits.
+//
+// The sin/cos kernels, their coefficients and the reduction thresholds below
+// follow FreeBSD msun (k_sinf.c, k_cosf.c, e_rem_pio2f.c, s_sinf.c, s_cosf.c;
+// as carried by musl), which carry this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
+//   (float versions: conversion to float by Ian Lance Taylor, Cygnus
+//   Support; optimized by Bruce D. Evans.)
 #pragma once
 
 #include <hip/hip_runtime.h>

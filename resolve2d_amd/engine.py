@@ -30,11 +30,13 @@ class GpuWorld:
         N.check(self.lib.r2d_world_create(self.ctx.handle, cell_width, table_mult, ctypes.byref(h)),
                 "r2d_world_create")
         self.handle = h
+        self.ctx.adopt(self)
 
     def close(self):
         if self.handle:
-            self.lib.r2d_world_destroy(self.handle)
+            N.check(self.lib.r2d_world_destroy(self.handle), "r2d_world_destroy")
             self.handle = None
+            self.ctx.release(self)
 
     # --- scene operations (same names/arguments as the oracle's engine)
     def _opts(self, px, py, vx, vy, angle, omega, mu, mass_kind, mass_val):

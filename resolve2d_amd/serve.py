@@ -32,11 +32,13 @@ class Server:
         h = ctypes.c_void_p()
         N.check(self.lib.r2d_serve_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)), "r2d_serve_create")
         self.handle = h
+        ctx.adopt(self, server=True)
 
     def close(self):
         if self.handle:
             N.check(self.lib.r2d_serve_destroy(self.handle), "r2d_serve_destroy")
             self.handle = None
+            self.ctx.release(self)
 
     def submit(self, world, steps: int) -> int:
         t = ctypes.c_uint64()

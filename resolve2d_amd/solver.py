@@ -34,10 +34,25 @@ class Context:
         N.check(self.lib.r2d_ctx_create(device, ctypes.byref(h)), "r2d_ctx_create")
         self.handle = h
         self.profiling = False
+        self._servers = {}   # open servers and worlds made on this context, by id()
+        self._worlds = {}
+
+    def adopt(self, obj, server: bool = False):
+        """Register an open world or server created on this context; close()
+        closes what is still open first (servers, then worlds), since the
+        library refuses to destroy a context that they still use."""
+        (self._servers if server else self._worlds)[id(obj)] = obj
+
+    def release(self, obj):
+        """obj has been closed."""
+        self._servers.pop(id(obj), None)
+        self._worlds.pop(id(obj), None)
 
     def close(self):
         if self.handle:
-            self.lib.r2d_ctx_destroy(self.handle)
+            for o in list(self._servers.values()) + list(self._worlds.values()):
+                o.close()
+            N.check(self.lib.r2d_ctx_destroy(self.handle), "r2d_ctx_destroy")
             self.handle = None
 
     def sync(self):
@@ -63,6 +78,11 @@ class Context:
 
     def set_lds_limit(self, max_bodies: int):
         N.check(self.lib.r2d_ctx_set_lds_limit(self.handle, max_bodies), "r2d_ctx_set_lds_limit")
+
+    def set_wide_record(self, on: bool):
+        """Phase kernels: batches of at most 64 LDS-state worlds take the
+        512-lane record kernel (default on).  Same bits either way."""
+        N.check(self.lib.r2d_ctx_set_wide_record(self.handle, 1 if on else 0), "r2d_ctx_set_wide_record")
 
     def set_fused(self, on: bool):
         """Whole steps per workgroup for batches of worlds up to 1,024 bodies
@@ -360,12 +380,18 @@ class Solver:
         h = ctypes.c_void_p()
         N.check(ctx.lib.r2d_world_create(ctx.handle, spatialhash_cell_width, spatialhash_table_size_mult,
                                          ctypes.byref(h)), "r2d_world_create")
-        return cls(h, ctx)
+        s = cls(h, ctx)
+        ctx.adopt(s)
+        return s
+
+    def close(self):
+        self.deinit()
 
     def deinit(self):
         if self.handle:
-            self.lib.r2d_world_destroy(self.handle)
+            N.check(self.lib.r2d_world_destroy(self.handle), "r2d_world_destroy")
             self.handle = None
+            self.ctx.release(self)
 
     def clear(self):
         N.check(self.lib.r2d_world_clear(self.handle), "r2d_world_clear")

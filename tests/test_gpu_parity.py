@@ -46,7 +46,39 @@ def test_device_math_bits(gpu_ctx, oracle):
     assert (out.view(np.uint32) == (pairs[:, 0] / pairs[:, 1]).view(np.uint32)).all()
 
 
+def _spring_oscillator(s):
+    """One disc on an anchored spring, no gravity, beside a hanging one."""
+    b = s._Builder()
+    i = b.disc(3.0, 1.0, 0.5, density=3.0, vx=0.5, vy=-1.0)
+    b.spring(i, 40.0, 0.7, 1.5, -2.0)
+    j = b.rect(-6.0, 0.0, 1.0, 0.5, angle=0.3, omega=0.5)
+    b.spring(j, 25.0, 2.0, -6.0, 4.0)
+    b.gravity(9.81)
+    return b.ops
+
+
+def _offset_pendulum(s):
+    """A box hung by one corner from a static pivot by an offset joint, with
+    a damper spring (k = 0) and a motor to omega = 0 on it, plus two free
+    bodies joined by an offset joint with both anchors off-centre."""
+    b = s._Builder()
+    b.gravity(9.81)
+    piv = b.disc(0.0, 10.0, 0.2)
+    b.static(piv)
+    box = b.rect(1.2, 7.6, 1.0, 0.5, angle=0.5)
+    b.joint(1, piv, box, r1=(0.0, 0.0), r2=(0.5, 0.25), target=2.0)
+    b.joint(3, box, target=0.0, beta=1.0)
+    b.spring(box, 0.0, 2.0, 0.0, 0.0)
+    b.exclude(piv, box)
+    i = b.rect(20.0, 5.0, 1.2, 0.6, angle=0.4, vx=0.3, vy=-0.2, omega=1.1, density=2.0)
+    j = b.disc(23.0, 6.0, 0.4, angle=-0.7, vx=-0.5, vy=0.6, omega=-2.0)
+    b.joint(1, i, j, r1=(0.5, 0.25), r2=(0.0, -0.35), target=2.0)
+    return b.ops
+
+
 SCENES = {
+    "spring_oscillator": (_spring_oscillator, 120),
+    "offset_pendulum": (_offset_pendulum, 240),
     "pyramid_5": (lambda s: s.pyramid(rows=5), 60),
     "pyramid_20": (lambda s: s.pyramid(rows=20), 40),
     "car": (lambda s: s.car_demo(), 60),
@@ -90,12 +122,28 @@ def test_batched_equals_single(gpu_ctx, oracle):
         w.close()
 
 
-@pytest.mark.parametrize("fused", [True, False], ids=["fused", "phases"])
-@pytest.mark.parametrize("variant", list(range(13)))
-def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, fused):
+def _variant_cases():
+    from resolve2d_amd.autotune import FUSED_VARIANTS
+    cases = []
+    for v in range(13):
+        for mode in ("fused", "phases"):
+            marks = ()
+            if mode == "fused" and v not in FUSED_VARIANTS:
+                marks = pytest.mark.skip(reason=f"variant {v} has no fused kernel (variant 8, the lockstep slot "
+                                                "walk, hung once in its fused form with no cause found, so that "
+                                                "form is not built); its phase-kernel form is tested")
+            cases.append(pytest.param(v, mode, id=f"{v}-{mode}", marks=marks))
+    cases.append(pytest.param(11, "phases-wide", id="11-phases-wide"))
+    return cases
+
+
+@pytest.mark.parametrize("variant,mode", _variant_cases())
+def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, mode):
     """Every launch configuration of the solve kernel (workgroup size per
     world), in the fused kernel where the variant has one and in the phase
-    kernels, gives the oracle's bits."""
+    kernels, gives the oracle's bits.  The phase-kernel cases turn off the
+    512-lane record kernel that small batches take by default, so the
+    variant's own record kernel runs; "phases-wide" runs that 512-lane one."""
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld, apply_scene
     specs = [scenes.batched_world(21, n_min=40, n_max=60), scenes.batched_world(22),
@@ -107,12 +155,14 @@ def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, fused):
         apply_scene(sc, g)
         apply_scene(sc, o)
     gpu_ctx.set_solve_variant(variant)
-    gpu_ctx.set_fused(fused)
+    gpu_ctx.set_fused(mode == "fused")
+    gpu_ctx.set_wide_record(mode == "phases-wide")
     try:
         gpu_ctx.process_batch_handles([g.handle for g in gw], scenes.DT, 4, 4, 60)
     finally:
         gpu_ctx.set_solve_variant(-1)
         gpu_ctx.set_fused(True)
+        gpu_ctx.set_wide_record(True)
     for o in ow:
         for _ in range(60):
             assert o.process(scenes.DT, 4, 4) == 0
