@@ -1,32 +1,41 @@
-"""Benchmark of the resolve2d step on one MI355X.
+"""Benchmark of the resolve2d step on one MI355X; prints one JSON line.
 
-Workload (the headline value): 1024 independent worlds, body counts uniform
-in [50, 500] (mixed discs / rectangles), each with 0-20 distance joints and
-0-10 anchored springs, stepped together (dt = 1/60, 4 sub-steps, 4 solver
-iterations).  A step is one Solver.process over every world of the batch;
-inputs are resident on the device before timing starts.
+Headline (configuration 3): 1024 independent worlds, body counts uniform in
+[50, 500] (mixed discs / rectangles), each with 0-20 distance joints and 0-10
+anchored springs, stepped together over a 240-step horizon (dt = 1/60,
+4 sub-steps, 4 solver iterations).  A step is one Solver.process over every
+world of the batch; inputs are resident on the device before timing starts.
 
-The timed region is one call of `steps` steps.  Worlds of at most 1,024
-bodies take whole steps per workgroup: one launch of the fused kernel runs
-every step of every world; `value` is bodies x steps / the call's wall time,
-`p50/p99_step_ms` are the world-steps' latencies (each workgroup's wall
-clock).  The launch shape (solve variant, fused or phase kernels) is picked
-by the autotuner on copies of the batch.
+The timed region is one call of `--steps` K steps taken from the settled end
+of the horizon: the batch is first advanced max(W, 240 - K) steps untimed
+(W = --warmup), so with K = 20 the window is steps 221-240, where the piles
+hold the most contacts.  Worlds of at most 1,024 bodies take whole steps per
+workgroup: one launch of the fused kernel runs every step of every world of
+the call; `value` is bodies x K / the call's wall time; `p50/p99_step_ms`
+are the world-steps' latencies (each workgroup's wall clock).  `horizon`
+reports the whole 240-step job (a fresh batch, all 240 steps in one call).
+The launch shape is picked by the autotuner on copies of the batch at the
+window's start.
 
-The printed JSON line also carries: the same scenes stepped one world per
-launch ("plain"); the 4,096-body single world in the reference order and in
-the coloured schedule; the 65,536-body constrained world; a served Poisson
-stream of scenes of 16-8192 bodies (continuous batching, p50/p99 step and
-request latency); accuracy (bit equality with the f32 oracle, error against
-an f64 run of the same method, maximum penetration, energy drift of the
-resting pyramid); a roofline entry for the dominant kernel (its launch timed
-with HIP events on the library's stream; HBM traffic from the committed
-rocprofv3 counter measurement; an LDS figure); a CPU baseline (the
-oracle restatement on a bounded sample, worlds in parallel over host cores);
-and the opt-in warm starting (the batch warm started: its throughput against
-the cold run; the resting pyramid warm against cold).
+Also in the line: the same scenes one world per launch over the same window
+and over the horizon ("plain", the reference's one-world process call); the
+4,096-body world (configuration 2, tuned hash of cell 2x the largest body
+extent) in the reference order and the coloured schedule with its FP64
+error, penetration and energy change; the 65,540-body constrained world
+(configuration 4) with its FP64 error at steps 1, 10, 120; a served Poisson
+stream of 240-step scenes of 16-8192 bodies (configuration 5: capacity from
+an overloaded stream, then a stream at 70 % of it; p50/p99 step and request
+latency; accuracy of sampled scenes); accuracy of the batch (bit equality
+with the f32 oracle, FP64 error, penetration) and of the resting 20-row
+pyramid (configuration 1: energy drift); the roofline of the dominant kernel
+(its launch timed with HIP events on the library's stream; HBM traffic from
+the committed rocprofv3 counters; the HBM copy rate measured in the same
+run); a CPU baseline (the oracle restatement, worlds in parallel over the
+host cores, median of 5 passes, for configurations 3 and 5); and the opt-in
+warm starting.
 """
 import argparse
+import ctypes
 import json
 import math
 import multiprocessing as mp
@@ -42,30 +51,27 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # LDS: 256 CUs x 128 B per clock x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
-# autotuner: samples of 30 steps from step 60 of a copy of the batch; the 3
-# fastest timed again over 120 steps from the same state
-TUNE_AT, TUNE_STEPS, TUNE_REFINE, TUNE_REFINE_STEPS = 60, 30, 3, 120
+HORIZON = 240           # steps of a configuration-3 world and of a served scene
 G = 9.81
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
-
+# host processes at most: the GPU box's CPU share for one GPU
+CPU_SHARE = 16
 
 # fused kernel shape of each solve variant: <slots, workgroup size, waves per SIMD>
 FUSED_SHAPES = {1: "4,256,3", 2: "2,512,3", 3: "3,256,3", 4: "4,256,2", 5: "8,128,2", 7: "4,256,3",
                 9: "0,256,3", 10: "0,256,4", 11: "0,192,3", 12: "0,128,2"}
 
 
-def traffic_for(kernel, variant, args):
-    """HBM bytes per launch of the kernel on this workload as counted by
-    rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
-    from the committed measurement; None when there is none for it."""
+def traffic_for(name, window, args):
+    """HBM bytes per launch of the kernel on this workload and window as
+    counted by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size), from the committed measurement; None when there is none."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    shape = FUSED_SHAPES.get(variant) if kernel == "k_fused" else None
-    name = f"{kernel}<{shape}>" if shape else kernel
-    return t.get(f"{name}/worlds{args.worlds}/steps{args.steps}/seed{args.seed}")
+    return t.get(f"{name}/worlds{args.worlds}/steps{window[0]}-{window[1]}/seed{args.seed}")
 
 
 def log(msg):
@@ -81,43 +87,59 @@ def count_bodies(sc):
     return sum(1 for op in sc if op[0] in ("disc", "rect"))
 
 
+def host_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 # ----------------------------------------------------------------- CPU baseline
-def _cpu_worker(args):
-    """One host core: the oracle (f32) on its share of the sampled worlds, each
-    over the full horizon, until its time budget is spent."""
-    specs, steps, budget = args
+def _cpu_worker(job):
+    """One host process: the f32 oracle stepping its share of the sample, every
+    world from its start over `steps` steps, `repeats` times; returns
+    (body-steps, seconds) per pass (world construction untimed)."""
+    specs, steps, repeats = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import apply_scene
-    body_steps, t_total, used = 0, 0.0, 0
-    for sc in specs:
-        o = pyoracle.OracleWorld("f32")
-        apply_scene(sc, o)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
-        t_total += time.perf_counter() - t0
-        body_steps += count_bodies(sc) * steps
-        used += 1
-        o.close()
-        if t_total > budget:
-            break
-    return body_steps, t_total, used
+    out = []
+    for _ in range(repeats):
+        bs, t = 0, 0.0
+        for sc in specs:
+            o = pyoracle.OracleWorld("f32")
+            apply_scene(sc, o)
+            a = time.perf_counter()
+            for _ in range(steps):
+                o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+            t += time.perf_counter() - a
+            bs += count_bodies(sc) * steps
+            o.close()
+        out.append((bs, t))
+    return out
 
 
-def cpu_baseline(specs, steps, budget_s, cores):
-    """Worlds in parallel over `cores` host processes (started before the GPU is
-    touched); each process steps whole worlds spread over the batch."""
-    order = list(range(0, len(specs), 7)) + [i for i in range(len(specs)) if i % 7]
-    shares = [[specs[order[j]] for j in range(k, len(order), cores)] for k in range(cores)]
-    t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(cores) as pool:
-        res = pool.map(_cpu_worker, [(sh, steps, budget_s) for sh in shares])
-    wall = time.perf_counter() - t0
-    body_steps = sum(r[0] for r in res)
-    busy = max(r[1] for r in res)
-    return body_steps / busy, sum(r[2] for r in res), busy, wall
+def cpu_baseline(specs, steps, cores, repeats):
+    """The sample's worlds in parallel over `cores` host processes (started
+    before the GPU is touched; shares balanced by body count), `repeats`
+    passes; a pass's rate is its body-steps over its slowest process's time.
+    Returns the median pass rate and every pass's rate."""
+    order = sorted(range(len(specs)), key=lambda i: -count_bodies(specs[i]))
+    shares, load = [[] for _ in range(cores)], [0] * cores
+    for i in order:
+        k = int(np.argmin(load))
+        shares[k].append(specs[i])
+        load[k] += count_bodies(specs[i])
+    shares = [s for s in shares if s]
+    with mp.get_context("fork").Pool(len(shares)) as pool:
+        res = pool.map(_cpu_worker, [(sh, steps, repeats) for sh in shares])
+    rates = []
+    for p in range(repeats):
+        bs = sum(r[p][0] for r in res)
+        t = max(r[p][1] for r in res)
+        rates.append(bs / t)
+    return float(np.median(rates)), rates, len(shares)
 
 
 # ------------------------------------------------------------------- accuracy
@@ -142,6 +164,10 @@ def state_error(gpu, ref):
     return dict(max_abs_pos_m=float(dp), max_abs_vel_ms=float(dv),
                 rel_pos=float(dp / max(1.0, np.abs(ref[dyn, 3:5]).max(initial=0.0))),
                 rel_vel=float(dv / max(1e-6, np.abs(v_r).max(initial=0.0))))
+
+
+def bits_equal(a, b):
+    return bool(np.array_equal(a.astype(np.float32).view(np.uint32), b.astype(np.float32).view(np.uint32)))
 
 
 def max_penetration(man):
@@ -179,9 +205,7 @@ def accuracy(ctx, specs, sample, steps, snaps):
             errs = []
             for i in range(len(sample)):
                 g = gw[i].bodies()
-                a32 = o32[i].bodies()
-                bit_equal &= bool((g.astype(np.float32).view(np.uint32) ==
-                                   a32.astype(np.float32).view(np.uint32)).all())
+                bit_equal &= bits_equal(g, o32[i].bodies())
                 errs.append(state_error(g, o64[i].bodies()))
             res[f"step_{s}"] = {k: max(e[k] for e in errs) for k in errs[0]}
     pen = max(max_penetration(w.manifolds()) for w in gw)
@@ -223,31 +247,85 @@ def pyramid_energy(ctx, steps):
     apply_scene(scenes.pyramid(20), g)
     apply_scene(scenes.pyramid(20), o)
     e0 = energy(g.bodies())
-    worst, gpu_s = 0.0, 0.0
+    worst, gpu_s, lat = 0.0, 0.0, []
     for s in range(1, steps + 1):
         a = time.perf_counter()
         checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "pyramid step")
-        gpu_s += time.perf_counter() - a
+        lat.append(time.perf_counter() - a)
         o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
         if s % 20 == 0 or s == steps:
             worst = max(worst, abs(energy(g.bodies()) - e0) / abs(e0))
     b, r = g.bodies(), o.bodies()
+    lat = np.array(lat) * 1e3
     out = dict(steps=steps, drift=(energy(b) - e0) / abs(e0), max_abs_drift=worst,
                drift_f64=(energy(r) - e0) / abs(e0), max_penetration_m=max_penetration(g.manifolds()),
-               vs_f64=state_error(b, r), gpu_ms_per_step=gpu_s / steps * 1e3,
+               vs_f64=state_error(b, r), gpu_ms_per_step=float(lat.mean()),
+               gpu_p50_step_ms=float(np.percentile(lat, 50)), gpu_p99_step_ms=float(np.percentile(lat, 99)),
                cpu_single=cpu_single(scenes.pyramid(20), steps, 5.0))
     g.close()
     o.close()
     return out
 
 
-def warm_start_runs(ctx, specs, args, cold_value, factor=0.8):
+def single_world(ctx, steps, tuned_hash):
+    """Configuration 2: the 4,096-body world over its horizon, one world per
+    call, in the reference order and in the coloured schedule; the tuned
+    spatial hash (cell = twice the largest body extent, 2N buckets) unless
+    the reference hash is asked for.  Each mode: throughput, per-call p50/p99,
+    its error against the f64 oracle run of the same mode at steps 1, 60 and
+    the last, the deepest contact and the change of mechanical energy."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    ops = scenes.mixed(4096, seed=1, hash_cell="auto" if tuned_hash else None)
+    out = {"hash": ({"mode": "tuned", "cell_width_m": ops[0][1], "table": "2N"} if tuned_hash else
+                    {"mode": "reference", "cell_width_m": 4.0, "table": "2N"})}
+    snaps = {1, 60, steps}
+    for mode, label in ((0, "reference_order"), (1, "colored")):
+        w = GpuWorld(ctx)
+        o = pyoracle.OracleWorld("f64")
+        for e in (w, o):
+            apply_scene(ops, e)
+            if mode:
+                e.set_schedule(1)
+        e0 = energy(w.bodies())
+        n = w.bodies().shape[0]
+        ctx.sync()
+        lat, errs, pen, o_steps = [], {}, 0.0, 0
+        for i in range(1, steps + 1):
+            a = time.perf_counter()
+            checked(w.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), f"single-world step ({label})")
+            lat.append(time.perf_counter() - a)
+            if i in snaps:
+                g = w.bodies()
+                while o_steps < i:
+                    o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+                    o_steps += 1
+                errs[f"step_{i}"] = state_error(g, o.bodies())
+                pen = max(pen, max_penetration(w.manifolds()))
+        t1 = float(np.sum(lat))
+        lat = np.array(lat) * 1e3
+        out[label] = {"value": n * steps / t1, "unit": "body-steps/s", "ms_per_step": t1 / steps * 1e3,
+                      "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+                      "vs_f64_oracle": errs, "max_penetration_m": pen,
+                      "energy_change": (energy(w.bodies()) - e0) / abs(e0)}
+        log(f"single world {label}: {t1 / steps * 1e3:.3f} ms per step")
+        w.close()
+        o.close()
+    out["cpu_single"] = cpu_single(ops, steps, 6.0)
+    out["note"] = ("one r2d_process call per step, host preparation and synchronisation included; energy_change "
+                   "is the relative change of kinetic + potential energy of a pile falling onto the ground "
+                   "(collisions dissipate it)")
+    return out
+
+
+def warm_start_runs(ctx, specs, window, cold_value, factor=0.8):
     """Opt-in warm starting (r2d_set_warm_start): the batched workload with
-    every world warm started (same warm-up and timed steps as the cold run
-    above, so the ratio is the feature's cost), and the resting 20-row
-    pyramid over 600 steps warm against cold: deepest contact, energy drift,
-    contacts kept, GPU time per step and bit equality with the f32 oracle's
-    warm starting."""
+    every world warm started, timed over the same window as the cold run, and
+    the resting 20-row pyramid over 600 steps warm against cold: deepest
+    contact, energy drift, contacts kept, GPU time per step and bit equality
+    with the f32 oracle's warm starting."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from resolve2d_amd import scenes
@@ -261,24 +339,24 @@ def warm_start_runs(ctx, specs, args, cold_value, factor=0.8):
         worlds.append(w)
     n_bodies = sum(count_bodies(sc) for sc in specs)
     batch = ctx.prepare_batch([w.handle for w in worlds])
-    for _ in range(args.warmup):
-        batch.process(dt, S, K, 1)
+    first, last = window
+    batch.process(dt, S, K, first - 1)
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     ctx.sync()
     t0 = time.perf_counter()
-    batch.process(dt, S, K, args.steps)
+    batch.process(dt, S, K, last - first + 1)
     ctx.sync()
     elapsed = time.perf_counter() - t0
     kernel_ms, launches, _ = ctx.fused_times()
     ctx.set_profiling(False)
-    value = n_bodies * args.steps / elapsed
+    value = n_bodies * (last - first + 1) / elapsed
     for w in worlds:
         w.close()
     out = {"factor": factor, "batched": {"value": value, "unit": "body-steps/s", "vs_cold": value / cold_value,
                                          "fused_launches": launches, "kernel_ms": kernel_ms,
-                                         "note": "every world warm started; same warm-up and timed steps as the "
-                                                 "cold headline run"}}
+                                         "note": f"every world warm started; steps {first}-{last} as the cold "
+                                                 f"headline window"}}
     pyr = {}
     for f in (0.0, factor):
         g = GpuWorld(ctx)
@@ -301,8 +379,7 @@ def warm_start_runs(ctx, specs, args, cold_value, factor=0.8):
         pyr["warm" if f else "cold"] = {
             "max_penetration_m": pen, "drift": (energy(b) - e0) / abs(e0), "max_abs_drift": worst,
             "manifolds_at_600": int(g.manifolds().shape[0]), "gpu_ms_per_step": gpu_s / 600 * 1e3,
-            "bit_equal_to_f32_oracle": bool(np.array_equal(b.astype(np.float32).view(np.uint32),
-                                                           o.bodies().astype(np.float32).view(np.uint32)))}
+            "bit_equal_to_f32_oracle": bits_equal(b, o.bodies())}
         g.close()
         o.close()
     out["pyramid_600"] = pyr
@@ -324,7 +401,7 @@ def large_world(ctx, steps, snaps, with_f64):
     apply_scene(ops, g)
     n = g.bodies().shape[0]
     ctx.sync()
-    shots, lat, t = {}, [], 0.0
+    shots, lat = {}, []
     for s in range(1, steps + 1):
         a = time.perf_counter()
         checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), "large-world step")
@@ -337,7 +414,8 @@ def large_world(ctx, steps, snaps, with_f64):
            "value": n * steps / float(np.sum(lat)), "unit": "body-steps/s",
            "p50_step_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
            "p99_step_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
-           "max_penetration_m": max_penetration(man)}
+           "max_penetration_m": max_penetration(man),
+           "note": "one r2d_process call per step from the start, host synchronisation per step included"}
     if with_f64:
         o = pyoracle.OracleWorld("f64")
         apply_scene(ops, o)
@@ -353,27 +431,33 @@ def large_world(ctx, steps, snaps, with_f64):
 
 
 # --------------------------------------------------------------------- serving
+MEAN_SERVED_N = (8192 - 16) / math.log(8192 / 16)   # mean of log-uniform [16, 8192]
+
+
 def served_arrivals(capacity, seconds, load, seed, steps):
-    """A Poisson stream of scenes (16-8192 bodies, log-uniform) offering `load`
-    x `capacity` body-steps/s for `seconds`."""
+    """A Poisson stream of scenes (16-8192 bodies, log-uniform) offering
+    `load` x `capacity` body-steps/s for `seconds`."""
     from resolve2d_amd import scenes
-    mean_n = (8192 - 16) / math.log(8192 / 16)
-    rate = load * capacity / (mean_n * steps)
+    rate = load * capacity / (MEAN_SERVED_N * steps)
     arr = scenes.poisson_stream(seed, rate, max(1, int(rate * seconds)))
     return rate, [(t, (n, s)) for t, n, s in arr]
 
 
-def serve_stream(ctx, rate, arrivals, steps, load, per_tick):
-    """Serve a stream with continuous batching; returns its figures."""
+def serve_stream(ctx, rate, arrivals, steps, load, per_tick, keep=()):
+    """Serve a stream with continuous batching; returns its figures (and the
+    final body / manifold arrays of the scenes whose payloads are in keep)."""
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import GpuWorld
     from resolve2d_amd.serve import Server, run_stream
+    kept = {}
 
     def build(payload):
         n, s = payload
         return scenes.build_served(GpuWorld(ctx), s, n)
 
     def done(payload, w, st):
+        if payload in keep:
+            kept[payload] = (w.bodies(), w.manifolds())
         w.close()
 
     sv = Server(ctx, hbm_budget_bytes=64 << 30, max_worlds=4096, steps_per_tick=per_tick)
@@ -382,14 +466,39 @@ def serve_stream(ctx, rate, arrivals, steps, load, per_tick):
     finally:
         sv.close()
     st = r["stats"]
-    return {"value": st["body_steps"] / r["wall_s"], "unit": "body-steps/s", "offered_load": load,
-            "rate_scenes_per_s": rate, "scenes": st["completed"], "steps_per_scene": steps, "wall_s": r["wall_s"],
-            "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"],
-            "p50_batch_ms": st["p50_batch_ms"], "p99_batch_ms": st["p99_batch_ms"],
-            "p50_request_ms": r.get("request_p50_ms"), "p99_request_ms": r.get("request_p99_ms"),
-            "mean_scenes_per_batch": st["scene_steps"] / max(1, st["batch_steps"]), "lanes": st["lanes"],
-            "steps_per_launch": per_tick,
-            "peak_resident_bytes": st["peak_resident_bytes"]}
+    out = {"value": st["body_steps"] / r["wall_s"], "unit": "body-steps/s", "offered_load": load,
+           "rate_scenes_per_s": rate, "scenes": st["completed"], "steps_per_scene": steps, "wall_s": r["wall_s"],
+           "admitted_scenes_per_s": st["completed"] / r["wall_s"],
+           "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"],
+           "p50_batch_ms": st["p50_batch_ms"], "p99_batch_ms": st["p99_batch_ms"],
+           "p50_request_ms": r.get("request_p50_ms"), "p99_request_ms": r.get("request_p99_ms"),
+           "mean_scenes_per_batch": st["scene_steps"] / max(1, st["batch_steps"]), "lanes": st["lanes"],
+           "steps_per_launch": per_tick, "peak_resident_bytes": st["peak_resident_bytes"]}
+    return out, kept
+
+
+def served_accuracy(kept, steps):
+    """Served scenes against the oracle over their whole request: bit
+    equality with the f32 restatement, error against f64, deepest contact."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import apply_scene
+    rows, bit_eq = [], True
+    for (n, s), (gb, gm) in sorted(kept.items()):
+        ops = scenes.served_ops(s, n)
+        o32, o64 = pyoracle.OracleWorld("f32"), pyoracle.OracleWorld("f64")
+        for o in (o32, o64):
+            apply_scene(ops, o)
+            for _ in range(steps):
+                o.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS)
+        eq = bits_equal(gb, o32.bodies())
+        bit_eq &= eq
+        rows.append({"bodies": n, "bit_equal_to_f32_oracle": eq, "vs_f64_oracle": state_error(gb, o64.bodies()),
+                     "max_penetration_m": max_penetration(gm)})
+        o32.close()
+        o64.close()
+    return {"scenes": rows, "bit_equal_to_f32_oracle": bit_eq, "steps": steps}
 
 
 def plain_served(ctx, arrivals, steps, count):
@@ -413,25 +522,64 @@ def plain_served(ctx, arrivals, steps, count):
     return bs / t, [a[1][0] for a in pick]
 
 
+# ------------------------------------------------------------- HBM copy rate
+def hbm_copy_rate(mib=1024, reps=8):
+    """Device-to-device copy of `mib` MiB (hipMemcpy, timed with HIP events):
+    bytes read + written per second, the measured HBM rate the roofline is
+    also reported against."""
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    P = ctypes.c_void_p
+    n = mib << 20
+    a, b = P(), P()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        return None
+    e0, e1 = P(), P()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(n))
+        for _ in range(2):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)
+        hip.hipDeviceSynchronize()
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    return {"value": 2.0 * n * reps / (ms.value / 1e3) / 1e9, "unit": "GB/s",
+            "sample": f"hipMemcpy device to device, {mib} MiB x {reps}, read + write bytes"}
+
+
+# ---------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=240)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps: the last K of the 240-step horizon")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the window, at least")
     ap.add_argument("--worlds", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--plain-worlds", type=int, default=16)
-    ap.add_argument("--plain-steps", type=int, default=0, help="plain steps per world (0: the full horizon)")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--cpu-cores", type=int, default=0, help="host processes for the CPU baseline (0: up to 16)")
+    ap.add_argument("--cpu-cores", type=int, default=0, help="host processes for the CPU baseline (0: all, <= 16)")
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--cpu-worlds", type=int, default=128, help="configuration-3 worlds in the CPU sample")
+    ap.add_argument("--cpu-scenes", type=int, default=32, help="configuration-5 scenes in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--single-steps", type=int, default=300)
+    ap.add_argument("--reference-hash", action="store_true",
+                    help="configuration 2 with the reference's hash (cell 4.0) instead of the tuned one")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-load", type=float, default=0.7)
-    ap.add_argument("--serve-sat-seconds", type=float, default=4.0,
-                    help="also serve an overloaded stream (3x capacity) for this long (0: skip)")
+    ap.add_argument("--serve-sat-seconds", type=float, default=2.0,
+                    help="seconds of arrivals of the overloaded stream that measures the served capacity")
     ap.add_argument("--serve-plain-scenes", type=int, default=6)
     ap.add_argument("--serve-steps-per-tick", type=int, default=8,
                     help="steps a serving lane's launch takes (fewer when a running scene has fewer left)")
@@ -439,8 +587,9 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-starting runs")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
-    ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the timed region")
-    ap.add_argument("--tuned-hash", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline window and the horizon call (for profiling runs)")
+    ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the horizon")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB between two device buffers (a kernel of known HBM traffic, for "
@@ -462,75 +611,93 @@ def main():
         print(f"bench.py measures one GPU: worlds are independent and would be sharded whole across GPUs "
               f"with no exchange; --gpus {args.gpus} is not run here.", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_cpu = args.no_single = args.no_serve = args.no_accuracy = args.no_large = args.no_warm = True
+        args.plain_worlds = 0
+    K = max(1, args.steps)
+    first = max(args.warmup, HORIZON - K) + 1 if K < HORIZON else args.warmup + 1
+    window = (first, first + K - 1)
 
     from resolve2d_amd import scenes
     t0 = time.perf_counter()
     specs = [scenes.batched_world(args.seed + i) for i in range(args.worlds)]
     n_bodies = sum(count_bodies(sc) for sc in specs)
-    log(f"generated {args.worlds} worlds, {n_bodies} bodies in {time.perf_counter() - t0:.1f} s")
+    log(f"generated {args.worlds} worlds, {n_bodies} bodies in {time.perf_counter() - t0:.1f} s; "
+        f"timed window: steps {window[0]}-{window[1]}")
 
     # CPU baseline first: its worker processes start before anything touches the GPU
     cpu = None
     if not args.no_cpu:
-        cores = args.cpu_cores or max(1, min(16, os.cpu_count() or 1))
-        log(f"cpu baseline on {cores} processes")
-        rate, used, busy, wall = cpu_baseline(specs, args.steps, args.cpu_budget, cores)
-        cpu = {"value": rate, "unit": "body-steps/s", "cores": cores, "kind": "port",
-               "sample": f"oracle f32 restatement (the reference's algorithm in C), {used} worlds spread over "
-                         f"the batch, each x {args.steps} steps, worlds in parallel over {cores} processes "
-                         f"({busy:.1f} s per process)"}
+        ncpu = host_cpus()
+        cores = args.cpu_cores or min(CPU_SHARE, ncpu)
+        stride = max(1, len(specs) // args.cpu_worlds)
+        sample = specs[::stride][:args.cpu_worlds]
+        log(f"cpu baseline: {len(sample)} worlds x {HORIZON} steps on {cores} processes, {args.cpu_repeats} passes")
+        rate, rates, used = cpu_baseline(sample, HORIZON, cores, args.cpu_repeats)
+        s_bodies = sum(count_bodies(sc) for sc in sample)
+        cpu = {"value": rate, "unit": "body-steps/s", "cores": used, "host_cpus": ncpu, "kind": "port",
+               "repeats": args.cpu_repeats, "pass_values": rates,
+               "sample": f"oracle f32 restatement (the reference's algorithm in C), {len(sample)} of the "
+                         f"{args.worlds} configuration-3 worlds ({s_bodies} bodies, every {stride}th) from their "
+                         f"start over the {HORIZON}-step horizon, worlds in parallel over {used} processes (the "
+                         f"box's CPU share is {CPU_SHARE}; the host shows {ncpu}); median of "
+                         f"{args.cpu_repeats} passes"}
+        # configuration 5: served scenes sampled from the same size law
+        rng = np.random.default_rng(args.seed + 7)
+        sizes = np.exp(np.linspace(math.log(16), math.log(8192), args.cpu_scenes)).round().astype(int)
+        sc5 = [scenes.served_ops(int(rng.integers(0, 2 ** 31)), int(n)) for n in sizes]
+        rate5, rates5, used5 = cpu_baseline(sc5, HORIZON, cores, args.cpu_repeats)
+        cpu["config5"] = {"value": rate5, "unit": "body-steps/s", "cores": used5, "pass_values": rates5,
+                          "sample": f"{len(sc5)} served scenes of 16-8192 bodies (log-spaced), each from its start "
+                                    f"over {HORIZON} steps, worlds in parallel; median of {args.cpu_repeats} passes"}
+        log(f"cpu baseline: {rate / 1e6:.1f} M (config 3), {rate5 / 1e6:.1f} M (config 5) body-steps/s")
 
     from resolve2d_amd.engine import GpuWorld, apply_scene
     from resolve2d_amd.solver import Context
     ctx = Context(0)
     ctx.set_solve_variant(args.variant)
+    dt, S, K_IT = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
+
+    def build_batch():
+        ws = []
+        for sc in specs:
+            w = GpuWorld(ctx)
+            apply_scene(sc, w)
+            ws.append(w)
+        return ws
+
     t0 = time.perf_counter()
-    worlds = []
-    for sc in specs:
-        w = GpuWorld(ctx)
-        apply_scene(sc, w)
-        worlds.append(w)
-    handles = [w.handle for w in worlds]
+    worlds = build_batch()
     setup_s = time.perf_counter() - t0
-    dt, S, K = scenes.DT, scenes.SUB_STEPS, scenes.ITERS
     log(f"built {args.worlds} device worlds in {setup_s:.1f} s")
-    batch = ctx.prepare_batch(handles)
-    for _ in range(args.warmup):
-        batch.process(dt, S, K, 1)
+    batch = ctx.prepare_batch([w.handle for w in worlds])
+
     tuned = None
     if args.variant < 0 and not args.no_tune:
-        # launch shape picked on copies of the batch advanced to step 60 (every
-        # shape gives the same bits, so the copies change nothing measured):
-        # each candidate runs one 30-step call from that same state
+        # launch shape picked on copies of the batch advanced to the window's
+        # start (every shape gives the same bits, so the copies change nothing
+        # measured): each candidate times one call over the window's steps,
+        # and the three fastest are timed again from the same state
         from resolve2d_amd import autotune
 
         def fresh_copy():
-            copy = []
-            for sc in specs:
-                w = GpuWorld(ctx)
-                apply_scene(sc, w)
-                copy.append(w)
-            ctx.process_batch_handles([w.handle for w in copy], dt, S, K, TUNE_AT)
+            copy = build_batch()
+            ctx.process_batch_handles([w.handle for w in copy], dt, S, K_IT, window[0] - 1)
             return copy
 
-        cands = autotune.candidates(lds_limits=(0,))
-        tuned = autotune.tune(ctx, None, dt, S, K, rounds=1, cands=cands, cache_path=None,
-                              steps_per_sample=TUNE_STEPS, make_batch=fresh_copy, refine=TUNE_REFINE,
-                              refine_steps=TUNE_REFINE_STEPS)
+        cands = [c for c in autotune.candidates(lds_limits=(0,)) if c[2] or c[0] in (1, 11)]
+        tuned = autotune.tune(ctx, None, dt, S, K_IT, rounds=1, cands=cands, cache_path=None,
+                              steps_per_sample=K, make_batch=fresh_copy, refine=3, refine_steps=K)
         log(f"autotune: variant {tuned['variant']} fused {tuned['fused']} ({tuned['step_ms']:.3f} ms per step)")
-    ctx.sync()
 
-    # timed region: exactly `steps` batched steps in one call.  Worlds of at
-    # most 1,024 bodies take whole steps per workgroup: one fused kernel runs
-    # every step of every world, with HIP events around the launch and each
-    # workgroup's wall clock per step (the world-step latency).  Otherwise the
-    # hipGraph of a step is replayed back to back with HIP events between the
-    # phases of every step on the library's stream.
+    # timed region: the window, one call of K steps after the untimed advance
+    batch.process(dt, S, K_IT, window[0] - 1)
+    ctx.sync()
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     ctx.sync()
     t_start = time.perf_counter()
-    batch.process(dt, S, K, args.steps)
+    batch.process(dt, S, K_IT, K)
     ctx.sync()
     elapsed = time.perf_counter() - t_start
     phase_ms, phase_steps = ctx.phase_times()
@@ -540,34 +707,73 @@ def main():
     lat_ms = np.array(ctx.step_times())
     counts = ctx.last_counts()
     ctx.set_profiling(False)
-    value = n_bodies * args.steps / elapsed
-    log(f"batched: {args.steps} steps in {elapsed:.3f} s ({value / 1e6:.1f} M body-steps/s)")
-    # interactive use: one step per call, host work and synchronisation included
-    call_lat = []
-    for _ in range(args.call_steps):
-        a = time.perf_counter()
-        batch.process(dt, S, K, 1)
-        call_lat.append((time.perf_counter() - a) * 1e3)
+    value = n_bodies * K / elapsed
+    log(f"batched: steps {window[0]}-{window[1]} in {elapsed:.4f} s ({value / 1e6:.1f} M body-steps/s)")
     batch_pen = max(max_penetration(w.manifolds()) for w in worlds[:: max(1, len(worlds) // 64)])
     for w in worlds:
         w.close()
 
-    # plain: the same scenes, one world per launch, over the same horizon
-    plain_steps = args.plain_steps or args.steps
-    plain_body_steps, plain_t = 0, 0.0
-    for sc in specs[:: max(1, len(specs) // args.plain_worlds)][:args.plain_worlds]:
-        w = GpuWorld(ctx)
-        apply_scene(sc, w)
-        ctx.sync()
+    # the whole job: a fresh batch, all 240 steps in one call
+    worlds = build_batch()
+    batch = ctx.prepare_batch([w.handle for w in worlds])
+    ctx.sync()
+    ctx.set_profiling(True)
+    ctx.reset_phase_times()
+    ctx.sync()
+    t_h = time.perf_counter()
+    batch.process(dt, S, K_IT, HORIZON)
+    ctx.sync()
+    el_h = time.perf_counter() - t_h
+    h_fused_ms, h_launches, _ = ctx.fused_times()
+    h_lat = np.array(ctx.step_times())
+    ctx.set_profiling(False)
+    horizon = {"steps": f"1-{HORIZON}", "value": n_bodies * HORIZON / el_h, "unit": "body-steps/s",
+               "ms_per_step": el_h / HORIZON * 1e3,
+               "launch_ms": h_fused_ms / h_launches if h_launches else None,
+               "p50_step_ms": float(np.percentile(h_lat, 50)) if len(h_lat) else None,
+               "p99_step_ms": float(np.percentile(h_lat, 99)) if len(h_lat) else None,
+               "note": "the whole configuration-3 job: a fresh batch, all 240 steps in one call"}
+    log(f"horizon: {HORIZON} steps in {el_h:.3f} s ({horizon['value'] / 1e6:.1f} M body-steps/s)")
+    # interactive use: one step per call, host work and synchronisation included
+    call_lat = []
+    for _ in range(args.call_steps):
         a = time.perf_counter()
-        for _ in range(plain_steps):
-            checked(w.process(dt, S, K), "plain step")
-        ctx.sync()
-        plain_t += time.perf_counter() - a
-        plain_body_steps += count_bodies(sc) * plain_steps
+        batch.process(dt, S, K_IT, 1)
+        call_lat.append((time.perf_counter() - a) * 1e3)
+    for w in worlds:
         w.close()
-    plain_value = plain_body_steps / plain_t
-    log("plain done")
+
+    # plain: the same scenes, one world per launch (the reference's
+    # one-world process call), over the same window and over the horizon
+    plain = None
+    if args.plain_worlds:
+        pick = specs[:: max(1, len(specs) // args.plain_worlds)][:args.plain_worlds]
+        pw_bs, pw_t, ph_bs, ph_t = 0, 0.0, 0, 0.0
+        for sc in pick:
+            w = GpuWorld(ctx)
+            apply_scene(sc, w)
+            ctx.sync()
+            a = time.perf_counter()
+            for s in range(1, window[1] + 1):
+                if s == window[0]:
+                    ctx.sync()
+                    b0 = time.perf_counter()
+                checked(w.process(dt, S, K_IT), "plain step")
+            ctx.sync()
+            b1 = time.perf_counter()
+            pw_t += b1 - b0
+            pw_bs += count_bodies(sc) * K
+            if window[1] >= HORIZON:
+                ph_t += b1 - a
+                ph_bs += count_bodies(sc) * window[1]
+            w.close()
+        plain = {"value": pw_bs / pw_t, "unit": "body-steps/s",
+                 "sample": f"{len(pick)} worlds spread over the batch, one world per launch (r2d_process per "
+                           f"step); steps {window[0]}-{window[1]} timed"}
+        if ph_t:
+            plain["horizon"] = {"value": ph_bs / ph_t, "unit": "body-steps/s",
+                                "sample": f"the same worlds over steps 1-{window[1]}"}
+        log(f"plain: {plain['value'] / 1e6:.2f} M body-steps/s")
 
     # roofline of the dominant kernel.  Fused: k_fused, one launch for the
     # whole timed region; its algorithmic HBM bytes are the step's
@@ -580,12 +786,13 @@ def main():
     # Phase kernels: the solve phase, 112 B per body + 120 B per manifold +
     # 16 B per solver item.
     joint_items = max(0, work["items"] - work["manifolds"])
+    used_variant = tuned["variant"] if tuned else (args.variant if args.variant >= 0 else 1)
     if fused_launches:
-        kernel = "k_fused"
+        kernel = f"k_fused<{FUSED_SHAPES.get(used_variant, '?')}>"
         launch_s = fused_ms / fused_launches / 1e3
         launch_bytes = (72 * work["bodies"] + 16 * work["entries"] + 8 * work["raw_candidates"] +
                         176 * work["manifolds"]) / fused_launches
-        lds_bytes = S * K * (64 * work["manifolds"] + 48 * joint_items) / fused_launches
+        lds_bytes = S * K_IT * (64 * work["manifolds"] + 48 * joint_items) / fused_launches
         per_step = {k: v / max(world_steps, 1) for k, v in fused_phases.items()}
         per_step_note = "per world-step, workgroup wall clock"
     else:
@@ -594,48 +801,24 @@ def main():
         per_step = {n: phase_ms[i] / max(phase_steps, 1) for i, n in enumerate(names)}
         launch_s = per_step["solve"] / 1e3
         launch_bytes = 112 * n_bodies + 120 * counts["manifolds"] + 16 * counts["items"]
-        lds_bytes = S * K * (64 * counts["manifolds"] + 48 * max(0, counts["items"] - counts["manifolds"]))
+        lds_bytes = S * K_IT * (64 * counts["manifolds"] + 48 * max(0, counts["items"] - counts["manifolds"]))
         per_step_note = "per step, HIP events between phase launches"
     dom = max(per_step, key=per_step.get)
-    used_variant = tuned["variant"] if tuned else (args.variant if args.variant >= 0 else 1)
     achieved = launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
     lds_achieved = lds_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
+    copy = hbm_copy_rate()
 
-    # configuration 2: one 4,096-body world, full horizon, one world per launch,
-    # reference order vs the coloured schedule
-    single = {}
+    single = None
     if not args.no_single:
-        for mode, label in ((0, "reference_order"), (1, "colored")):
-            w = GpuWorld(ctx)
-            apply_scene(scenes.mixed(4096, seed=1, hash_cell=2.0 if args.tuned_hash else None), w)
-            if mode:
-                w.set_schedule(1)
-            ctx.sync()
-            lat1 = []
-            a0 = time.perf_counter()
-            for i in range(args.single_steps):
-                a = time.perf_counter()
-                checked(w.process(dt, S, K), f"single-world step ({label})")
-                lat1.append(time.perf_counter() - a)
-                if i % 100 == 99:
-                    log(f"single world {label}: {i + 1} steps")
-            ctx.sync()
-            t1 = time.perf_counter() - a0
-            single[label] = {"value": 4097 * args.single_steps / t1, "unit": "body-steps/s",
-                             "ms_per_step": t1 / args.single_steps * 1e3,
-                             "p50_step_ms": float(np.percentile(np.array(lat1) * 1e3, 50)),
-                             "p99_step_ms": float(np.percentile(np.array(lat1) * 1e3, 99))}
-            w.close()
-        single["cpu_single"] = cpu_single(scenes.mixed(4096, seed=1, hash_cell=2.0 if args.tuned_hash else None),
-                                          args.single_steps, 6.0)
+        single = single_world(ctx, args.single_steps, not args.reference_hash)
 
-    large = None
     warm = None
     if not args.no_warm:
         log("warm starting")
-        warm = warm_start_runs(ctx, specs, args, value)
+        warm = warm_start_runs(ctx, specs, window, value)
         log(f"warm starting: batched {warm['batched']['value'] / 1e6:.1f} M body-steps/s "
             f"({warm['batched']['vs_cold']:.2f}x cold)")
+    large = None
     if not args.no_large:
         log("large constrained world")
         large = large_world(ctx, 120, {1, 10, 120}, not args.no_accuracy)
@@ -644,53 +827,62 @@ def main():
     if not args.no_accuracy:
         log("accuracy")
         sample = list(range(0, args.worlds, max(1, args.worlds // 4)))[:4]
-        horizon = args.steps
-        errs, bit_eq, pen = accuracy(ctx, specs, sample, horizon, {1, 60, horizon})
+        errs, bit_eq, pen = accuracy(ctx, specs, sample, HORIZON, {1, 60, HORIZON})
         acc = {"worlds": sample, "bit_equal_to_f32_oracle": bit_eq, "tolerance_vs_reference_f32": 1e-5,
                "vs_f64_oracle": errs, "max_penetration_m": pen, "batch_max_penetration_m": batch_pen,
                "pyramid_600": pyramid_energy(ctx, 600)}
 
     serve = None
     if not args.no_serve:
-        log("serving stream")
-        rate, arrivals = served_arrivals(value, args.serve_seconds, args.serve_load, args.seed, args.steps)
-        serve = serve_stream(ctx, rate, arrivals, args.steps, args.serve_load, args.serve_steps_per_tick)
+        # capacity: an overloaded stream (2x the batched horizon rate) drained
+        # to the end; then a stream offering serve_load x that capacity
+        log("serving: overloaded stream")
+        rate2, arr2 = served_arrivals(horizon["value"], args.serve_sat_seconds, 2.0, args.seed + 1, HORIZON)
+        sat, _ = serve_stream(ctx, rate2, arr2, HORIZON, 2.0, args.serve_steps_per_tick)
+        capacity = sat["value"]
+        log(f"served capacity: {capacity / 1e6:.1f} M body-steps/s")
+        rate, arrivals = served_arrivals(capacity, args.serve_seconds, args.serve_load, args.seed, HORIZON)
+        by_size = sorted(arrivals, key=lambda a: a[1][0])
+        keep = {by_size[0][1], by_size[len(by_size) // 2][1],
+                min(by_size, key=lambda a: abs(a[1][0] - 2000))[1]}
+        serve, kept = serve_stream(ctx, rate, arrivals, HORIZON, args.serve_load, args.serve_steps_per_tick,
+                                   keep=keep)
         log(f"served {serve['scenes']} scenes, {serve['value'] / 1e6:.1f} M body-steps/s")
-        if args.serve_sat_seconds > 0:
-            rate2, arr2 = served_arrivals(value, args.serve_sat_seconds, 3.0, args.seed + 1, args.steps)
-            sat = serve_stream(ctx, rate2, arr2, args.steps, 3.0, args.serve_steps_per_tick)
-            serve["saturated"] = {k: sat[k] for k in ("value", "offered_load", "scenes", "wall_s", "p50_step_ms",
-                                                      "p99_step_ms", "p50_request_ms", "p99_request_ms")}
-            log(f"saturated: {sat['value'] / 1e6:.1f} M body-steps/s")
-        pv, sizes = plain_served(ctx, arrivals, args.steps, args.serve_plain_scenes)
+        serve["capacity"] = {k: sat[k] for k in ("value", "offered_load", "rate_scenes_per_s", "scenes", "wall_s",
+                                                 "admitted_scenes_per_s", "p50_step_ms", "p99_step_ms",
+                                                 "p50_request_ms", "p99_request_ms")}
+        serve["offered_body_steps_per_s"] = args.serve_load * capacity
+        pv, sizes = plain_served(ctx, arrivals, HORIZON, args.serve_plain_scenes)
         serve["plain"] = {"value": pv, "unit": "body-steps/s",
-                          "sample": f"{len(sizes)} served scenes of {sizes} bodies, {args.steps} steps each, one "
+                          "sample": f"{len(sizes)} served scenes of {sizes} bodies, {HORIZON} steps each, one "
                                     f"world per launch"}
         serve["served_vs_plain"] = serve["value"] / pv
-        if "saturated" in serve:
-            serve["saturated_vs_plain"] = serve["saturated"]["value"] / pv
+        serve["capacity_vs_plain"] = capacity / pv
+        serve["accuracy"] = served_accuracy(kept, HORIZON)
 
     out = {
         "metric": "body-steps/sec batched vs plain on 1 MI355X; p50/p99 step latency",
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": elapsed / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scenes)",
-        "config": {"workload": "batched worlds: 1024 worlds x U[50,500] mixed discs/rects, 0-20 distance joints, "
-                               "0-10 springs per world; dt=1/60, 4 sub-steps, 4 iterations",
-                   "worlds": args.worlds, "bodies": n_bodies, "manifolds_last_step": counts["manifolds"],
+        "config": {"workload": f"batched worlds (configuration 3): 1024 worlds x U[50,500] mixed discs/rects, 0-20 "
+                               f"distance joints, 0-10 springs per world; dt=1/60, 4 sub-steps, 4 iterations; "
+                               f"{HORIZON}-step horizon, timed window steps {window[0]}-{window[1]} (one call of "
+                               f"{K} steps after {window[0] - 1} untimed)",
+                   "worlds": args.worlds, "bodies": n_bodies, "window": list(window),
+                   "manifolds_last_step": counts["manifolds"],
                    "schedule": "reference order (bit-exact)", "parallelism": "1 GPU",
                    "solve_launch": ({"autotuned": True, "variant": tuned["variant"], "fused": tuned["fused"],
-                                     "tuned_at_step": TUNE_AT, "sample_steps": TUNE_STEPS,
+                                     "tuned_at_step": window[0], "sample_steps": K,
                                      "candidates_step_ms (variant/lds_limit/fused)": tuned["all_ms"],
-                                     "refine_steps": TUNE_REFINE_STEPS,
                                      "refined_step_ms": tuned.get("refined_ms")}
                                     if tuned else {"autotuned": False, "variant": args.variant})},
         "p50_step_ms": float(np.percentile(lat_ms, 50)) if len(lat_ms) else None,
@@ -700,14 +892,15 @@ def main():
         "step_latency": ("each world's step, wall clock of its workgroup in the fused kernel (all world-steps of the "
                          "timed region)" if fused_launches else
                          "device time per step from HIP events (the timed region's steps)"),
+        "horizon": horizon,
         "per_call": {"p50_ms": float(np.percentile(call_lat, 50)) if call_lat else None,
                      "p99_ms": float(np.percentile(call_lat, 99)) if call_lat else None,
-                     "calls": len(call_lat), "note": "one step per r2d_process_batch call after the timed "
-                                                     "region, host preparation and synchronisation included"},
-        "plain": {"value": plain_value, "unit": "body-steps/s",
-                  "sample": f"{args.plain_worlds} worlds spread over the batch, {plain_steps} steps each, "
-                            f"one world per launch"},
-        "batched_vs_plain": value / plain_value,
+                     "calls": len(call_lat), "note": "one step per r2d_process_batch call after the horizon, "
+                                                     "host preparation and synchronisation included"},
+        "plain": plain,
+        "batched_vs_plain": value / plain["value"] if plain else None,
+        "horizon_vs_plain": (horizon["value"] / plain["horizon"]["value"]
+                             if plain and "horizon" in plain else None),
         "served": serve,
         "single_world_4096": single,
         "large_world_65536": large,
@@ -717,17 +910,18 @@ def main():
                  "phase kernels: one launch sequence (hipGraph) per step"),
         "phase_ms": per_step,
         "phase_ms_note": per_step_note,
-        "roofline": {"kernel": (f"{kernel}<{FUSED_SHAPES[used_variant]}>" if kernel == "k_fused" and
-                                used_variant in FUSED_SHAPES else kernel), "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kernel, used_variant, args),
+        "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kernel, window, args),
                      "launch_ms": launch_s * 1e3, "algorithmic_bytes_per_launch": launch_bytes,
+                     "measured_copy_rate": copy,
+                     "frac_of_measured_copy": achieved / copy["value"] if copy else None,
                      "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                              "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes},
                      "dominant_phase": dom},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
     ctx.close()
 
 

@@ -79,11 +79,40 @@ def pyramid(rows: int = 20, mu: float = 0.5, g: float = 9.81):
     return b.ops
 
 
+def max_extent(ops) -> float:
+    """Largest side of a dynamic body's axis-aligned box as created: 2 r for a
+    disc, |w cos a| + |h sin a| or |w sin a| + |h cos a| for a rectangle."""
+    static = {op[1] for op in ops if op[0] == "static" and op[2]}
+    best, i = 0.0, 0
+    for op in ops:
+        if op[0] not in ("disc", "rect"):
+            continue
+        if i not in static:
+            if op[0] == "disc":
+                best = max(best, 2.0 * op[10])
+            else:
+                c, s = abs(math.cos(op[5])), abs(math.sin(op[5]))
+                w, h = op[10], op[11]
+                best = max(best, w * c + h * s, w * s + h * c)
+        i += 1
+    return best
+
+
+def tuned_cell(ops) -> float:
+    """Cell width of the tuned spatial hash: twice the largest body extent."""
+    return f32(2.0 * max_extent(ops))
+
+
 def mixed(n: int = 4096, seed: int = 1, width: float = 100.0, y0: float = 1.0, height: float = 200.0,
-          ground_w: float = 120.0, mu: float = 0.4, g: float = 9.81, hash_cell: float = None):
+          ground_w: float = 120.0, mu: float = 0.4, g: float = 9.81, hash_cell=None):
     """n bodies alternating disc / rectangle: disc radius U[0.2, 0.6]; rectangle
     sides U[0.3, 1.0], angle U[0, 2pi); positions uniform in a width x height box
-    above a static ground; density 1.  hash_cell selects the tuned hash mode."""
+    above a static ground; density 1.  hash_cell selects the tuned hash mode
+    with that cell width (2N buckets); "auto" takes twice the largest body
+    extent (tuned_cell)."""
+    if hash_cell == "auto":
+        base = mixed(n, seed, width, y0, height, ground_w, mu, g)
+        return [("hash", tuned_cell(base), 2)] + base
     rng = np.random.default_rng(seed)
     b = _Builder()
     if hash_cell is not None:

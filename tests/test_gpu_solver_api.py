@@ -103,7 +103,8 @@ def test_props_edits_removal_and_handles(gpu_ctx, oracle):
     from resolve2d_amd._native import InvalidRigidBodyId
     from resolve2d_amd.engine import apply_scene
     from resolve2d_amd.solver import Solver
-    ops = scenes.car_demo()
+    # the car scene plus a few free boxes (pyramid rows without their ground)
+    ops = scenes.car_demo() + [op for op in scenes.pyramid(rows=4) if op[0] == "rect"][1:]
     s = Solver.init(ctx=gpu_ctx)
     o = oracle.OracleWorld("f32")
     apply_via_factory(ops, s)
@@ -112,7 +113,12 @@ def test_props_edits_removal_and_handles(gpu_ctx, oracle):
         s.process(DT, 4, 4)
         o.process(DT, 4, 4)
     n = s.num_bodies()
-    victim = s.body_id_at(3)
+    # a body no joint names (removing a jointed one makes the next step fail
+    # with InvalidRigidBodyId, as in the reference)
+    jointed = {op[5] for op in ops if op[0] == "joint"} | {op[6] for op in ops if op[0] == "joint"}
+    slot = next(i for i in range(3, n - 2) if s.body_id_at(i) not in jointed and not
+                s.bodyHandle(s.body_id_at(i)).body_unwrap().static)
+    victim = s.body_id_at(slot)
     last = s.body_id_at(n - 1)
     edit = s.body_id_at(n - 2)
     b = s.bodyHandle(edit).body_unwrap()
@@ -122,20 +128,20 @@ def test_props_edits_removal_and_handles(gpu_ctx, oracle):
     row = [r for r in o.bodies() if int(r[0]) == edit][0]
     o.set_body(edit, row[3], row[4], np.float32(1.5), np.float32(-0.25), row[7], row[8], row[10],
                np.float32(0.75), np.float32(2.0))
-    pinned = s.body_id_at(5)
+    pinned = s.body_id_at(slot + 1)
     s.bodyHandle(pinned).body_unwrap().static = True
     o.set_static(pinned, True)
     assert s.bodyHandle(pinned).body_unwrap().static
     s.removeRigidBody(victim)
     o.remove_body(victim)
-    assert s.body_id_at(3) == last
+    assert s.body_id_at(slot) == last
     assert s.bodyHandle(victim).body() is None
     with pytest.raises(InvalidRigidBodyId):
         s.removeRigidBody(victim)
     _compare(s, o, "edited")
     for _ in range(30):
         s.process(DT, 4, 4)
-        o.process(DT, 4, 4)
+        assert o.process(DT, 4, 4) == 0
     _compare(s, o, "after edits")
     s.deinit()
 
