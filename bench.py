@@ -430,6 +430,79 @@ def large_world(ctx, steps, snaps, with_f64):
     return out
 
 
+# ------------------------------------------------------- critical path of a step
+def chain_depth(ops, states, manifolds, iters):
+    """Longest dependency chain of one sub-step's solver in the reference order
+    (lib.zig:226-236: collision_iters passes of the constraints in order, then
+    the manifolds in discovery order).  An item waits for every earlier item
+    (over the passes) that shares one of its ordering bodies: both bodies of a
+    distance / offset joint, the body of a fixed-position / motor joint, and
+    those bodies of a manifold that are dynamic or held by a distance / offset
+    joint (a contact's zero change of any other static body orders nothing).
+    Bodies as scene ids; states: the world's body rows (id col 0, static col 2);
+    manifolds: (ref id, inc id) rows.  Returns (items, depth)."""
+    static = {int(r[0]) for r in states if r[2]}
+    items, jointed = [], set()
+    for op in ops:
+        if op[0] != "joint":
+            continue
+        t, a, b = op[1], op[5], op[6]
+        if t <= 1:
+            items.append((a, b))
+            jointed |= {a, b}
+        else:
+            items.append((a,))
+    for ref, inc in manifolds:
+        items.append(tuple(x for x in (int(ref), int(inc)) if x not in static or x in jointed))
+    last, depth = {}, 0
+    for _ in range(iters):
+        for bodies in items:
+            lv = 1 + max((last.get(x, 0) for x in bodies), default=0)
+            for x in bodies:
+                last[x] = lv
+            depth = max(depth, lv)
+    return len(items), depth
+
+
+def critical_path(ctx, label, ops, warm_steps, timed_steps, cpu_ms):
+    """One world stepped one call per step: after warm_steps, timed_steps
+    with HIP-event / workgroup-clock profiling, then the chain depth of its
+    last step from the world's own manifold readback; the solve's device time
+    per step over (sub-steps x depth) is the GPU's time per chain link."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    g = GpuWorld(ctx)
+    apply_scene(ops, g)
+    for _ in range(warm_steps):
+        checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), f"{label} step")
+    ctx.set_profiling(True)
+    ctx.reset_phase_times()
+    lat = []
+    for _ in range(timed_steps):
+        a = time.perf_counter()
+        checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), f"{label} step")
+        lat.append(time.perf_counter() - a)
+    fused_ms, launches, wsteps = ctx.fused_times()
+    if launches:
+        solve_ms = ctx.fused_phase_times()["solve"] / max(wsteps, 1)
+        kernel_ms = fused_ms / launches
+    else:
+        ph, n = ctx.phase_times()
+        solve_ms = ph[3] / max(n, 1)
+        kernel_ms = sum(ph) / max(n, 1)
+    ctx.set_profiling(False)
+    states, man = g.bodies(), g.manifolds()
+    items, depth = chain_depth(ops, states, man[:, :2], scenes.ITERS)
+    g.close()
+    links = scenes.SUB_STEPS * depth
+    return {"bodies": int(states.shape[0]), "items": items, "manifolds": int(man.shape[0]),
+            "depth_per_substep": depth, "links_per_step": links,
+            "solve_ms_per_step": solve_ms, "device_ms_per_step": kernel_ms,
+            "call_ms_per_step": float(np.mean(lat) * 1e3), "ns_per_link": solve_ms * 1e6 / max(links, 1),
+            "cpu_ms_per_step": cpu_ms, "at_step": warm_steps + timed_steps,
+            "mode": "fused kernel (one workgroup)" if launches else "phase kernels"}
+
+
 # --------------------------------------------------------------------- serving
 MEAN_SERVED_N = (8192 - 16) / math.log(8192 / 16)   # mean of log-uniform [16, 8192]
 
@@ -585,6 +658,7 @@ def main():
                     help="steps a serving lane's launch takes (fewer when a running scene has fewer left)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-critical", action="store_true", help="skip the one-world critical-path measurements")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-starting runs")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--headline-only", action="store_true",
@@ -613,6 +687,7 @@ def main():
         sys.exit(2)
     if args.headline_only:
         args.no_cpu = args.no_single = args.no_serve = args.no_accuracy = args.no_large = args.no_warm = True
+        args.no_critical = True
         args.plain_worlds = 0
     K = max(1, args.steps)
     first = max(args.warmup, HORIZON - K) + 1 if K < HORIZON else args.warmup + 1
@@ -832,6 +907,26 @@ def main():
                "vs_f64_oracle": errs, "max_penetration_m": pen, "batch_max_penetration_m": batch_pen,
                "pyramid_600": pyramid_energy(ctx, 600)}
 
+    crit = None
+    if not args.no_critical:
+        # the reference-order critical path of one-world steps, with the
+        # GPU's solve time per chain link (one world per call)
+        log("critical paths")
+        crit = {}
+        cpu1 = acc["pyramid_600"]["cpu_single"]["ms_per_step"] if acc else None
+        crit["pyramid_20_rows"] = critical_path(ctx, "pyramid", scenes.pyramid(20), 590, 10, cpu1)
+        if single:
+            cpu2 = single["cpu_single"]["ms_per_step"]
+            ops2 = scenes.mixed(4096, seed=1, hash_cell=None if args.reference_hash else "auto")
+            crit["world_4096"] = critical_path(ctx, "4,096-body world", ops2, args.single_steps - 10, 10, cpu2)
+        if large:
+            crit["world_65540"] = critical_path(ctx, "65,540-body world", scenes.large_constrained(), 110, 10,
+                                                large["cpu_single"]["ms_per_step"])
+        crit["note"] = ("depth: longest dependency chain of a sub-step's solver items (4 passes of joints then "
+                        "manifolds in discovery order, items ordered by their shared dynamic or jointed bodies), from "
+                        "the world's manifolds at the last timed step; links_per_step = 4 sub-steps x depth; "
+                        "ns_per_link = the solve's device time per step / links_per_step")
+
     serve = None
     if not args.no_serve:
         # capacity: an overloaded stream (2x the batched horizon rate) drained
@@ -903,6 +998,7 @@ def main():
                              if plain and "horizon" in plain else None),
         "served": serve,
         "single_world_4096": single,
+        "critical_path": crit,
         "large_world_65536": large,
         "accuracy": acc,
         "warm_start": warm,
