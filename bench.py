@@ -530,7 +530,7 @@ def serve_stream(ctx, rate, arrivals, steps, load, per_tick, keep=()):
 
     def done(payload, w, st):
         if payload in keep:
-            kept[payload] = (w.bodies(), w.manifolds())
+            kept[payload] = w.bodies()
         w.close()
 
     sv = Server(ctx, hbm_budget_bytes=64 << 30, max_worlds=4096, steps_per_tick=per_tick)
@@ -552,13 +552,16 @@ def serve_stream(ctx, rate, arrivals, steps, load, per_tick, keep=()):
 
 def served_accuracy(kept, steps):
     """Served scenes against the oracle over their whole request: bit
-    equality with the f32 restatement, error against f64, deepest contact."""
+    equality with the f32 restatement, error against f64, and the deepest
+    contact of the last step (from the f32 restatement's manifolds, whose
+    bodies the served world equals bit for bit; a served world's manifolds
+    live in its serving lane's scratch, which later steps reuse)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from resolve2d_amd import scenes
     from resolve2d_amd.engine import apply_scene
     rows, bit_eq = [], True
-    for (n, s), (gb, gm) in sorted(kept.items()):
+    for (n, s), gb in sorted(kept.items()):
         ops = scenes.served_ops(s, n)
         o32, o64 = pyoracle.OracleWorld("f32"), pyoracle.OracleWorld("f64")
         for o in (o32, o64):
@@ -568,7 +571,7 @@ def served_accuracy(kept, steps):
         eq = bits_equal(gb, o32.bodies())
         bit_eq &= eq
         rows.append({"bodies": n, "bit_equal_to_f32_oracle": eq, "vs_f64_oracle": state_error(gb, o64.bodies()),
-                     "max_penetration_m": max_penetration(gm)})
+                     "max_penetration_m": max_penetration(o32.manifolds())})
         o32.close()
         o64.close()
     return {"scenes": rows, "bit_equal_to_f32_oracle": bit_eq, "steps": steps}
@@ -630,6 +633,36 @@ def hbm_copy_rate(mib=1024, reps=8):
             "sample": f"hipMemcpy device to device, {mib} MiB x {reps}, read + write bytes"}
 
 
+def one_world(args):
+    """One configuration's single world over its horizon, one r2d_process
+    per step (the plain mode of configurations 1, 2 and 4), for profiling."""
+    from resolve2d_amd import scenes
+    from resolve2d_amd.engine import GpuWorld, apply_scene
+    from resolve2d_amd.solver import Context
+    ops, steps = {"config1": (scenes.pyramid(20), 600),
+                  "config2": (scenes.mixed(4096, seed=1, hash_cell=None if args.reference_hash else "auto"), 300),
+                  "config4": (scenes.large_constrained(), 120)}[args.only_world]
+    ctx = Context(0)
+    ctx.set_solve_variant(args.variant)
+    g = GpuWorld(ctx)
+    apply_scene(ops, g)
+    ctx.sync()
+    lat = []
+    for i in range(steps):
+        a = time.perf_counter()
+        checked(g.process(scenes.DT, scenes.SUB_STEPS, scenes.ITERS), f"{args.only_world} step")
+        lat.append(time.perf_counter() - a)
+        if i % 100 == 99:
+            log(f"{args.only_world}: {i + 1} steps")
+    n = g.bodies().shape[0]
+    g.close()
+    lat = np.array(lat) * 1e3
+    print(json.dumps({"world": args.only_world, "bodies": n, "steps": steps, "ms_per_step": float(lat.mean()),
+                      "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99))}),
+          flush=True)
+    ctx.close()
+
+
 # ---------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -663,6 +696,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline window and the horizon call (for profiling runs)")
+    ap.add_argument("--only-world", choices=["config1", "config2", "config4"], default=None,
+                    help="only step one configuration's single world over its horizon, one call per step, and "
+                         "print its time per step (for profiling runs)")
     ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the horizon")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -685,6 +721,9 @@ def main():
         print(f"bench.py measures one GPU: worlds are independent and would be sharded whole across GPUs "
               f"with no exchange; --gpus {args.gpus} is not run here.", file=sys.stderr)
         sys.exit(2)
+    if args.only_world:
+        one_world(args)
+        return
     if args.headline_only:
         args.no_cpu = args.no_single = args.no_serve = args.no_accuracy = args.no_large = args.no_warm = True
         args.no_critical = True
@@ -788,9 +827,13 @@ def main():
     for w in worlds:
         w.close()
 
-    # the whole job: a fresh batch, all 240 steps in one call
+    # the whole job: a fresh batch, all 240 steps in one call.  A call
+    # without sub-steps first makes the batch resident on the device: it runs
+    # only the broadphase and narrowphase (lib.zig:194), which change no body
+    # state, so the 240 timed steps start from the built state
     worlds = build_batch()
     batch = ctx.prepare_batch([w.handle for w in worlds])
+    batch.process(dt, 0, 0, 1)
     ctx.sync()
     ctx.set_profiling(True)
     ctx.reset_phase_times()

@@ -290,7 +290,10 @@ int r2d_debug_eval(r2d_ctx* ctx, int op, const float* in, float* out, size_t n);
 /* n bodies in one call, in order (ids as n calls of r2d_make_disc /
  * r2d_make_rect would assign them, EntityFactory.appendBody lib.zig:66-71):
  * kind[i] 0 = disc (dim_a = radius), 1 = rectangle (dim_a x dim_b).
- * ids_out may be NULL. */
+ * ids_out may be NULL.  From 32 bodies on, the device builds them: the
+ * inputs are copied to the device and one kernel writes mass, inertia,
+ * momenta and AABBs into the world's resident state (the same bits as the
+ * single calls); the call returns once the inputs have been read. */
 int r2d_make_bodies(r2d_world* w, size_t n, const r2d_body_options* bo, const int32_t* kind, const float* dim_a,
                     const float* dim_b, uint32_t* ids_out);
 

@@ -192,6 +192,8 @@ struct r2d_ctx {
     uint64_t fused_launches = 0;
     uint64_t fused_steps = 0;                     // world-steps of profiled fused launches
     uint64_t fused_sums[5] = {};                  // their bodies, entries, raw candidates, manifolds, items
+    void* ingest = nullptr;                       // r2d_make_bodies' inputs on the device
+    size_t ingest_cap = 0;                        // bodies they hold
 
     Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}, trace}; }
 };
@@ -1486,6 +1488,7 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     if (c->hcnt) (void)hipHostFree(c->hcnt);
     if (c->hscr) (void)hipHostFree(c->hscr);
     if (c->hpool) (void)hipHostFree(c->hpool);
+    if (c->ingest) (void)hipFree(c->ingest);
     for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(c->aux[i]);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2035,21 +2038,150 @@ R2D_EXPORT int r2d_ctx_last_counts(r2d_ctx* c, uint64_t* out)
     return R2D_OK;
 }
 
+// EntityFactory.makeDiscBody / makeRectangleBody (lib.zig:73-93) and the
+// shape constructors (Disc.zig:29-55, Rectangle.zig:33-64) for n bodies on
+// the device: the arithmetic of make_body_common, in the same order, written
+// into the world's device block at slots n0 .. n0 + n - 1 with ids id0 ..
+__global__ void k_make_bodies(uint32_t* blk, size_t cap, uint32_t n0, uint32_t n, const r2d_body_options* bo,
+                              const int32_t* kind, const float* dim_a, const float* dim_b, uint32_t id0)
+{
+    float* f = reinterpret_cast<float*>(blk);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const r2d_body_options o = bo[i];
+        const bool rect = kind[i] == 1;
+        const float a = dim_a[i], b = rect ? dim_b[i] : 0.f;
+        float mass, inertia;
+        if (!rect) {
+            mass = o.mass_kind ? o.mass_value : (float)M_PI * a * a * o.mass_value;
+            inertia = 0.5f * mass * a * a;
+        } else {
+            mass = o.mass_kind ? o.mass_value : a * b * o.mass_value;
+            inertia = mass * (a * a + b * b) / 12;
+        }
+        const size_t s = (size_t)n0 + i;
+        auto F = [&](int fld) -> float& { return f[(size_t)fld * cap + s]; };
+        F(F_PX) = o.pos_x; F(F_PY) = o.pos_y; F(F_ANG) = o.angle;
+        F(F_FX) = 0.f; F(F_FY) = 0.f; F(F_TQ) = 0.f;
+        F(F_MASS) = mass; F(F_INERTIA) = inertia; F(F_MU) = o.mu;
+        F(F_MX) = o.vel_x * mass; F(F_MY) = o.vel_y * mass;
+        F(F_L) = o.omega * inertia;
+        blk[(size_t)F_FLAGS * cap + s] = rect ? BF_RECT : 0u;
+        blk[(size_t)F_ID * cap + s] = id0 + i;
+        F(F_AX) = o.pos_x; F(F_AY) = o.pos_y;
+        if (!rect) {
+            F(F_RAD) = a; F(F_HW) = 0.f; F(F_HH) = 0.f;
+            F(F_AHW) = a; F(F_AHH) = a;
+        } else {
+            const float hw = a / 2, hh = b / 2;
+            F(F_RAD) = 0.f; F(F_HW) = hw; F(F_HH) = hh;
+            // Rectangle.updateAABB (Rectangle.zig:71-86)
+            v2 lv[4] = {mk(-hw, -hh), mk(-hw, hh), mk(hw, hh), mk(hw, -hh)};
+            const float c = cosf_r(o.angle), sn = sinf_r(o.angle);
+            float ew = 0.f, eh = 0.f;
+            for (int k = 0; k < 4; ++k) {
+                v2 r = rotcs(lv[k], c, sn);
+                if (r.x > ew) ew = r.x;
+                if (r.y > eh) eh = r.y;
+            }
+            F(F_AHW) = ew; F(F_AHH) = eh;
+        }
+    }
+}
+
+namespace {
+// bodies per r2d_make_bodies call from which the device builds them
+constexpr size_t kDeviceIngestMin = 32;
+
+// the per-context staging area of r2d_make_bodies' inputs
+int ensure_ingest(r2d_ctx* c, size_t n)
+{
+    if (n <= c->ingest_cap && c->ingest) return R2D_OK;
+    const size_t cap = std::max<size_t>(n, 2 * c->ingest_cap);
+    if (c->ingest) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->ingest));
+    }
+    c->ingest = nullptr;
+    c->ingest_cap = 0;
+    if (hipMalloc(&c->ingest, cap * (sizeof(r2d_body_options) + 12) + 1024) != hipSuccess) return R2D_E_OOM;
+    c->ingest_cap = cap;
+    return R2D_OK;
+}
+}  // namespace
+
 R2D_EXPORT int r2d_make_bodies(r2d_world* w, size_t n, const r2d_body_options* bo, const int32_t* kind,
                                const float* dim_a, const float* dim_b, uint32_t* ids_out)
 {
     if (!w || (n && (!bo || !kind || !dim_a || !dim_b))) return R2D_E_INVALID_ARG;
     for (size_t i = 0; i < n; ++i)
         if (kind[i] != 0 && kind[i] != 1) return R2D_E_INVALID_ARG;
-    int rc = begin_edit(w);
-    if (rc) return rc;
-    grow_host(w, (size_t)w->n + n);
-    for (size_t i = 0; i < n; ++i) {
-        uint32_t id = 0;
-        rc = make_body_common(w, &bo[i], kind[i] == 1, dim_a[i], kind[i] == 1 ? dim_b[i] : 0.f, &id);
+    if (w->in_service) return R2D_E_BUSY;
+    if (n < kDeviceIngestMin) {
+        int rc = begin_edit(w);
         if (rc) return rc;
+        grow_host(w, (size_t)w->n + n);
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t id = 0;
+            rc = make_body_common(w, &bo[i], kind[i] == 1, dim_a[i], kind[i] == 1 ? dim_b[i] : 0.f, &id);
+            if (rc) return rc;
+            if (ids_out) ids_out[i] = id;
+        }
+        return R2D_OK;
+    }
+    // On the device (appendBody order, lib.zig:66-71): the world's device
+    // block becomes the current copy with the new bodies at the end; the host
+    // mirror is refreshed when the host next reads it (ids, shapes and the
+    // id -> slot map are kept on the host at once).
+    r2d_ctx* c = w->ctx;
+    hipStream_t st = c->stream;
+    const size_t n0 = (size_t)w->n, need = n0 + n;
+    if (need > 0x7fffffffull || (uint64_t)w->current_body_id + n > 0xffffffffull) return R2D_E_CAPACITY;
+    int rc;
+    if (need > w->cap || need > w->dcap || !w->dblock) {
+        // grow: the current state through the host mirror into a larger block
+        if ((rc = download_bodies(w))) return rc;
+        grow_host(w, need);
+        if ((rc = ensure_world_device(w))) return rc;
+        if ((rc = upload_bodies(w))) return rc;
+    } else if (!w->dev_valid) {
+        if ((rc = upload_bodies(w))) return rc;
+    }
+    if ((rc = ensure_ingest(c, n))) return rc;
+    char* q = (char*)c->ingest;
+    r2d_body_options* dbo = (r2d_body_options*)q;
+    int32_t* dkind = (int32_t*)(q + align256(n * sizeof(r2d_body_options)));
+    float* da = (float*)((char*)dkind + align256(n * 4));
+    float* db = (float*)((char*)da + align256(n * 4));
+    HIPCHK(hipMemcpyAsync(dbo, bo, n * sizeof(r2d_body_options), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dkind, kind, n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(da, dim_a, n * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, dim_b, n * 4, hipMemcpyHostToDevice, st));
+    const uint32_t id0 = w->current_body_id;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_make_bodies, dim3(blocks), dim3(256), 0, st, w->dblock, w->dcap, (uint32_t)n0,
+                       (uint32_t)n, dbo, dkind, da, db, id0);
+    HIPCHK(hipGetLastError());
+    // host side: ids, the id -> slot map, the shapes the host keeps
+    if ((size_t)id0 + n > w->id2slot.size())
+        w->id2slot.resize(std::max<size_t>(64, ((size_t)id0 + n) * 2), -1);
+    uint32_t* hid = w->u(F_ID);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t id = id0 + (uint32_t)i;
+        const size_t slot = n0 + i;
+        w->id2slot[id] = (int32_t)slot;
+        hid[slot] = id;
+        const bool rect = kind[i] == 1;
+        w->width[slot] = rect ? dim_a[i] : 0.f;
+        w->height[slot] = rect ? dim_b[i] : 0.f;
         if (ids_out) ids_out[i] = id;
     }
+    w->n = (int32_t)need;
+    w->current_body_id = id0 + (uint32_t)n;
+    // the caller's arrays are read by the copies above
+    HIPCHK(hipStreamSynchronize(st));
+    w->host_valid = false;
+    w->dev_valid = true;
+    w->version += 1;
     return R2D_OK;
 }
 

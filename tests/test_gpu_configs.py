@@ -50,6 +50,28 @@ def test_config2_mixed_4096_full_horizon(gpu_ctx, oracle):
     assert mg.shape[0] > 1000 and bits_equal(mg, mo)
 
 
+def test_config2_tuned_hash_full_horizon(gpu_ctx, oracle):
+    """Configuration 2 as stated: 4,096 mixed bodies with the tuned spatial
+    hash (cell = twice the largest body extent, 2N buckets), 300 steps,
+    bit-exact in the reference order; and its coloured schedule against the
+    oracle's coloured schedule."""
+    sc = scenes.mixed(4096, seed=1, hash_cell="auto")
+    assert sc[0][0] == "hash" and 2.5 < sc[0][1] < 2.9
+    g, o = pair(gpu_ctx, oracle, sc)
+    for k in range(3):
+        step_both(g, o, 100)
+        assert_same(g, o, f"after {100 * (k + 1)} steps")
+    mg, mo = g.manifolds(), o.manifolds()
+    assert mg.shape[0] > 1000 and bits_equal(mg, mo)
+    gc, oc = pair(gpu_ctx, oracle, sc)
+    gc.set_schedule(1)
+    oc.set_schedule(1)
+    step_both(gc, oc, 60)
+    assert_same(gc, oc, "coloured, 60 steps")
+    for w in (g, gc):
+        w.close()
+
+
 def test_config3_batched_sample(gpu_ctx, oracle):
     """The benchmark batch (1,024 worlds, U[50,500] bodies, joints, springs)
     stepped together for 240 steps; eight worlds spread over the batch are
