@@ -177,6 +177,24 @@ int r2d_process(r2d_world* w, float dt, size_t sub_steps, size_t collision_iters
 int r2d_process_batch(r2d_ctx* ctx, r2d_world* const* worlds, size_t n, float dt, size_t sub_steps,
                       size_t collision_iters, size_t steps);
 
+/* The step of r2d_process without any allocation, for callers that cannot
+ * allocate device memory (code under hipGraph capture, serving loops and
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * step scratch is a static device array of 8 MiB that each GPU holds for
+ * the life of the process, shared by every call on that GPU: calls that
+ * overlap on one GPU race (calls from threads of this process are
+ * serialised by a per-device lock).  Bit-identical to r2d_process.  The
+ * world must be resident -- stepped once by r2d_process or
+ * r2d_process_batch since bodies, joints or force generators were last
+ * added or removed (r2d_set_body edits are fine) -- else R2D_E_INVALID_ARG;
+ * at most 1,024 bodies, 4,096 hash buckets and 2,048 joints, the reference
+ * order and no warm starting, else R2D_E_CAPACITY / R2D_E_INVALID_ARG.  A
+ * step that would need more than 16,384 cell entries, 65,536 candidate
+ * pairs, 4,096 manifolds or 6,144 solver items returns R2D_E_CAPACITY with
+ * the world unchanged.  r2d_get_manifolds reads this step's manifolds until
+ * the next r2d_process_static call on the same GPU (then R2D_E_RANGE). */
+int r2d_process_static(r2d_world* w, float dt, size_t sub_steps, size_t collision_iters);
+
 /* ----------------------------------------------------------- state I/O */
 /* solverGetNumBodies (src/wasm_root.zig:51-53). */
 int r2d_num_bodies(r2d_world* w, size_t* n_out);

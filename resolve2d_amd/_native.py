@@ -124,6 +124,7 @@ SIGNATURES = {
     "r2d_exclude_collision_pair": [P, U32, U32],
     "r2d_process": [P, F, SZ, SZ],
     "r2d_process_batch": [P, ctypes.POINTER(P), SZ, F, SZ, SZ, SZ],
+    "r2d_process_static": [P, F, SZ, SZ],
     "r2d_num_bodies": [P, SZP],
     "r2d_body_id_at": [P, SZ, U32P],
     "r2d_get_body": [P, U32, ctypes.POINTER(BodyState)],

@@ -15,8 +15,10 @@
 #include <chrono>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/r2d.h"
@@ -45,6 +47,117 @@ static void note_error(const char* what, hipError_t e, int line)
             return R2D_E_DEVICE;                         \
         }                                                \
     } while (0)
+
+// ------------------------------------------ step scratch of r2d_process_static
+// A file-scope device array holds the step scratch of r2d_process_static, the
+// entry point for callers that cannot allocate (code under graph capture,
+// serving loops that forbid allocation): every array of the fused step for
+// one world of up to 1,024 bodies.  Each GPU holds its own copy for the life
+// of the process; host code uses only the address hipGetSymbolAddress gave
+// for the current device (taken at the entry point's first call on a
+// device), never the array's name.  The array has external linkage: the HIP
+// runtime finds a device variable by its global symbol in the code object,
+// and a `static` one is a local symbol there (its lookup aborted the
+// process); hidden visibility keeps the name out of the library's exports.
+constexpr size_t kStaticScratchBytes = (size_t)8 << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_setup;
+void* g_scratch_ptr[kMaxDevices] = {};     // device address of `scratch`, per device
+std::mutex g_scratch_use[kMaxDevices];     // one r2d_process_static at a time per device
+uint64_t g_scratch_call[kMaxDevices] = {}; // number of the last call that wrote it, per device
+std::atomic<uint64_t> g_scratch_calls{0};
+
+// capacities of the static layout: bodies, buckets, cell entries, raw
+// candidate pairs, manifolds, joints, solver items
+constexpr uint32_t ST_N = FUSED_NMAX;
+constexpr uint32_t ST_B = 4 * ST_N;
+constexpr uint32_t ST_E = 16 * ST_N;
+constexpr uint32_t ST_R = 64 * ST_N;
+constexpr uint32_t ST_M = 4 * ST_N;
+constexpr uint32_t ST_J = 2 * ST_N;
+constexpr uint32_t ST_I = ST_M + ST_J;
+
+// Carves the step scratch of one world (every StepDev array, sized for the
+// capacities above) and its descriptor out of the static array at `base`.
+// Returns the bytes used; a null base only measures.
+size_t static_layout(char* base, StepDev* S, WorldDev** desc)
+{
+    const size_t N = ST_N, B = ST_B, E = ST_E, R = ST_R, M = ST_M, I = ST_I, W = 1;
+    size_t o = 0;
+    auto take = [&](auto** p, size_t bytes) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        *p = base ? reinterpret_cast<T>(base + o) : nullptr;
+        o += (bytes + 255) & ~(size_t)255;
+    };
+    StepDev& F = *S;
+    F = StepDev{};
+    take(desc, sizeof(WorldDev));
+    take(&F.bworld, N * 4);
+    take(&F.cx0, N * 8); take(&F.cy0, N * 8);
+    take(&F.cnx, N * 4); take(&F.cny, N * 4);
+    take(&F.ncell, (N + 1) * 4); take(&F.cell_off, (N + 1) * 4);
+    take(&F.cc, N * 4); take(&F.cs, N * 4); take(&F.sflag, N * 4);
+    take(&F.bkt_cnt, (B + 1) * 4); take(&F.bkt_off, (B + 1) * 4); take(&F.bkt_fill, (B + 1) * 4);
+    take(&F.bkt_items, (E + 1) * 4); take(&F.bkt_sorted, (E + 1) * 4);
+    take(&F.ent_body, (E + 1) * 4); take(&F.ent_bkt, (E + 1) * 4);
+    take(&F.ent_cnt, (E + 1) * 4); take(&F.ent_off, (E + 1) * 4);
+    take(&F.cand, (R + 1) * 16);
+    take(&F.cand_flag, (R + 1) * 4); take(&F.cand_idx, (R + 1) * 4);
+    take(&F.sat_nx, (R + 1) * 4); take(&F.sat_ny, (R + 1) * 4); take(&F.sat_info, (R + 1) * 4);
+    take(&F.m_ref, M * 4); take(&F.m_inc, M * 4);
+    take(&F.m_nx, M * 4); take(&F.m_ny, M * 4); take(&F.m_fric, M * 4);
+    take(&F.m_prev1, M * 4); take(&F.m_prev2, M * 4); take(&F.m_valid, M * 4);
+    take(&F.m_pt, (size_t)MP_NFIELDS * 2 * M * 4);
+    take(&F.w_man_begin, (W + 1) * 4); take(&F.w_item_base, (W + 1) * 4);
+    take(&F.w_status, (W + 1) * 4); take(&F.w_order, (W + 1) * 4);
+    take(&F.it_r0, I * 4); take(&F.it_r1, I * 4); take(&F.it_k0, I * 4); take(&F.it_k1, I * 4);
+    take(&F.deg, (N + 1) * 4); take(&F.deg_off, (N + 1) * 4); take(&F.deg_fill, (N + 1) * 4);
+    take(&F.lists, 2 * I * 4);
+    take(&F.it_color, I * 4); take(&F.it_perm, I * 4);
+    take(&F.recs, I * 128);
+    take(&F.cmask, 2 * N * 8);
+    take(&F.g_im, N * 4); take(&F.g_ii, N * 4); take(&F.g_cnt, N * 4);
+    take(&F.cmap, N * 4); take(&F.cinv, N * 4); take(&F.w_nc, (W + 1) * 4);
+    take(&F.counters, 64 * 4);
+    take(&F.partials, 1024 * 4);
+    take(&F.wscr, W * sizeof(WorldScratch));
+    take(&F.fpool, 8 * 4);
+    take(&F.w_done, W * 4);
+    take(&F.w_cnt, W * 32);
+    take(&F.w_man_end, W * 4);
+    F.cap_e = ST_E; F.cap_r = ST_R; F.cap_m = ST_M; F.cap_i = ST_I;
+    F.nb = 0; F.nworlds = 1; F.nbuckets = 0; F.njoints = 0;
+    F.worlds = base ? *desc : nullptr;
+    F.w_tstamp = nullptr; F.tcap = 0; F.w_prof = nullptr;
+    return o;
+}
+
+// the address of `scratch` on the current device, taken once per device
+int static_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return R2D_E_CAPACITY;
+    std::lock_guard<std::mutex> g(g_scratch_setup);
+    if (g_scratch_ptr[device]) return R2D_OK;
+    StepDev S;
+    WorldDev* d = nullptr;
+    if (static_layout(nullptr, &S, &d) > kStaticScratchBytes) return R2D_E_CAPACITY;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();   // not an error of a later call
+        std::snprintf(g_last_error, sizeof g_last_error, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                      hipGetErrorString(e));
+        return R2D_E_DEVICE;
+    }
+    g_scratch_ptr[device] = p;
+    return R2D_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -114,6 +227,7 @@ struct r2d_world {
     uint32_t batch_index = 0;
     bool fused_src = false;   // ... computed by the fused kernel (their range: w_man_begin / w_man_end)
     bool prestepped = true;   // ... by a step with sub-steps (a step without runs no preStep)
+    uint64_t sepoch = 0;      // ... or by r2d_process_static call number `sepoch` (its scratch holds them)
     // capacities of the world's slices of the fused step scratch (cell
     // entries, raw candidates, manifolds, solver items); 0: initial estimate
     uint32_t fz[4] = {0, 0, 0, 0};
@@ -1111,6 +1225,7 @@ int batch_finish(r2d_ctx* c, Batch& b)
         b.worlds[i]->batch_index = (uint32_t)i;
         b.worlds[i]->fused_src = b.fused;
         b.worlds[i]->prestepped = b.sub > 0;
+        b.worlds[i]->sepoch = 0;
     }
     return b.first_err;
 }
@@ -1741,6 +1856,124 @@ R2D_EXPORT int r2d_process_batch(r2d_ctx* c, r2d_world* const* worlds, size_t n,
     return step_batch(c, worlds, n, dt, sub_steps, iters, steps);
 }
 
+R2D_EXPORT int r2d_process_static(r2d_world* w, float dt, size_t sub_steps, size_t iters)
+{
+    if (!w) return R2D_E_INVALID_ARG;
+    if (w->in_service) return R2D_E_BUSY;
+    r2d_ctx* c = w->ctx;
+    // the world must be resident: its device block and topology uploaded by
+    // an earlier step and no bodies added since (state edits are fine)
+    if (w->n > 0 && (!w->dblock || w->dcap != w->cap || w->topo_dirty)) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "r2d_process_static: the world is not resident on the device (step it once with r2d_process "
+                      "after adding bodies, joints or force generators)");
+        return R2D_E_INVALID_ARG;
+    }
+    if (w->sched != 0 || w->warm > 0.0f) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "r2d_process_static: only the reference order without warm starting");
+        return R2D_E_INVALID_ARG;
+    }
+    const uint32_t n = (uint32_t)w->n, tb = table_size(w), nj = (uint32_t)w->joints.size();
+    if (n > ST_N || tb > ST_B || nj > ST_J) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "r2d_process_static: %u bodies, %u buckets, %u joints; at most %u, %u, %u", n, tb, nj, ST_N,
+                      ST_B, ST_J);
+        return R2D_E_CAPACITY;
+    }
+    // the reference raises InvalidRigidBodyId from the first constraint solve
+    if (sub_steps > 0 && iters > 0)
+        for (const HJoint& h : w->joints)
+            if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
+    int variant = c->solve_variant >= 0 ? c->solve_variant : 1;
+    if (!fused_available(variant)) variant = 11;
+    size_t lds = fused_lds_bytes(n, tb, fused_slots(variant), fused_threads(variant));
+    if (lds > kFusedLdsMax) {
+        variant = 11;
+        lds = fused_lds_bytes(n, tb, fused_slots(variant), fused_threads(variant));
+        if (lds > kFusedLdsMax) return R2D_E_CAPACITY;
+    }
+    if (n == 0) {   // nothing moves and no manifold exists
+        w->epoch = 0;
+        w->sepoch = 0;
+        return R2D_OK;
+    }
+    const int dev = c->device;
+    if (dev < 0 || dev >= kMaxDevices) return R2D_E_CAPACITY;
+    if (!g_scratch_ptr[dev]) {
+        // first call on this device: the static scratch's address here
+        HIPCHK(hipSetDevice(dev));
+        const int rc = static_scratch_setup(dev);
+        if (rc) return rc;
+    }
+    char* base = (char*)g_scratch_ptr[dev];
+    std::lock_guard<std::mutex> guard(g_scratch_use[dev]);
+    hipStream_t st = c->stream;
+    if (!w->dev_valid) {
+        HIPCHK(hipMemcpyAsync(w->dblock, w->block.data(), (size_t)F_NFIELDS * w->cap * 4, hipMemcpyHostToDevice, st));
+        w->dev_valid = true;
+    }
+    if (w->desc_version != w->version) {
+        w->desc = make_desc(w);
+        w->desc_version = w->version;
+    }
+    StepDev F;
+    WorldDev* ddesc = nullptr;
+    static_layout(base, &F, &ddesc);
+    F.nb = n; F.nbuckets = tb; F.njoints = nj;
+    // host values the asynchronous uploads read; the call waits for the
+    // stream before they go out of scope
+    WorldDev d = w->desc;
+    d.body_base = 0; d.bucket_base = 0; d.joint_base = 0;
+    const WorldScratch x{0u, ST_E, 0u, ST_R, 0u, ST_M, 0u, ST_I};
+    // free ends at the capacities: a world that outgrows a slice stops
+    // before its step (WS_OVERFLOW) instead of taking more
+    const uint32_t pool[8] = {ST_E + 1, ST_R + 1, ST_M, ST_I, ST_E + 1, ST_R + 1, ST_M, ST_I};
+    const uint32_t order = 0u;
+    HIPCHK(hipMemcpyAsync(ddesc, &d, sizeof d, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(F.wscr, &x, sizeof x, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(F.fpool, pool, sizeof pool, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(F.w_order, &order, 4, hipMemcpyHostToDevice, st));
+    // what the fused kernel expects to start at zero; the array keeps the
+    // previous call's contents otherwise
+    HIPCHK(hipMemsetAsync(F.counters, 0, 64 * 4, st));
+    HIPCHK(hipMemsetAsync(F.w_done, 0, 4, st));
+    HIPCHK(hipMemsetAsync(F.w_status, 0, 4, st));
+    HIPCHK(hipMemsetAsync(F.w_cnt, 0, 32, st));
+    (void)hipGetLastError();
+    launch_fused(F, variant, 1u, dt, (uint32_t)sub_steps, (uint32_t)iters, 1u, lds, st);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        note_error("launch_fused (r2d_process_static)", le, __LINE__);
+        return R2D_E_DEVICE;
+    }
+    int32_t status = 0;
+    uint32_t cnt[8] = {};
+    HIPCHK(hipMemcpyAsync(&status, F.w_status, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cnt, F.w_cnt, 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    g_scratch_call[dev] = g_scratch_calls.fetch_add(1) + 1;
+    w->epoch = 0;
+    w->sepoch = 0;
+    if (status == WS_OVERFLOW) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "r2d_process_static: the step needs %u cell entries, %u candidate pairs, %u manifolds, %u "
+                      "solver items; the static scratch holds %u, %u, %u, %u (the world is unchanged)",
+                      cnt[4], cnt[5], cnt[6], cnt[7], ST_E, ST_R, ST_M, ST_I);
+        return R2D_E_CAPACITY;
+    }
+    w->host_valid = false;
+    w->sepoch = g_scratch_call[dev];
+    w->prestepped = sub_steps > 0;
+    if (status == WS_NONFINITE) return R2D_E_NONFINITE;
+    if (status == WS_STALL) {
+        std::snprintf(g_last_error, sizeof g_last_error, "solver schedule stalled (r2d_process_static)");
+        return R2D_E_DEVICE;
+    }
+    if (status != WS_OK) return R2D_E_RANGE;
+    return R2D_OK;
+}
+
 R2D_EXPORT int r2d_num_bodies(r2d_world* w, size_t* n)
 {
     if (!w || !n) return R2D_E_INVALID_ARG;
@@ -1802,18 +2035,34 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
         int rc = server_collect(c->server);
         if (rc) return rc;
     }
-    if (w->epoch == 0 || w->epoch != c->epoch) {
+    // manifolds of r2d_process_static: in the static scratch until the next call on the device
+    std::unique_lock<std::mutex> guard;
+    StepDev S;
+    const bool from_static = w->epoch == 0 && w->sepoch != 0;
+    if (from_static) {
+        guard = std::unique_lock<std::mutex>(g_scratch_use[c->device]);
+        if (g_scratch_call[c->device] != w->sepoch) {
+            if (n) *n = 0;
+            std::snprintf(g_last_error, sizeof g_last_error,
+                          "the manifolds of this world's r2d_process_static step were overwritten by a later call");
+            return R2D_E_RANGE;
+        }
+        WorldDev* dd = nullptr;
+        static_layout((char*)g_scratch_ptr[c->device], &S, &dd);
+    } else if (w->epoch == 0 || w->epoch != c->epoch) {
         if (n) *n = 0;
         return w->epoch == 0 ? R2D_OK : R2D_E_RANGE;
-    }
-    // manifolds of the fused kernel are in the world's slice of the fused arena
-    const StepDev S = w->fused_src ? fused_view(c) : c->S;
-    uint32_t b[2];
-    if (w->fused_src) {
-        HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b + 1, S.w_man_end + w->batch_index, 4, hipMemcpyDeviceToHost));
     } else {
-        HIPCHK(hipMemcpy(b, S.w_man_begin + w->batch_index, 8, hipMemcpyDeviceToHost));
+        // manifolds of the fused kernel are in the world's slice of the fused arena
+        S = w->fused_src ? fused_view(c) : c->S;
+    }
+    const uint32_t bidx = from_static ? 0u : w->batch_index;
+    uint32_t b[2];
+    if (from_static || w->fused_src) {
+        HIPCHK(hipMemcpy(b, S.w_man_begin + bidx, 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b + 1, S.w_man_end + bidx, 4, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(hipMemcpy(b, S.w_man_begin + bidx, 8, hipMemcpyDeviceToHost));
     }
     size_t cnt = b[1] - b[0];
     if (n) *n = cnt;
@@ -1832,9 +2081,9 @@ R2D_EXPORT int r2d_get_manifolds(r2d_world* w, r2d_manifold_state* out, size_t c
         HIPCHK(hipMemcpy(pt.data() + (size_t)f * m, S.m_pt + (size_t)f * S.cap_m + b[0], m * 4, hipMemcpyDeviceToHost));
     int rc = download_bodies(w);
     if (rc) return rc;
-    uint32_t wb[1];
+    uint32_t wb[1] = {0u};
     // batch body base of this world = first body's batch index
-    HIPCHK(hipMemcpy(wb, &c->dworlds[w->batch_index].body_base, 4, hipMemcpyDeviceToHost));
+    if (!from_static) HIPCHK(hipMemcpy(wb, &c->dworlds[w->batch_index].body_base, 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < m; ++i) {
         r2d_manifold_state& o = out[i];
         std::memset(&o, 0, sizeof o);
