@@ -74,6 +74,21 @@ def traffic_for(name, window, args):
     return t.get(f"{name}/worlds{args.worlds}/steps{window[0]}-{window[1]}/seed{args.seed}")
 
 
+LDS_FILE = os.path.join(ROOT, "profiles", "lds.json")
+
+
+def lds_counted_for(name, window, args):
+    """LDS instructions of one launch of the kernel on this workload and
+    window as counted by rocprofv3 (SQ_INSTS_LDS) and the bytes they move at
+    least, from the committed measurement; None when there is none."""
+    try:
+        with open(LDS_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get(f"{name}/worlds{args.worlds}/steps{window[0]}-{window[1]}/seed{args.seed}")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -924,6 +939,12 @@ def main():
     dom = max(per_step, key=per_step.get)
     achieved = launch_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
     lds_achieved = lds_bytes / launch_s / 1e9 if launch_s > 0 else 0.0
+    lds_counted = lds_counted_for(kernel, window, args)
+    if lds_counted and launch_s > 0:
+        gbs = lds_counted["bytes_lower_bound"] / launch_s / 1e9
+        lds_counted = dict(lds_counted, achieved_lower_bound=gbs, frac_lower_bound=gbs / LDS_PEAK_GBS,
+                           note="SQ_INSTS_LDS of the committed rocprofv3 run of this window, x 256 B (64 lanes x "
+                                "4 B, the least a wave's LDS instruction moves), over this run's launch time")
     copy = hbm_copy_rate()
 
     single = None
@@ -1055,7 +1076,8 @@ def main():
                      "measured_copy_rate": copy,
                      "frac_of_measured_copy": achieved / copy["value"] if copy else None,
                      "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes},
+                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes,
+                             "counted": lds_counted},
                      "dominant_phase": dom},
         "cpu_baseline": cpu,
         "setup_s": setup_s,
