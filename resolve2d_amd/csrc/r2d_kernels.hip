@@ -866,17 +866,112 @@ constexpr uint32_t COLOR_SPIN_LIMIT = 1u << 22;
 // until every earlier item on its bodies has taken its colour), the pattern
 // of the solver; the order is a stable block-wide counting sort.  Worlds in
 // the reference order get colour 0 and the identity order.
-__global__ __launch_bounds__(256) void k_color(StepDev S)
+// Reference order: the execution positions of a world's items (it_perm) by
+// level -- an item's level is 1 + the larger level of its predecessors, the
+// items just before it in its ordering bodies' lists.  Items of one level
+// share no ordering body, so their order among themselves is free.  The solve's lanes
+// take positions t, t + T, ... of each pass in turn; with positions in item
+// order a lane's next item often waits behind a long chain while items at
+// later positions could run, and the lane holds them back.  Any order that
+// lists every item after its predecessors keeps the dataflow free of
+// deadlock (the earliest unfinished item in (pass, position) order is always
+// its lane's current item, and its predecessors have run), and the counters
+// still give each body its items in the reference order, so the results are
+// the same bits; only which lane runs what, and when, changes.  Work arrays
+// in dynamic LDS (lds_words): levels, packed predecessors, level counts,
+// 3 nit + 2 words; a world with more items keeps the item order.  Levels
+// are found by sweeps in which each thread walks a contiguous chunk of items
+// in item order (a predecessor in the same chunk is seen in the same sweep),
+// until a sweep changes nothing.
+__device__ void level_order(StepDev& S, uint32_t w, uint32_t i0, uint32_t nit, uint32_t* lds, uint32_t lds_words,
+                            uint32_t* flags)
 {
+    const uint32_t T = blockDim.x, t = threadIdx.x;
+    if (nit < 2u || nit >= 0xffffu || 3u * nit + 2u > lds_words) {
+        for (uint32_t p = t; p < nit; p += T) S.it_perm[i0 + p] = p;
+        return;
+    }
+    uint32_t* lev = lds;
+    uint32_t* pred = lds + nit;
+    uint32_t* hist = lds + 2u * nit;
+    for (uint32_t p = t; p < nit; p += T) {
+        const uint32_t it = i0 + p;
+        const int32_t r0 = S.it_r0[it], r1 = S.it_r1[it];
+        uint32_t q0 = 0xffffu, q1 = 0xffffu;
+        if (r0 >= 0 && S.it_k0[it] > 0u) q0 = (S.lists[S.deg_off[r0] + S.it_k0[it] - 1u] >> 1) - i0;
+        if (r1 >= 0 && S.it_k1[it] > 0u) q1 = (S.lists[S.deg_off[r1] + S.it_k1[it] - 1u] >> 1) - i0;
+        pred[p] = q0 | (q1 << 16);
+        lev[p] = 1u;
+    }
+    if (t == 0) { flags[0] = 0u; flags[1] = 0u; flags[2] = 0u; }
+    __syncthreads();
+    const uint32_t chunk = (nit + T - 1) / T;
+    const uint32_t lo = umin(t * chunk, nit), hi = umin(lo + chunk, nit);
+    // sweep p reports a change in flags[p % 3] and clears the flag of sweep
+    // p + 1, which every thread read before the previous barrier
+    for (uint32_t sweep = 0;; ++sweep) {
+        bool changed = false;
+        for (uint32_t p = lo; p < hi; ++p) {
+            const uint32_t pw = pred[p];
+            const uint32_t q0 = pw & 0xffffu, q1 = pw >> 16;
+            uint32_t l = 1u;
+            if (q0 != 0xffffu)
+                l = umax(l, 1u + __hip_atomic_load(&lev[q0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (q1 != 0xffffu)
+                l = umax(l, 1u + __hip_atomic_load(&lev[q1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (l != __hip_atomic_load(&lev[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                __hip_atomic_store(&lev[p], l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                changed = true;
+            }
+        }
+        if (changed) flags[sweep % 3u] = 1u;
+        if (t == 0) flags[(sweep + 1u) % 3u] = 0u;
+        __syncthreads();
+        if (!flags[sweep % 3u]) break;
+    }
+    // counting sort by level: counts, exclusive offsets, then each item takes
+    // the next free position of its level
+    if (t == 0) flags[0] = 0u;
+    __syncthreads();
+    for (uint32_t p = lo; p < hi; ++p) atomicMax(&flags[0], lev[p]);
+    __syncthreads();
+    const uint32_t top = flags[0];
+    for (uint32_t l = t; l <= top + 1u; l += T) hist[l] = 0u;
+    __syncthreads();
+    for (uint32_t p = t; p < nit; p += T) atomicAdd(&hist[lev[p]], 1u);
+    __syncthreads();
+    if (t < 64u) {
+        const uint32_t len = top + 1u, c = (len + 63u) / 64u;
+        const uint32_t a = umin(t * c, len), b = umin(a + c, len);
+        uint32_t sum = 0u;
+        for (uint32_t l = a; l < b; ++l) sum += hist[l];
+        uint32_t run = wave_incl_scan(sum) - sum;
+        for (uint32_t l = a; l < b; ++l) {
+            const uint32_t x = hist[l];
+            hist[l] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = t; p < nit; p += T) S.it_perm[i0 + atomicAdd(&hist[lev[p]], 1u)] = p;
+    __syncthreads();
+    (void)w;
+}
+
+__global__ __launch_bounds__(256) void k_color(StepDev S, uint32_t lds_words)
+{
+    extern __shared__ __align__(16) uint32_t dyn[];
     __shared__ uint32_t s_scan[256];
     __shared__ uint32_t s_maxc, s_abort, s_base;
+    __shared__ uint32_t s_flags[3];
     if (S.counters[C_OVERFLOW]) return;
     const uint32_t w = blockIdx.x;
     const WorldDev& D = S.worlds[w];
     const uint32_t i0 = S.w_item_base[w], nit = S.w_item_base[w + 1] - i0;
     const uint32_t T = blockDim.x, t = threadIdx.x;
     if (D.sched == 0) {
-        for (uint32_t p = t; p < nit; p += T) { S.it_color[i0 + p] = 0u; S.it_perm[i0 + p] = p; }
+        for (uint32_t p = t; p < nit; p += T) S.it_color[i0 + p] = 0u;
+        level_order(S, w, i0, nit, dyn, lds_words, s_flags);
         return;
     }
     const uint32_t base = D.body_base;
@@ -3527,7 +3622,12 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
         tmark(ss, "k_list_fill", st);
         hipLaunchKernelGGL(k_list_rank<false>, dim3(blocks_for(S.nb, BS)), dim3(BS), 0, st, S);
         tmark(ss, "k_list_rank<false>", st);
-        hipLaunchKernelGGL(k_color, dim3(S.nworlds), dim3(256), 0, st, S);
+        {
+            // the level order's work arrays: 96 KB for batches of up to 256
+            // worlds (items per world up to ~8,000), 32 KB for larger ones
+            const size_t lds = S.nworlds <= 256u ? 96u * 1024u : 32u * 1024u;
+            hipLaunchKernelGGL(k_color, dim3(S.nworlds), dim3(256), lds, st, S, (uint32_t)(lds / 4u));
+        }
         tmark(ss, "k_color", st);
         hipLaunchKernelGGL(k_world_order, dim3(1), dim3(1024), 0, st, S);
         tmark(ss, "k_world_order", st);
@@ -3630,6 +3730,7 @@ void launch_phase(const StepDev& S, const Streams& ss, int phase, float dt, uint
 
 void set_solve_smem_limit()
 {
+    (void)hipFuncSetAttribute((const void*)k_color, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_big_chain<256, false, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_solve<REC_WIDE_THREADS, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
