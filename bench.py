@@ -59,7 +59,8 @@ CPU_SHARE = 16
 
 # fused kernel shape of each solve variant: <slots, workgroup size, waves per SIMD>
 FUSED_SHAPES = {1: "4,256,3", 2: "2,512,3", 3: "3,256,3", 4: "4,256,2", 5: "8,128,2", 7: "4,256,3",
-                9: "0,256,3", 10: "0,256,4", 11: "0,192,3", 12: "0,128,2"}
+                9: "0,256,3", 10: "0,256,4", 11: "0,192,3", 12: "0,128,2", 13: "0,512,2",
+                14: "0,768,3"}
 
 
 def traffic_for(name, window, args):
@@ -876,6 +877,13 @@ def main():
     for w in worlds:
         w.close()
 
+    # one world per call (plain mode, the single-world configurations, the
+    # accuracy runs): a context of its own, whose launch shape is the
+    # library's default for a few worlds (the batch's tuned shape is for
+    # 1,024 worlds sharing the CUs)
+    octx = Context(0)
+    octx.set_solve_variant(args.variant)
+
     # plain: the same scenes, one world per launch (the reference's
     # one-world process call), over the same window and over the horizon
     plain = None
@@ -883,16 +891,16 @@ def main():
         pick = specs[:: max(1, len(specs) // args.plain_worlds)][:args.plain_worlds]
         pw_bs, pw_t, ph_bs, ph_t = 0, 0.0, 0, 0.0
         for sc in pick:
-            w = GpuWorld(ctx)
+            w = GpuWorld(octx)
             apply_scene(sc, w)
-            ctx.sync()
+            octx.sync()
             a = time.perf_counter()
             for s in range(1, window[1] + 1):
                 if s == window[0]:
-                    ctx.sync()
+                    octx.sync()
                     b0 = time.perf_counter()
                 checked(w.process(dt, S, K_IT), "plain step")
-            ctx.sync()
+            octx.sync()
             b1 = time.perf_counter()
             pw_t += b1 - b0
             pw_bs += count_bodies(sc) * K
@@ -902,7 +910,9 @@ def main():
             w.close()
         plain = {"value": pw_bs / pw_t, "unit": "body-steps/s",
                  "sample": f"{len(pick)} worlds spread over the batch, one world per launch (r2d_process per "
-                           f"step); steps {window[0]}-{window[1]} timed"}
+                           f"step); steps {window[0]}-{window[1]} timed",
+                 "launch": "the library's default shape for one world (k_fused<0,768,3>: a lane per solver item "
+                           "up to 768 items)" if args.variant < 0 else f"variant {args.variant}"}
         if ph_t:
             plain["horizon"] = {"value": ph_bs / ph_t, "unit": "body-steps/s",
                                 "sample": f"the same worlds over steps 1-{window[1]}"}
@@ -949,7 +959,7 @@ def main():
 
     single = None
     if not args.no_single:
-        single = single_world(ctx, args.single_steps, not args.reference_hash)
+        single = single_world(octx, args.single_steps, not args.reference_hash)
 
     warm = None
     if not args.no_warm:
@@ -960,16 +970,16 @@ def main():
     large = None
     if not args.no_large:
         log("large constrained world")
-        large = large_world(ctx, 120, {1, 10, 120}, not args.no_accuracy)
+        large = large_world(octx, 120, {1, 10, 120}, not args.no_accuracy)
 
     acc = None
     if not args.no_accuracy:
         log("accuracy")
         sample = list(range(0, args.worlds, max(1, args.worlds // 4)))[:4]
-        errs, bit_eq, pen = accuracy(ctx, specs, sample, HORIZON, {1, 60, HORIZON})
+        errs, bit_eq, pen = accuracy(octx, specs, sample, HORIZON, {1, 60, HORIZON})
         acc = {"worlds": sample, "bit_equal_to_f32_oracle": bit_eq, "tolerance_vs_reference_f32": 1e-5,
                "vs_f64_oracle": errs, "max_penetration_m": pen, "batch_max_penetration_m": batch_pen,
-               "pyramid_600": pyramid_energy(ctx, 600)}
+               "pyramid_600": pyramid_energy(octx, 600)}
 
     crit = None
     if not args.no_critical:
@@ -978,13 +988,13 @@ def main():
         log("critical paths")
         crit = {}
         cpu1 = acc["pyramid_600"]["cpu_single"]["ms_per_step"] if acc else None
-        crit["pyramid_20_rows"] = critical_path(ctx, "pyramid", scenes.pyramid(20), 590, 10, cpu1)
+        crit["pyramid_20_rows"] = critical_path(octx, "pyramid", scenes.pyramid(20), 590, 10, cpu1)
         if single:
             cpu2 = single["cpu_single"]["ms_per_step"]
             ops2 = scenes.mixed(4096, seed=1, hash_cell=None if args.reference_hash else "auto")
-            crit["world_4096"] = critical_path(ctx, "4,096-body world", ops2, args.single_steps - 10, 10, cpu2)
+            crit["world_4096"] = critical_path(octx, "4,096-body world", ops2, args.single_steps - 10, 10, cpu2)
         if large:
-            crit["world_65540"] = critical_path(ctx, "65,540-body world", scenes.large_constrained(), 110, 10,
+            crit["world_65540"] = critical_path(octx, "65,540-body world", scenes.large_constrained(), 110, 10,
                                                 large["cpu_single"]["ms_per_step"])
         crit["note"] = ("depth: longest dependency chain of a sub-step's solver items (4 passes of joints then "
                         "manifolds in discovery order, items ordered by their shared dynamic or jointed bodies), from "
@@ -1011,7 +1021,7 @@ def main():
                                                  "admitted_scenes_per_s", "p50_step_ms", "p99_step_ms",
                                                  "p50_request_ms", "p99_request_ms")}
         serve["offered_body_steps_per_s"] = args.serve_load * capacity
-        pv, sizes = plain_served(ctx, arrivals, HORIZON, args.serve_plain_scenes)
+        pv, sizes = plain_served(octx, arrivals, HORIZON, args.serve_plain_scenes)
         serve["plain"] = {"value": pv, "unit": "body-steps/s",
                           "sample": f"{len(sizes)} served scenes of {sizes} bodies, {HORIZON} steps each, one "
                                     f"world per launch"}
@@ -1083,6 +1093,7 @@ def main():
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)
+    octx.close()
     ctx.close()
 
 

@@ -24,10 +24,10 @@ import numpy as np
 
 from . import _native as N
 
-SOLVE_VARIANTS = 13
+SOLVE_VARIANTS = 15
 # variants that differ on the phase kernels (9 and 10 are variant 6 there, 12
-# is variant 7; 11 is the record kernel with 192 threads)
-PHASE_VARIANTS = tuple(range(9)) + (11,)
+# is variant 7; 11 is the record kernel with 192 threads, 13 with 512)
+PHASE_VARIANTS = tuple(range(9)) + (11, 13)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "resolve2d_amd", "autotune.json")
 
 
@@ -81,7 +81,7 @@ def save_cache(cache: Dict[str, dict], path: str = DEFAULT_CACHE):
 
 # variants with a fused kernel (the register-slot shapes, lockstep walk aside,
 # and the records-only shapes)
-FUSED_VARIANTS = (1, 2, 3, 4, 5, 7, 9, 10, 11, 12)
+FUSED_VARIANTS = (1, 2, 3, 4, 5, 7, 9, 10, 11, 12, 13, 14)
 
 
 def candidates(variants: Optional[Iterable[int]] = None, lds_limits: Iterable[int] = (0, 1024),

@@ -613,6 +613,12 @@ void destroy_graphs(r2d_ctx* c)
     c->graph_key.clear();
 }
 
+// The launch shape of a batch the caller has not tuned: a few worlds (each
+// alone on a CU, latency-bound on their chains) take the 512-lane record
+// kernel; more take the register-slot kernel for small worlds and the
+// 256-thread record kernel for the others.
+int default_variant(size_t nworlds) { return nworlds <= WIDE_MAX_WORLDS ? WIDE_VARIANT : 1; }
+
 template <class Fn>
 bool capture_graph(r2d_ctx* c, hipGraphExec_t* out, Fn&& enqueue)
 {
@@ -931,9 +937,8 @@ int batch_prepare(r2d_ctx* c, r2d_world* const* worlds, size_t nw, float dt, siz
     if (b.N >= 0x7fffffffull || b.B >= 0x7fffffffull) return R2D_E_CAPACITY;
 
     // solve variant: context override (autotuner), environment, then the
-    // default (register-slot kernel for small worlds, 256-thread record
-    // kernels for the others)
-    int variant = 1;
+    // default (default_variant)
+    int variant = default_variant(nw);
     if (c->solve_variant >= 0) variant = c->solve_variant;
     if (const char* e = std::getenv("R2D_SOLVE_VARIANT")) {
         int v = std::atoi(e);
@@ -1885,7 +1890,7 @@ R2D_EXPORT int r2d_process_static(r2d_world* w, float dt, size_t sub_steps, size
     if (sub_steps > 0 && iters > 0)
         for (const HJoint& h : w->joints)
             if (w->slot_of(h.id1) < 0 || (h.type <= 1 && w->slot_of(h.id2) < 0)) return R2D_E_INVALID_ID;
-    int variant = c->solve_variant >= 0 ? c->solve_variant : 1;
+    int variant = c->solve_variant >= 0 ? c->solve_variant : default_variant(1);
     if (!fused_available(variant)) variant = 11;
     size_t lds = fused_lds_bytes(n, tb, fused_slots(variant), fused_threads(variant));
     if (lds > kFusedLdsMax) {

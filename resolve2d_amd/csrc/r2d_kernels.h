@@ -24,7 +24,12 @@ enum { PH_BROAD = 0, PH_NARROW, PH_SCHED, PH_SOLVE, PH_N };
 
 // solve-kernel variants (register-slot kernel shape + record-kernel workgroup
 // size; table in r2d_kernels.hip)
-constexpr int SOLVE_VARIANTS = 13;
+constexpr int SOLVE_VARIANTS = 15;
+// the default for batches of at most WIDE_MAX_WORLDS worlds (each alone on a
+// CU): records only, 768-thread workgroups, so a world of up to 768 solver
+// items holds one item per lane and its chain never picks up a record
+constexpr int WIDE_VARIANT = 14;
+constexpr size_t WIDE_MAX_WORLDS = 64;
 // the variant every batch with a warm-started world takes (not tunable)
 constexpr int WARM_VARIANT = SOLVE_VARIANTS;
 constexpr int SOLVE_TABLE = SOLVE_VARIANTS + 1;

@@ -3401,6 +3401,7 @@ struct SolveVariant {
     RecFn rec_fn;        // record kernel (LDS body state) for the worlds the slot kernel leaves
     unsigned threads;
     FusedFn fused_fn;    // whole steps per workgroup with the slot kernel's shape (nullptr: none)
+    unsigned fused_t;    // the fused kernel's workgroup size when it is neither slot_threads nor threads
 };
 // solve variants: the register-slot kernel's slots x workgroup size, and the
 // record kernel's workgroup size for worlds with more items than slots
@@ -3429,6 +3430,16 @@ static const SolveVariant kSolveVariants[SOLVE_TABLE] = {
     // records only, 128-thread workgroups at 2 waves per SIMD: the same four
     // worlds per CU with the full register file (no spills)
     {nullptr, 0, 0, k_solve<128, 2>, 128, k_fused<0, 128, 2, false>},
+    // records only, 512-thread workgroups at 2 waves per SIMD (the full
+    // register file): for a few worlds, each alone on a CU, whose chains are
+    // latency-bound; up to 512 items no lane ever changes record
+    {nullptr, 0, 0, k_solve<REC_WIDE_THREADS, 1>, REC_WIDE_THREADS, k_fused<0, 512, 2, false>},
+    // WIDE_VARIANT: 768 lanes at 3 waves per SIMD (the register budget of
+    // variant 11), up to 768 items without a pickup.  A lane that holds two
+    // items serialises them, which can triple a chain (the 20-row pyramid's
+    // 520-590 items: 148 links per sub-step with a lane per item, 403-437
+    // with 512 lanes), so the widest shape that does not spill wins
+    {nullptr, 0, 0, k_solve<REC_WIDE_THREADS, 1>, REC_WIDE_THREADS, k_fused<0, 768, 3, false>, 768},
     // WARM_VARIANT: batches with a warm-started world (the shape of variant
     // 11 with the warm-start passes compiled in; the kernels above carry none)
     {nullptr, 0, 0, k_solve<192, 3, true>, 192, k_fused<0, 192, 3, false, true>},
@@ -3442,12 +3453,18 @@ bool fused_slots(int v)
 uint32_t fused_threads(int v)
 {
     const SolveVariant& sv = kSolveVariants[v];
+    if (sv.fused_t) return sv.fused_t;
     return sv.slot_fn ? (uint32_t)sv.slot_threads : (uint32_t)sv.threads;
 }
 
+// a fused kernel runs only at the workgroup size it was compiled for (its
+// loops stride by MAXT): set_solve_smem_limit checks each against the width
+// it would be launched with, and a mismatch leaves the variant without one
+static bool g_fused_width_ok[SOLVE_TABLE];
+
 bool fused_available(int v)
 {
-    return v >= 0 && v < SOLVE_TABLE && kSolveVariants[v].fused_fn != nullptr;
+    return v >= 0 && v < SOLVE_TABLE && kSolveVariants[v].fused_fn != nullptr && g_fused_width_ok[v];
 }
 
 hipError_t fused_attributes(int v, hipFuncAttributes* out)
@@ -3747,9 +3764,13 @@ void set_solve_smem_limit()
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)kSolveVariants[v].rec_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        if (kSolveVariants[v].fused_fn)
+        if (kSolveVariants[v].fused_fn) {
             (void)hipFuncSetAttribute((const void*)kSolveVariants[v].fused_fn,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncAttributes fa{};
+            g_fused_width_ok[v] = hipFuncGetAttributes(&fa, (const void*)kSolveVariants[v].fused_fn) == hipSuccess &&
+                                  fa.maxThreadsPerBlock == (int)fused_threads(v);
+        }
     }
     (void)hipGetLastError();   // a refused attribute is not an error of a later call
 }

@@ -123,9 +123,9 @@ def test_batched_equals_single(gpu_ctx, oracle):
 
 
 def _variant_cases():
-    from resolve2d_amd.autotune import FUSED_VARIANTS
+    from resolve2d_amd.autotune import FUSED_VARIANTS, SOLVE_VARIANTS
     cases = []
-    for v in range(13):
+    for v in range(SOLVE_VARIANTS):
         for mode in ("fused", "phases"):
             marks = ()
             if mode == "fused" and v not in FUSED_VARIANTS:
@@ -157,12 +157,18 @@ def test_solve_variants_bit_identical(gpu_ctx, oracle, variant, mode):
     gpu_ctx.set_solve_variant(variant)
     gpu_ctx.set_fused(mode == "fused")
     gpu_ctx.set_wide_record(mode == "phases-wide")
+    gpu_ctx.set_profiling(True)
+    gpu_ctx.reset_phase_times()
     try:
         gpu_ctx.process_batch_handles([g.handle for g in gw], scenes.DT, 4, 4, 60)
+        launches = gpu_ctx.fused_times()[1]
     finally:
+        gpu_ctx.set_profiling(False)
         gpu_ctx.set_solve_variant(-1)
         gpu_ctx.set_fused(True)
         gpu_ctx.set_wide_record(True)
+    # the fused cases ran the variant's fused kernel, not the phase kernels
+    assert (launches > 0) == (mode == "fused"), (variant, mode, launches)
     for o in ow:
         for _ in range(60):
             assert o.process(scenes.DT, 4, 4) == 0
