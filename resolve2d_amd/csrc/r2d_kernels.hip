@@ -878,28 +878,33 @@ constexpr uint32_t COLOR_SPIN_LIMIT = 1u << 22;
 // its lane's current item, and its predecessors have run), and the counters
 // still give each body its items in the reference order, so the results are
 // the same bits; only which lane runs what, and when, changes.  Work arrays
-// in dynamic LDS (lds_words): levels, packed predecessors, level counts,
-// 3 nit + 2 words; a world with more items keeps the item order.  Levels
-// are found by sweeps in which each thread walks a contiguous chunk of items
-// in item order (a predecessor in the same chunk is seen in the same sweep),
-// until a sweep changes nothing.
-__device__ void level_order(StepDev& S, uint32_t w, uint32_t i0, uint32_t nit, uint32_t* lds, uint32_t lds_words,
-                            uint32_t* flags)
+// in LDS (lds_words): levels, packed predecessors, level counts, 3 nit + 2
+// words; a world with more items keeps the item order.  Levels are found by
+// sweeps in which each thread walks a contiguous chunk of items in item
+// order (a predecessor in the same chunk is seen in the same sweep), until a
+// sweep changes nothing.  The world's item arrays start at its first item
+// (r0s .. perm); a body r's item list is lists[off[r - base] ..], whose
+// entries are (item + sub) * 2 + side.  Not inlined: it runs once per step,
+// and kept out of the caller it leaves the caller's register allocation as
+// it is (the fused step kernel is at its register budget).
+__device__ __attribute__((noinline)) void level_order(const int32_t* r0s, const int32_t* r1s, const uint32_t* k0s,
+                                                      const uint32_t* k1s, uint32_t* perm, const uint32_t* lists,
+                                                      const uint32_t* off, uint32_t base, uint32_t sub, uint32_t nit,
+                                                      uint32_t* lds, uint32_t lds_words, uint32_t* flags)
 {
     const uint32_t T = blockDim.x, t = threadIdx.x;
     if (nit < 2u || nit >= 0xffffu || 3u * nit + 2u > lds_words) {
-        for (uint32_t p = t; p < nit; p += T) S.it_perm[i0 + p] = p;
+        for (uint32_t p = t; p < nit; p += T) perm[p] = p;
         return;
     }
     uint32_t* lev = lds;
     uint32_t* pred = lds + nit;
     uint32_t* hist = lds + 2u * nit;
     for (uint32_t p = t; p < nit; p += T) {
-        const uint32_t it = i0 + p;
-        const int32_t r0 = S.it_r0[it], r1 = S.it_r1[it];
+        const int32_t r0 = r0s[p], r1 = r1s[p];
         uint32_t q0 = 0xffffu, q1 = 0xffffu;
-        if (r0 >= 0 && S.it_k0[it] > 0u) q0 = (S.lists[S.deg_off[r0] + S.it_k0[it] - 1u] >> 1) - i0;
-        if (r1 >= 0 && S.it_k1[it] > 0u) q1 = (S.lists[S.deg_off[r1] + S.it_k1[it] - 1u] >> 1) - i0;
+        if (r0 >= 0 && k0s[p] > 0u) q0 = (lists[off[(uint32_t)r0 - base] + k0s[p] - 1u] >> 1) - sub;
+        if (r1 >= 0 && k1s[p] > 0u) q1 = (lists[off[(uint32_t)r1 - base] + k1s[p] - 1u] >> 1) - sub;
         pred[p] = q0 | (q1 << 16);
         lev[p] = 1u;
     }
@@ -953,9 +958,8 @@ __device__ void level_order(StepDev& S, uint32_t w, uint32_t i0, uint32_t nit, u
         }
     }
     __syncthreads();
-    for (uint32_t p = t; p < nit; p += T) S.it_perm[i0 + atomicAdd(&hist[lev[p]], 1u)] = p;
+    for (uint32_t p = t; p < nit; p += T) perm[atomicAdd(&hist[lev[p]], 1u)] = p;
     __syncthreads();
-    (void)w;
 }
 
 __global__ __launch_bounds__(256) void k_color(StepDev S, uint32_t lds_words)
@@ -971,7 +975,8 @@ __global__ __launch_bounds__(256) void k_color(StepDev S, uint32_t lds_words)
     const uint32_t T = blockDim.x, t = threadIdx.x;
     if (D.sched == 0) {
         for (uint32_t p = t; p < nit; p += T) S.it_color[i0 + p] = 0u;
-        level_order(S, w, i0, nit, dyn, lds_words, s_flags);
+        level_order(S.it_r0 + i0, S.it_r1 + i0, S.it_k0 + i0, S.it_k1 + i0, S.it_perm + i0, S.lists, S.deg_off, 0u,
+                    i0, nit, dyn, lds_words, s_flags);
         return;
     }
     const uint32_t base = D.body_base;
@@ -3093,6 +3098,7 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
     extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_sm[16];
     __shared__ uint32_t s_err, s_abort, s_slot;
+    __shared__ uint32_t s_lvl[3];
     const uint32_t w = S.w_order[blockIdx.x];
     const WorldDev& D = S.worlds[w];
     WorldScratch X = S.wscr[w];
@@ -3326,6 +3332,12 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
             }
         }
         __syncthreads();
+        // execution positions by dependency level when a lane holds more than
+        // one item (with a lane per item the order of positions is moot); its
+        // work arrays in the solve's part of LDS, free until the solve
+        if (nit > T)
+            level_order(S.it_r0 + X.i0, S.it_r1 + X.i0, S.it_k0 + X.i0, S.it_k1 + X.i0, S.it_perm + X.i0, lists, L.doff,
+                        base, 0u, nit, L.solve, fused_solve_words(NSLOT > 0) * n, s_lvl);
         if (WARM && D.warm > 0.0f) {
             // warm starting: match last step's manifolds, then each body's carried impulse
             for (uint32_t q = t; q < M; q += T) warm_match(S, D, X.m0 + q, base, sub_steps);
