@@ -403,6 +403,11 @@ int r2d_serve_tick(r2d_server* sv, r2d_serve_tick_info* info);
 /* Ticks until every request has finished. */
 int r2d_serve_drain(r2d_server* sv);
 int r2d_serve_poll(r2d_server* sv, uint64_t ticket, r2d_request_status* out);
+/* Tickets of the requests that finished (done or failed) since the last call,
+ * oldest first: up to cap of them into tickets, their number in *n; the rest
+ * wait for the next call.  Lets a serving loop visit only finished requests
+ * instead of polling every one it holds. */
+int r2d_serve_completed(r2d_server* sv, uint64_t* tickets, uint32_t cap, uint32_t* n);
 int r2d_serve_stats(r2d_server* sv, r2d_serve_summary* out);
 
 #ifdef __cplusplus

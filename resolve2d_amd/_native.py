@@ -158,6 +158,7 @@ SIGNATURES = {
     "r2d_serve_tick": [P, ctypes.POINTER(ServeTickInfo)],
     "r2d_serve_drain": [P],
     "r2d_serve_poll": [P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)],
+    "r2d_serve_completed": [P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
     "r2d_serve_stats": [P, ctypes.POINTER(ServeSummary)],
 }
 

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <unordered_map>
 #include <new>
 #include <thread>
 #include <type_traits>
@@ -308,6 +309,14 @@ struct r2d_ctx {
     uint64_t fused_sums[5] = {};                  // their bodies, entries, raw candidates, manifolds, items
     void* ingest = nullptr;                       // r2d_make_bodies' inputs on the device
     size_t ingest_cap = 0;                        // bodies they hold
+    // device blocks of this context's worlds, reused across worlds: hipFree
+    // waits for the whole device, so a world destroyed while other work runs
+    // (a served scene retired beside running lanes) would stall the host on
+    // every destroy.  Blocks come in power-of-two size classes; free ones wait
+    // in per-class lists until a world needs one or the context goes.
+    std::unordered_map<size_t, std::vector<void*>> pool_free;
+    std::unordered_map<void*, size_t> pool_class;  // every block handed out -> its class
+    size_t pool_cached = 0;                        // bytes in the free lists
 
     Streams streams() const { return Streams{stream, {aux[0], aux[1]}, fork, {join[0], join[1]}, trace}; }
 };
@@ -316,16 +325,86 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// the free lists hold at most this many bytes; a block released beyond it is
+// freed (with hipFree's wait)
+constexpr size_t kPoolCacheMax = (size_t)8 << 30;
+
+size_t pool_class_of(size_t bytes)
+{
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+// a device block of at least `bytes` for a world of context c
+int pool_alloc(r2d_ctx* c, void** out, size_t bytes)
+{
+    const size_t cls = pool_class_of(bytes);
+    std::vector<void*>& fl = c->pool_free[cls];
+    void* p = nullptr;
+    if (!fl.empty()) {
+        p = fl.back();
+        fl.pop_back();
+        c->pool_cached -= cls;
+    } else if (hipMalloc(&p, cls) != hipSuccess) {
+        // the cached blocks of other classes back to the device, then once more
+        (void)hipGetLastError();
+        for (auto& kv : c->pool_free)
+            for (void* q : kv.second) (void)hipFree(q);
+        c->pool_free.clear();
+        c->pool_cached = 0;
+        p = nullptr;
+        if (hipMalloc(&p, cls) != hipSuccess) {
+            (void)hipGetLastError();
+            return R2D_E_OOM;
+        }
+    }
+    c->pool_class[p] = cls;
+    *out = p;
+    return R2D_OK;
+}
+
+// a block from pool_alloc back to its free list; no device work may use it
+// any more (the caller has waited for the streams that did)
+void pool_release(r2d_ctx* c, void* p)
+{
+    if (!p) return;
+    auto it = c->pool_class.find(p);
+    if (it == c->pool_class.end()) {   // not from the pool
+        (void)hipFree(p);
+        return;
+    }
+    const size_t cls = it->second;
+    c->pool_class.erase(it);
+    if (c->pool_cached + cls > kPoolCacheMax) {
+        (void)hipFree(p);
+        return;
+    }
+    c->pool_free[cls].push_back(p);
+    c->pool_cached += cls;
+}
+
+void pool_destroy(r2d_ctx* c)
+{
+    for (auto& kv : c->pool_free)
+        for (void* q : kv.second) (void)hipFree(q);
+    c->pool_free.clear();
+    c->pool_cached = 0;
+}
+
 void destroy_graphs(r2d_ctx* c);
 int server_collect(r2d_server* sv);
 
 int ensure_world_device(r2d_world* w)
 {
     if (w->dcap < w->cap || !w->dblock) {
-        if (w->dblock) HIPCHK(hipFree(w->dblock));
+        if (w->dblock) {
+            HIPCHK(hipStreamSynchronize(w->ctx->stream));
+            pool_release(w->owner, w->dblock);
+        }
         w->dblock = nullptr;
         size_t c = std::max<size_t>(w->cap, 16);
-        if (hipMalloc(&w->dblock, (size_t)F_NFIELDS * c * 4) != hipSuccess) return R2D_E_OOM;
+        if (pool_alloc(w->owner, (void**)&w->dblock, (size_t)F_NFIELDS * c * 4)) return R2D_E_OOM;
         w->dcap = c;
         w->dev_valid = false;
         w->version += 1;
@@ -344,18 +423,19 @@ int ensure_warm(r2d_world* w)
         uint4* tab = nullptr;
         uint32_t* cnt = nullptr;
         float4* sum = nullptr;
-        if (hipMalloc(&tab, c * WARM_K * 3 * sizeof(uint4)) != hipSuccess) return R2D_E_OOM;
-        if (hipMalloc(&cnt, c * 4) != hipSuccess) { (void)hipFree(tab); return R2D_E_OOM; }
-        if (hipMalloc(&sum, c * sizeof(float4)) != hipSuccess) { (void)hipFree(tab); (void)hipFree(cnt); return R2D_E_OOM; }
+        r2d_ctx* o = w->owner;
+        if (pool_alloc(o, (void**)&tab, c * WARM_K * 3 * sizeof(uint4))) return R2D_E_OOM;
+        if (pool_alloc(o, (void**)&cnt, c * 4)) { pool_release(o, tab); return R2D_E_OOM; }
+        if (pool_alloc(o, (void**)&sum, c * sizeof(float4))) { pool_release(o, tab); pool_release(o, cnt); return R2D_E_OOM; }
         HIPCHK(hipMemsetAsync(cnt, 0, c * 4, st));
         if (w->dwcap && !w->warm_clear) {
             HIPCHK(hipMemcpyAsync(tab, w->dwtab, w->dwcap * WARM_K * 3 * sizeof(uint4), hipMemcpyDeviceToDevice, st));
             HIPCHK(hipMemcpyAsync(cnt, w->dwcnt, w->dwcap * 4, hipMemcpyDeviceToDevice, st));
         }
         HIPCHK(hipStreamSynchronize(st));
-        if (w->dwtab) (void)hipFree(w->dwtab);
-        if (w->dwcnt) (void)hipFree(w->dwcnt);
-        if (w->dwsum) (void)hipFree(w->dwsum);
+        pool_release(o, w->dwtab);
+        pool_release(o, w->dwcnt);
+        pool_release(o, w->dwsum);
         w->dwtab = tab; w->dwcnt = cnt; w->dwsum = sum;
         w->dwcap = c;
         w->version += 1;
@@ -454,10 +534,13 @@ int upload_topology(r2d_world* w)
         d.r1x = h.r1x; d.r1y = h.r1y; d.r2x = h.r2x; d.r2y = h.r2y;
         d.target = h.target; d.tx = h.tx; d.ty = h.ty;
     }
+    const bool grow = js.size() > w->djcap || w->gens.size() > w->dgcap || w->excl.size() > w->dxcap;
+    if (grow) HIPCHK(hipStreamSynchronize(st));   // the blocks replaced below are idle
     if (js.size() > w->djcap) {
-        if (w->djoints) HIPCHK(hipFree(w->djoints));
+        pool_release(w->owner, w->djoints);
+        w->djoints = nullptr;
         w->djcap = std::max<size_t>(16, js.size() * 2);
-        if (hipMalloc(&w->djoints, w->djcap * sizeof(Joint)) != hipSuccess) return R2D_E_OOM;
+        if (pool_alloc(w->owner, (void**)&w->djoints, w->djcap * sizeof(Joint))) { w->djcap = 0; return R2D_E_OOM; }
     }
     if (!js.empty()) HIPCHK(hipMemcpyAsync(w->djoints, js.data(), js.size() * sizeof(Joint), hipMemcpyHostToDevice, st));
     std::vector<Gen> gs(w->gens.size());
@@ -469,15 +552,17 @@ int upload_topology(r2d_world* w)
         d.k = h.k; d.c = h.c; d.ax = h.ax; d.ay = h.ay;
     }
     if (gs.size() > w->dgcap) {
-        if (w->dgens) HIPCHK(hipFree(w->dgens));
+        pool_release(w->owner, w->dgens);
+        w->dgens = nullptr;
         w->dgcap = std::max<size_t>(16, gs.size() * 2);
-        if (hipMalloc(&w->dgens, w->dgcap * sizeof(Gen)) != hipSuccess) return R2D_E_OOM;
+        if (pool_alloc(w->owner, (void**)&w->dgens, w->dgcap * sizeof(Gen))) { w->dgcap = 0; return R2D_E_OOM; }
     }
     if (!gs.empty()) HIPCHK(hipMemcpyAsync(w->dgens, gs.data(), gs.size() * sizeof(Gen), hipMemcpyHostToDevice, st));
     if (w->excl.size() > w->dxcap) {
-        if (w->dexcl) HIPCHK(hipFree(w->dexcl));
+        pool_release(w->owner, w->dexcl);
+        w->dexcl = nullptr;
         w->dxcap = std::max<size_t>(16, w->excl.size() * 2);
-        if (hipMalloc(&w->dexcl, w->dxcap * sizeof(uint64_t)) != hipSuccess) return R2D_E_OOM;
+        if (pool_alloc(w->owner, (void**)&w->dexcl, w->dxcap * sizeof(uint64_t))) { w->dxcap = 0; return R2D_E_OOM; }
     }
     if (!w->excl.empty())
         HIPCHK(hipMemcpyAsync(w->dexcl, w->excl.data(), w->excl.size() * 8, hipMemcpyHostToDevice, st));
@@ -1394,6 +1479,7 @@ struct r2d_server {
     std::vector<float> step_ms;               // device time of each batch step
     std::vector<uint32_t> step_worlds;        // scenes in it
     uint32_t last_completed = 0;
+    std::deque<uint64_t> done_tickets;        // finished since the last r2d_serve_completed
     double last_gpu_ms = 0, last_wall_ms = 0;
 };
 
@@ -1420,6 +1506,7 @@ void retire(r2d_server* sv, uint64_t t, int32_t state, int32_t status)
     sv->resident -= q.bytes;
     sv->completed += 1;
     sv->last_completed += 1;
+    sv->done_tickets.push_back(t);
 }
 
 // wait for a lane's launched batch step and account for it
@@ -1609,6 +1696,7 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     if (c->hscr) (void)hipHostFree(c->hscr);
     if (c->hpool) (void)hipHostFree(c->hpool);
     if (c->ingest) (void)hipFree(c->ingest);
+    pool_destroy(c);
     for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(c->aux[i]);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1640,15 +1728,18 @@ R2D_EXPORT int r2d_world_destroy(r2d_world* w)
 {
     if (!w) return R2D_E_INVALID_ARG;
     if (w->in_service) return R2D_E_BUSY;
+    // the world's last launches ran on its context's stream (a served world
+    // is out of service only once its lane's launches are collected)
     (void)hipStreamSynchronize(w->ctx->stream);
-    if (w->dblock) (void)hipFree(w->dblock);
-    if (w->djoints) (void)hipFree(w->djoints);
-    if (w->dgens) (void)hipFree(w->dgens);
-    if (w->dexcl) (void)hipFree(w->dexcl);
-    if (w->dwtab) (void)hipFree(w->dwtab);
-    if (w->dwcnt) (void)hipFree(w->dwcnt);
-    if (w->dwsum) (void)hipFree(w->dwsum);
-    w->owner->live_worlds -= 1;
+    r2d_ctx* o = w->owner;
+    pool_release(o, w->dblock);
+    pool_release(o, w->djoints);
+    pool_release(o, w->dgens);
+    pool_release(o, w->dexcl);
+    pool_release(o, w->dwtab);
+    pool_release(o, w->dwcnt);
+    pool_release(o, w->dwsum);
+    o->live_worlds -= 1;
     delete w;
     return R2D_OK;
 }
@@ -2579,6 +2670,18 @@ R2D_EXPORT int r2d_serve_drain(r2d_server* sv)
         if (rc && first == R2D_OK) first = rc;
     }
     return first;
+}
+
+R2D_EXPORT int r2d_serve_completed(r2d_server* sv, uint64_t* tickets, uint32_t cap, uint32_t* n)
+{
+    if (!sv || !n || (cap && !tickets)) return R2D_E_INVALID_ARG;
+    uint32_t k = 0;
+    while (k < cap && !sv->done_tickets.empty()) {
+        tickets[k++] = sv->done_tickets.front();
+        sv->done_tickets.pop_front();
+    }
+    *n = k;
+    return R2D_OK;
 }
 
 R2D_EXPORT int r2d_serve_poll(r2d_server* sv, uint64_t ticket, r2d_request_status* out)

@@ -58,6 +58,17 @@ class Server:
         N.check(self.lib.r2d_serve_poll(self.handle, ticket, ctypes.byref(st)), "r2d_serve_poll")
         return {f: getattr(st, f) for f, _ in N.RequestStatus._fields_}
 
+    def completed(self) -> list:
+        """Tickets of the requests finished since the last call, oldest first."""
+        out = []
+        buf = (ctypes.c_uint64 * 256)()
+        n = ctypes.c_uint32()
+        while True:
+            N.check(self.lib.r2d_serve_completed(self.handle, buf, 256, ctypes.byref(n)), "r2d_serve_completed")
+            out.extend(buf[:n.value])
+            if n.value < 256:
+                return out
+
     def stats(self) -> dict:
         s = N.ServeSummary()
         N.check(self.lib.r2d_serve_stats(self.handle, ctypes.byref(s)), "r2d_serve_stats")
@@ -89,13 +100,14 @@ def run_stream(server: Server, arrivals, build, steps: int, time_limit: float = 
             submitted += 1
         info = server.tick()
         if info["completed"]:
-            for t in [t for t in live]:
+            for t in server.completed():
+                if t not in live:
+                    continue
                 st = server.poll(t)
-                if st["state"] >= N.REQ_DONE:
-                    payload, w = live.pop(t)
-                    lat.append(st["total_ms"])
-                    if on_done:
-                        on_done(payload, w, st)
+                payload, w = live.pop(t)
+                lat.append(st["total_ms"])
+                if on_done:
+                    on_done(payload, w, st)
         if not live and pending:
             # idle until the next arrival
             time.sleep(max(0.0, min(pending[-1][0] - (time.perf_counter() - t0), 0.05)))

@@ -66,9 +66,13 @@ def test_served_results_equal_alone(gpu_ctx, per_tick):
             tickets.append(sv.submit(mix[i][1], STEPS))
         nxt = 3
         ticks = 0
+        finished = []
         while True:
             info = sv.tick()
             ticks += 1
+            done_now = sv.completed()
+            assert len(done_now) == info["completed"]
+            finished += done_now
             assert info["resident_bytes"] <= budget
             assert info["running"] <= 4
             if nxt < len(mix) and ticks % 3 == 0:
@@ -90,6 +94,8 @@ def test_served_results_equal_alone(gpu_ctx, per_tick):
             assert p["state"] == 2 and p["status"] == 0 and p["steps_done"] == STEPS, p
             lanes.add(p["lane"])
         assert lanes == {0, 1, 2}
+        # every request reported finished exactly once
+        assert sorted(finished) == sorted(tickets) and sv.completed() == []
     finally:
         sv.close()
     for i, (sp, w) in enumerate(mix):
