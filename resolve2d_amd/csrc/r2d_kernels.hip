@@ -1496,6 +1496,10 @@ __device__ __forceinline__ void contact_impulses(Item& z, uint32_t l1, uint32_t 
     // carried impulse that is too large can be taken back.
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
+        // the second point only when a lane running this link has one (a
+        // wave whose links are all one-point manifolds skips its half of the
+        // instructions; for an absent point the selects below change nothing)
+        if (p == 1 && __ballot((z.flags & RF_P1) != 0u) == 0ull) break;
         const bool on = (z.flags & (RF_P0 << p)) != 0u;
         const bool sep = z.bias[p] < 0.f;
         v2 r1 = mk(z.p[p][0], z.p[p][1]), r2 = mk(z.p[p][2], z.p[p][3]);
