@@ -295,7 +295,8 @@ struct r2d_ctx {
     uint32_t* hdone = nullptr;                    // pinned: steps done per world, last-step / needed counts
     uint32_t* hcnt = nullptr;
     WorldScratch* hscr = nullptr;                 // pinned: the slice table as the last launch left it
-    uint32_t* hpool = nullptr;                    // pinned: the arrays' free ends and capacities (S.fpool)
+    uint32_t* dfsum = nullptr;                    // a fused launch's summary gathered on the device (fsum_words)
+    uint32_t* hfsum = nullptr;                    // pinned: the same, copied back in one piece
     size_t hfused_cap = 0;
     uint64_t* tstamp = nullptr;                   // per-world step times of a profiled fused call
     size_t tstamp_words = 0;
@@ -324,6 +325,12 @@ struct r2d_ctx {
 namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// words of a fused launch's summary for nw worlds (k_fused_gather): the 64
+// counters, then per world status, steps done, 8 counts, 8-word slice entry
+constexpr size_t FSUM_WORLD_WORDS = 2 + 8 + sizeof(WorldScratch) / 4;
+static_assert(sizeof(WorldScratch) == 32, "the summary packs 8-word slice table entries");
+size_t fsum_words(size_t nw) { return 64 + FSUM_WORLD_WORDS * nw; }
 
 // the free lists hold at most this many bytes; a block released beyond it is
 // freed (with hipFree's wait)
@@ -844,12 +851,17 @@ int ensure_fused_world(r2d_ctx* c, size_t W)
         c->hcnt = nullptr;
         c->hscr = nullptr;
         c->hfused_cap = 0;
+        if (c->dfsum) HIPCHK(hipFree(c->dfsum));
+        if (c->hfsum) HIPCHK(hipHostFree(c->hfsum));
+        c->dfsum = nullptr;
+        c->hfsum = nullptr;
         if (hipHostMalloc((void**)&c->hdone, cap * 4) != hipSuccess) return R2D_E_OOM;
         if (hipHostMalloc((void**)&c->hcnt, cap * 32) != hipSuccess) return R2D_E_OOM;
         if (hipHostMalloc((void**)&c->hscr, cap * sizeof(WorldScratch)) != hipSuccess) return R2D_E_OOM;
+        if (hipMalloc((void**)&c->dfsum, fsum_words(cap) * 4) != hipSuccess) return R2D_E_OOM;
+        if (hipHostMalloc((void**)&c->hfsum, fsum_words(cap) * 4) != hipSuccess) return R2D_E_OOM;
         c->hfused_cap = cap;
     }
-    if (!c->hpool && hipHostMalloc((void**)&c->hpool, 8 * 4) != hipSuccess) return R2D_E_OOM;
     return R2D_OK;
 }
 
@@ -962,6 +974,57 @@ int fused_regrow(r2d_ctx* c, Batch& b)
 
 // the fused kernel's view: the step scratch's per-body / per-world arrays
 // and the fused arena's slices
+// A fused launch's bookkeeping in two small kernels instead of four fills, a
+// copy in and five copies out (each an enqueue of its own, ~10 us of a
+// one-world call): k_fused_prep clears the counters (and, on a call's first
+// launch, the worlds' steps done, status and counts) and writes the arrays'
+// free ends; k_fused_gather packs what the host reads back into one block,
+// copied in one piece: the 64 counters, then per world status, steps done,
+// the 8 counts and the 8-word slice table entry.
+
+struct PoolArgs {
+    uint32_t v[8];
+};
+
+__global__ void k_fused_prep(uint32_t* counters, int32_t* w_status, uint32_t* w_done, uint32_t* w_cnt, uint32_t* fpool,
+                             PoolArgs pool, uint32_t nw, uint32_t first)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 64u) counters[i] = 0u;
+    if (i < 8u) fpool[i] = pool.v[i];
+    if (first) {
+        if (i < nw) { w_status[i] = 0; w_done[i] = 0u; }
+        if (i < 8u * nw) w_cnt[i] = 0u;
+    }
+}
+
+__global__ void k_fused_gather(const uint32_t* counters, const int32_t* w_status, const uint32_t* w_done,
+                               const uint32_t* w_cnt, const WorldScratch* wscr, uint32_t* out, uint32_t nw)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 64u) out[i] = counters[i];
+    uint32_t* o = out + 64;
+    if (i < nw) { o[i] = (uint32_t)w_status[i]; o[nw + i] = w_done[i]; }
+    if (i < 8u * nw) {
+        o[2u * nw + i] = w_cnt[i];
+        o[10u * nw + i] = reinterpret_cast<const uint32_t*>(wscr)[i];
+    }
+}
+
+unsigned fsum_blocks(size_t nw) { return (unsigned)((std::max<size_t>(64, 8 * nw) + 255) / 256); }
+
+// the gathered summary into the host arrays the rest of the code reads
+void fsum_unpack(r2d_ctx* c, size_t nw)
+{
+    const uint32_t* h = c->hfsum;
+    std::memcpy(c->hcounters, h, 64 * 4);
+    h += 64;
+    std::memcpy(c->hstatus, h, nw * 4);
+    std::memcpy(c->hdone, h + nw, nw * 4);
+    std::memcpy(c->hcnt, h + 2 * nw, nw * 32);
+    std::memcpy(c->hscr, h + 10 * nw, nw * sizeof(WorldScratch));
+}
+
 StepDev fused_view(const r2d_ctx* c)
 {
     StepDev F = c->S;
@@ -1196,21 +1259,17 @@ int fused_launch(r2d_ctx* c, Batch& b)
         if (b.attempts == 0) HIPCHK(hipMemsetAsync(c->wprof, 0, nw * FP_N * 8, c->stream));
         F.w_prof = c->wprof;
     }
-    HIPCHK(hipMemsetAsync(F.counters, 0, 64 * 4, c->stream));
     // free ends of the arrays: where outgrown worlds take new slices
     uint64_t end[4];
     fused_used(c, end);
     const uint32_t caps[4] = {c->fcap_e, c->fcap_r, c->fcap_m, c->fcap_i};
+    PoolArgs pool;
     for (int k = 0; k < 4; ++k) {
-        c->hpool[k] = (uint32_t)std::min<uint64_t>(end[k], caps[k]);
-        c->hpool[4 + k] = caps[k];
+        pool.v[k] = (uint32_t)std::min<uint64_t>(end[k], caps[k]);
+        pool.v[4 + k] = caps[k];
     }
-    HIPCHK(hipMemcpyAsync(F.fpool, c->hpool, 8 * 4, hipMemcpyHostToDevice, c->stream));
-    if (b.attempts == 0) {
-        HIPCHK(hipMemsetAsync(F.w_done, 0, nw * 4, c->stream));
-        HIPCHK(hipMemsetAsync(F.w_status, 0, nw * 4, c->stream));
-        HIPCHK(hipMemsetAsync(F.w_cnt, 0, nw * 32, c->stream));
-    }
+    hipLaunchKernelGGL(k_fused_prep, dim3(fsum_blocks(nw)), dim3(256), 0, c->stream, F.counters, F.w_status,
+                       F.w_done, F.w_cnt, F.fpool, pool, (uint32_t)nw, b.attempts == 0 ? 1u : 0u);
     HIPCHK(hipEventRecord(c->ev_begin, c->stream));
     (void)hipGetLastError();   // the launch check below sees this launch only
     launch_fused(F, b.variant, (uint32_t)nw, b.dt, (uint32_t)b.sub, (uint32_t)b.iters, (uint32_t)b.steps,
@@ -1227,11 +1286,9 @@ int fused_launch(r2d_ctx* c, Batch& b)
         return R2D_E_DEVICE;
     }
     HIPCHK(hipEventRecord(c->ev_kend, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hcounters, F.counters, 64 * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hstatus, F.w_status, nw * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hdone, F.w_done, nw * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hcnt, F.w_cnt, nw * 32, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hscr, F.wscr, nw * sizeof(WorldScratch), hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_fused_gather, dim3(fsum_blocks(nw)), dim3(256), 0, c->stream, F.counters, F.w_status,
+                       F.w_done, F.w_cnt, F.wscr, c->dfsum, (uint32_t)nw);
+    HIPCHK(hipMemcpyAsync(c->hfsum, c->dfsum, fsum_words(nw) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->ev_end, c->stream));
     b.attempts += 1;
     return R2D_OK;
@@ -1263,6 +1320,7 @@ int fused_wait(r2d_ctx* c, Batch& b)
     const size_t nw = b.worlds.size();
     for (;;) {
         HIPCHK(hipEventSynchronize(c->ev_end));
+        fsum_unpack(c, nw);
         if (c->profiling) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev_begin, c->ev_kend));
@@ -1694,7 +1752,8 @@ R2D_EXPORT int r2d_ctx_destroy(r2d_ctx* c)
     if (c->hdone) (void)hipHostFree(c->hdone);
     if (c->hcnt) (void)hipHostFree(c->hcnt);
     if (c->hscr) (void)hipHostFree(c->hscr);
-    if (c->hpool) (void)hipHostFree(c->hpool);
+    if (c->dfsum) (void)hipFree(c->dfsum);
+    if (c->hfsum) (void)hipHostFree(c->hfsum);
     if (c->ingest) (void)hipFree(c->ingest);
     pool_destroy(c);
     for (int i = 0; i < 2; ++i) (void)hipStreamDestroy(c->aux[i]);
