@@ -1990,9 +1990,12 @@ __device__ __forceinline__ void rec_put_acc(uint4* recs, uint32_t pos, const flo
                                                  __float_as_uint(acc[2]), __float_as_uint(acc[3]));
 }
 
-__device__ __forceinline__ void lane_load(Rec& r, const uint4* q)
+// a lane's slot rows are blockDim.x apart (row r of lane t at lanes[r * T + t]):
+// a wave's 16-byte row reads and writes then cover consecutive LDS words,
+// free of bank conflicts (rows contiguous per lane, 80 bytes apart, were not)
+__device__ __forceinline__ void lane_load(Rec& r, const uint4* q, uint32_t stride)
 {
-    const uint4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    const uint4 a = q[0], b = q[stride], c = q[2 * stride], d = q[3 * stride], e = q[4 * stride];
     r.nx = __uint_as_float(a.x); r.ny = __uint_as_float(a.y); r.fric = __uint_as_float(a.z);
     r.p[0][0] = __uint_as_float(a.w);
     r.p[0][1] = __uint_as_float(b.x); r.p[0][2] = __uint_as_float(b.y); r.p[0][3] = __uint_as_float(b.z);
@@ -2030,7 +2033,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
 {
     static_assert(!HYBRID || (GLOBAL && !COMPACT), "the LDS counters go with global-memory state");
     const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
-    uint4* slot = lanes + (size_t)threadIdx.x * LANE_ROWS;
+    uint4* slot = lanes + threadIdx.x;          // row r at slot[r * p_step]
     uint32_t pos = p_begin, k = (p_begin < nitems) ? 0u : iters;
     uint32_t held = NONE;                    // position whose record the slot holds
     uint32_t flags = 0, a = 0, b = 0, rk0 = 0, rk1 = 0, degs = 0;
@@ -2050,7 +2053,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 const uint4* q = recs + (size_t)pos * 8;
                 const uint4 h0 = q[0], h1 = q[1], ac = q[REC_ACC];
 #pragma unroll
-                for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row] = q[REC_LINK + row];
+                for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row * p_step] = q[REC_LINK + row];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
                 if (HYBRID) cpair = h1.w;
                 if (COMPACT) {
@@ -2111,7 +2114,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             lds_order();
             Rec cur;
             cur.flags = flags; cur.a = a; cur.b = b;
-            lane_load(cur, slot);
+            lane_load(cur, slot, p_step);
 #pragma unroll
             for (int f = 0; f < 4; ++f) cur.acc[f] = acc[f];
             const BodyState b1 = GLOBAL ? body_state_ordered(st, contact ? a : 0u) : body_state(st, contact ? a : 0u);
