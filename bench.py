@@ -710,6 +710,9 @@ def main():
     ap.add_argument("--no-critical", action="store_true", help="skip the one-world critical-path measurements")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-starting runs")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--tune-fused-only", action="store_true",
+                    help="the autotuner tries the fused shapes only (for counter runs of a profiler: the "
+                         "phase-kernel candidates add thousands of dispatches)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline window and the horizon call (for profiling runs)")
     ap.add_argument("--only-world", choices=["config1", "config2", "config4"], default=None,
@@ -815,9 +818,10 @@ def main():
             ctx.process_batch_handles([w.handle for w in copy], dt, S, K_IT, window[0] - 1)
             return copy
 
-        cands = [c for c in autotune.candidates(lds_limits=(0,)) if c[2] or c[0] in (1, 11)]
+        cands = [c for c in autotune.candidates(lds_limits=(0,))
+                 if c[2] or (c[0] in (1, 11) and not args.tune_fused_only)]
         tuned = autotune.tune(ctx, None, dt, S, K_IT, rounds=1, cands=cands, cache_path=None,
-                              steps_per_sample=K, make_batch=fresh_copy, refine=3, refine_steps=K)
+                              steps_per_sample=K, make_batch=fresh_copy, refine=3, refine_steps=K, log=log)
         log(f"autotune: variant {tuned['variant']} fused {tuned['fused']} ({tuned['step_ms']:.3f} ms per step)")
 
     # timed region: the window, one call of K steps after the untimed advance

@@ -118,7 +118,7 @@ def pick(times: Dict[tuple, list]) -> tuple:
 def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: int = 2,
          cands=None, cache_path: Optional[str] = DEFAULT_CACHE, use_cache: bool = True,
          steps_per_sample: int = 30, make_batch: Optional[Callable[[], list]] = None,
-         refine: int = 0, refine_steps: int = 120) -> dict:
+         refine: int = 0, refine_steps: int = 120, log: Optional[Callable[[str], None]] = None) -> dict:
     """Pick the launch shape for stepping `handles` together and apply it to
     the context.  Spends len(cands) * rounds * steps_per_sample real steps of
     the batch (untimed by the caller's measure; the results are the same as
@@ -130,7 +130,7 @@ def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: 
     (a short sample can rank shapes differently from a long run, whose later
     steps hold more contacts), and that time decides.  Returns {"variant",
     "lds_limit", "fused", "step_ms", "key", "cached", "all_ms"} (plus
-    "refined_ms" after a refinement)."""
+    "refined_ms" after a refinement).  log, if given, gets a line per sample."""
     probe = None
     if handles is None:
         probe = make_batch()
@@ -166,9 +166,15 @@ def tune(ctx, handles: Sequence, dt: float, sub_steps: int, iters: int, rounds: 
         for _ in range(rounds):
             for c in cands:
                 times[c].append(sample(c, steps_per_sample))
+                if log:
+                    log(f"autotune sample {c}: {times[c][-1]:.3f} ms per step")
         if refine > 1:
             ranked = sorted(times, key=lambda c: float(np.median(times[c])))[:refine]
-            refined = {c: [sample(c, refine_steps)] for c in ranked}
+            refined = {}
+            for c in ranked:
+                refined[c] = [sample(c, refine_steps)]
+                if log:
+                    log(f"autotune refine {c}: {refined[c][0]:.3f} ms per step")
     finally:
         ctx.set_profiling(was)
     best = pick(refined) if refined else pick(times)
