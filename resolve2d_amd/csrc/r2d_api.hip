@@ -1238,13 +1238,18 @@ int fused_launch(r2d_ctx* c, Batch& b)
     F.nb = (uint32_t)b.N; F.nworlds = (uint32_t)nw; F.nbuckets = (uint32_t)b.B; F.njoints = (uint32_t)b.J;
     F.worlds = c->dworlds;
     if (c->profiling) {
+        // The timing buffers grow rarely and by a lot: a launch that follows a
+        // hipMalloc closely can stall all of its waves for several ms at once
+        // (measured: one ~7 ms world-step in 889 of 1,024 worlds of the first
+        // profiled 20-step call, none once the buffers existed before it).
         const size_t words = nw * (b.steps + 1);
         if (c->tstamp_words < words) {
+            const size_t want = std::max<size_t>({words, 2 * c->tstamp_words, (size_t)1 << 18});
             if (c->tstamp) HIPCHK(hipFree(c->tstamp));
             c->tstamp = nullptr;
             c->tstamp_words = 0;
-            if (hipMalloc((void**)&c->tstamp, words * 8) != hipSuccess) return R2D_E_OOM;
-            c->tstamp_words = words;
+            if (hipMalloc((void**)&c->tstamp, want * 8) != hipSuccess) return R2D_E_OOM;
+            c->tstamp_words = want;
         }
         if (b.attempts == 0) HIPCHK(hipMemsetAsync(c->tstamp, 0, words * 8, c->stream));
         F.w_tstamp = c->tstamp;
@@ -1252,9 +1257,10 @@ int fused_launch(r2d_ctx* c, Batch& b)
         if (c->wprof_worlds < nw) {
             if (c->wprof) HIPCHK(hipFree(c->wprof));
             c->wprof = nullptr;
+            const size_t want = std::max<size_t>({nw, 2 * c->wprof_worlds, 4096});
             c->wprof_worlds = 0;
-            if (hipMalloc((void**)&c->wprof, nw * FP_N * 8) != hipSuccess) return R2D_E_OOM;
-            c->wprof_worlds = nw;
+            if (hipMalloc((void**)&c->wprof, want * FP_N * 8) != hipSuccess) return R2D_E_OOM;
+            c->wprof_worlds = want;
         }
         if (b.attempts == 0) HIPCHK(hipMemsetAsync(c->wprof, 0, nw * FP_N * 8, c->stream));
         F.w_prof = c->wprof;
