@@ -75,13 +75,15 @@ class Server:
         return {f: getattr(s, f) for f, _ in N.ServeSummary._fields_}
 
 
-def run_stream(server: Server, arrivals, build, steps: int, time_limit: float = None, on_done=None):
+def run_stream(server: Server, arrivals, build, steps: int, time_limit: float = None, on_done=None, trace=None):
     """Serve a timed stream.  `arrivals` is a list of (t_seconds, payload);
     `build(payload)` returns a new world for it (built on the host while the
     device steps the running batch); `on_done(payload, world, status)` is
     called as requests finish (the caller may read and free the world).
-    Arrivals after `time_limit` seconds are not submitted.  Returns a dict of
-    wall time, request latencies and the server's statistics."""
+    Arrivals after `time_limit` seconds are not submitted.  trace, if a list,
+    gets (seconds since the start, body-steps collected so far) after every
+    tick.  Returns a dict of wall time, request latencies and the server's
+    statistics."""
     t0 = time.perf_counter()
     pending = list(arrivals)
     pending.reverse()
@@ -99,6 +101,8 @@ def run_stream(server: Server, arrivals, build, steps: int, time_limit: float = 
             live[server.submit(w, steps)] = (payload, w)
             submitted += 1
         info = server.tick()
+        if trace is not None:
+            trace.append((time.perf_counter() - t0, server.stats()["body_steps"]))
         if info["completed"]:
             for t in server.completed():
                 if t not in live:
