@@ -661,6 +661,7 @@ int ensure_scratch(r2d_ctx* c, uint32_t n, uint32_t b, uint32_t wn, uint32_t j, 
     add((void**)&S.it_color, I * 4);
     add((void**)&S.it_perm, I * 4);
     add((void**)&S.recs, I * 128);
+    add((void**)&S.rmass, I * 16);
     add((void**)&S.cmask, 2 * N * 8);
     add((void**)&S.g_im, N * 4); add((void**)&S.g_ii, N * 4); add((void**)&S.g_cnt, N * 4);
     add((void**)&S.cmap, N * 4); add((void**)&S.cinv, N * 4); add((void**)&S.w_nc, (W + 1) * 4);

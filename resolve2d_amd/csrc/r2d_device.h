@@ -145,6 +145,8 @@ struct StepDev {
     uint32_t* it_perm;                     // [cap_i] world-local items in execution order
     uint64_t* cmask;                       // [2*N] colours used per body (coloured schedule)
     uint4* recs;                           // [8*cap_i] solver item records (128 B each)
+    uint4* rmass;                          // [cap_i] worlds beyond LDS: inverse mass and inertia of each item's
+                                           // two bodies (im a, ii a, im b, ii b) for the light compact chain
     // global-memory solver state (worlds too large for LDS)
     float *g_im, *g_ii;
     uint32_t* g_cnt;

@@ -1466,6 +1466,46 @@ __device__ __forceinline__ BodyState body_state(const ST& st, uint32_t i)
                      lds_relaxed(&st.ii[i])};
 }
 
+// The light compact chain's state (worlds beyond LDS whose touched bodies
+// exceed the full compact state but whose momentum and counters fit LDS):
+// momentum and counter of each touched body in LDS under its compact index;
+// a link's inverse masses and inertias come with its item (a sixth row of the
+// lane slot, from S.rmass), and the pair proxies answer st.im[i] / st.ii[i]
+// for the item's two bodies (the first body's index, else the second's); an
+// item count per body (only items whose counts need more than 16 bits read
+// it) is looked up in global memory.
+struct PairValue {
+    uint32_t a;
+    float va, vb;
+    __device__ __forceinline__ float operator[](uint32_t i) const { return i == a ? va : vb; }
+};
+struct DegLookup {
+    const uint32_t* deg;
+    const uint32_t* cinv;
+    uint32_t base;
+    __device__ __forceinline__ uint32_t operator[](uint32_t c) const { return deg[base + cinv[base + c]]; }
+};
+struct LiteState {
+    float *mx, *my, *L;
+    uint32_t* cnt;
+    PairValue im, ii;
+    DegLookup deg;
+    const uint4* rmass;      // the world's item masses, by execution position
+};
+template <class ST>
+struct IsLite {
+    static constexpr bool value = false;
+};
+template <>
+struct IsLite<LiteState> {
+    static constexpr bool value = true;
+};
+
+__device__ __forceinline__ BodyState body_state(const LiteState& st, uint32_t i)
+{
+    return BodyState{lds_relaxed(&st.mx[i]), lds_relaxed(&st.my[i]), lds_relaxed(&st.L[i]), st.im[i], st.ii[i]};
+}
+
 // the same with ordinary loads (after the item's counters were acquired)
 template <class ST>
 __device__ __forceinline__ BodyState body_state_ordered(const ST& st, uint32_t i)
@@ -2032,6 +2072,8 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                                                  uint32_t& s_abort, uint32_t* lcnt = nullptr)
 {
     static_assert(!HYBRID || (GLOBAL && !COMPACT), "the LDS counters go with global-memory state");
+    constexpr bool LITE = IsLite<ST>::value;
+    static_assert(!LITE || (COMPACT && !GLOBAL), "the light state is a compact LDS state");
     const uint32_t p_begin = threadIdx.x, p_step = blockDim.x;
     uint4* slot = lanes + threadIdx.x;          // row r at slot[r * p_step]
     uint32_t pos = p_begin, k = (p_begin < nitems) ? 0u : iters;
@@ -2054,6 +2096,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 const uint4 h0 = q[0], h1 = q[1], ac = q[REC_ACC];
 #pragma unroll
                 for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row * p_step] = q[REC_LINK + row];
+                if constexpr (LITE) slot[LANE_ROWS * p_step] = st.rmass[pos];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
                 if (HYBRID) cpair = h1.w;
                 if (COMPACT) {
@@ -2117,12 +2160,19 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
             lane_load(cur, slot, p_step);
 #pragma unroll
             for (int f = 0; f < 4; ++f) cur.acc[f] = acc[f];
-            const BodyState b1 = GLOBAL ? body_state_ordered(st, contact ? a : 0u) : body_state(st, contact ? a : 0u);
-            const BodyState b2 = GLOBAL ? body_state_ordered(st, contact ? b : 0u) : body_state(st, contact ? b : 0u);
+            // the light state's masses of this item's bodies, from its slot
+            ST sl = st;
+            if constexpr (LITE) {
+                const uint4 m = slot[LANE_ROWS * p_step];
+                sl.im = PairValue{a, __uint_as_float(m.x), __uint_as_float(m.z)};
+                sl.ii = PairValue{a, __uint_as_float(m.y), __uint_as_float(m.w)};
+            }
+            const BodyState b1 = GLOBAL ? body_state_ordered(sl, contact ? a : 0u) : body_state(sl, contact ? a : 0u);
+            const BodyState b2 = GLOBAL ? body_state_ordered(sl, contact ? b : 0u) : body_state(sl, contact ? b : 0u);
             if (!contact) {
-                rec_solve_joint(cur, st, a, b == NONE ? a : b, sub_dt);
+                rec_solve_joint(cur, sl, a, b == NONE ? a : b, sub_dt);
             } else {
-                contact_impulses<WARM>(cur, a, b, b1, b2, st);
+                contact_impulses<WARM>(cur, a, b, b1, b2, sl);
 #pragma unroll
                 for (int f = 0; f < 4; ++f) acc[f] = cur.acc[f];
             }
@@ -2427,21 +2477,34 @@ __global__ __launch_bounds__(BIG_THREADS) void k_big_items(StepDev S, uint32_t l
     for (uint32_t pos = t; pos < nitems; pos += T) {
         uint4* q = recs + (size_t)pos * 8;
         const uint4 h = q[0];
-        uint32_t c0, c1;
+        uint32_t c0, c1, s0, s1;
         if (h.x & RF_JOINT) {
             const Joint& J = D.joints[h.w];
             c0 = S.cmap[base + J.s1];
             c1 = J.s2 >= 0 ? S.cmap[base + J.s2] : 0xffffu;
+            s0 = (uint32_t)J.s1;
+            s1 = J.s2 >= 0 ? (uint32_t)J.s2 : s0;
         } else {
             c0 = S.cmap[base + h.y];
             c1 = S.cmap[base + h.z];
+            s0 = h.y;
+            s1 = h.z;
         }
         q[1].w = c0 | (c1 << 16);
+        // the light compact chain's masses of the item's bodies
+        S.rmass[ibase + pos] = make_uint4(__float_as_uint(S.g_im[base + s0]), __float_as_uint(S.g_ii[base + s0]),
+                                          __float_as_uint(S.g_im[base + s1]), __float_as_uint(S.g_ii[base + s1]));
     }
 }
 
 // LDS bytes of the compact chain per body (momentum, inverse masses, counter, item count)
 constexpr size_t COMPACT_BYTES = 7 * 4;
+constexpr size_t LITE_BYTES = 4 * 4;   // the light compact state: momentum and counter
+// the light compact chain runs in the 256-lane compact launch, with lane
+// slots of six rows; both chain launches decide with this one size, so
+// exactly one of them takes each world
+constexpr uint32_t LITE_LANES = 256;
+constexpr size_t LITE_SLOT_BYTES = (size_t)LITE_LANES * (LANE_ROWS + 1) * 16u;
 constexpr size_t COMPACT_LDS = 160 * 1024 - 1024;
 // lanes of the global-memory chain's workgroup
 constexpr int BIG_GLOBAL_THREADS = 1024;
@@ -2470,11 +2533,40 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
     uint4* recs = S.recs + (size_t)ibase * 8;
     const uint32_t nc = S.w_nc[w];
     const bool compact = (size_t)nc * COMPACT_BYTES <= COMPACT_LDS;
-    if (compact != (PART == 0)) return;   // uniform: the other launch takes this world
+    // the light compact state (momentum and counters, 16 B per touched body,
+    // beside lane slots of six rows) for the worlds the full one does not fit
+    const bool lite = !compact && nc < 0xffffu && (size_t)nc * LITE_BYTES + LITE_SLOT_BYTES <= COMPACT_LDS;
+    static_assert(PART != 0 || MAXT == LITE_LANES, "the light chain's slots are sized for the compact launch");
+    if ((compact || lite) != (PART == 0)) return;   // uniform: the other launch takes this world
     // the lane-slot chain when the slots fit beside the compact state
     constexpr size_t LANE_BYTES = (size_t)MAXT * LANE_WORDS * 4u;
     const bool lanes = (size_t)nc * COMPACT_BYTES + LANE_BYTES <= COMPACT_LDS;
     if constexpr (PART == 0) {
+        if (lite) {
+            LiteState st;
+            st.mx = (float*)(smem + LITE_SLOT_BYTES / 4);
+            st.my = st.mx + nc;
+            st.L = st.my + nc;
+            st.cnt = (uint32_t*)(st.L + nc);
+            st.im = PairValue{0u, 0.f, 0.f};
+            st.ii = PairValue{0u, 0.f, 0.f};
+            st.deg = DegLookup{S.deg, S.cinv, base};
+            st.rmass = S.rmass + ibase;
+            for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+                const uint32_t i = S.cinv[base + c];
+                st.mx[c] = D.mx[i]; st.my[c] = D.my[i]; st.L[c] = D.L[i];
+                st.cnt[c] = 0u;
+            }
+            __syncthreads();
+            item_chain_lanes<true, false, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+                                                s_abort);
+            __syncthreads();
+            for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+                const uint32_t i = S.cinv[base + c];
+                D.mx[i] = st.mx[c]; D.my[i] = st.my[c]; D.L[i] = st.L[c];
+            }
+            return;
+        }
         SolverState<true> st;
         st.mx = (float*)(smem + (lanes ? MAXT * LANE_WORDS : 0u));
         st.my = st.mx + nc;
