@@ -3100,7 +3100,7 @@ size_t fused_lds_bytes(uint32_t n, uint32_t tb, bool slots, uint32_t threads)
 // exclusive scan of a[0, len) in place (LDS or global), a[len] = the total,
 // which is returned; called by the whole workgroup after a barrier, ends
 // with one
-__device__ uint32_t group_scan(uint32_t* a, uint32_t len, uint32_t* sm)
+__device__ __forceinline__ uint32_t group_scan(uint32_t* a, uint32_t len, uint32_t* sm)
 {
     const uint32_t T = blockDim.x, t = threadIdx.x;
     const uint32_t chunk = (len + T - 1) / T;
@@ -3124,7 +3124,10 @@ __device__ uint32_t group_scan(uint32_t* a, uint32_t len, uint32_t* sm)
 // moves to a larger slice taken from the free end of the arrays (S.fpool).
 // False when the arrays are full; the world then stops (fused_overflow).
 // Called by the whole workgroup; the slice table entry is updated for the host.
-__device__ bool fused_grow(const StepDev& S, uint32_t w, WorldScratch& X, int k, uint32_t need, uint32_t& s_slot)
+// Inlined: as a call it takes the kernel's StepDev by reference, and the
+// kernel then copies its whole argument block to scratch at entry.
+__device__ __forceinline__ bool fused_grow(const StepDev& S, uint32_t w, WorldScratch& X, int k, uint32_t need,
+                                           uint32_t& s_slot)
 {
     const uint32_t cap = need + need / 2 + 64;
     const uint32_t take = k < 2 ? cap + 1 : cap;   // entry and candidate slices end with a scan total
