@@ -49,8 +49,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# LDS: 256 CUs x 128 B per clock x 2.4 GHz
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# LDS: 256 CUs x 256 B per clock x 2.4 GHz.  gfx950 serves 128-bit LDS reads
+# at twice the 128 B per clock of gfx942: the library's own LDS read
+# microbenchmark (r2d_debug_peaks) measures ~200 B per clock per CU
+LDS_PEAK_GBS = 256 * 256 * 2.4
 HORIZON = 240           # steps of a configuration-3 world and of a served scene
 G = 9.81
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
@@ -735,6 +737,8 @@ def main():
                          "print its time per step (for profiling runs)")
     ap.add_argument("--call-steps", type=int, default=40, help="one-step calls timed after the horizon")
     ap.add_argument("--variant", type=int, default=-1, help="solve-kernel variant (-1: library default)")
+    ap.add_argument("--peaks-only", action="store_true",
+                    help="only run the library's peak microbenchmarks (HBM read, copy, LDS) and print them")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB between two device buffers (a kernel of known HBM traffic, for "
                          "scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) and exit")
@@ -750,6 +754,12 @@ def main():
         assert bool(torch.equal(dst, src))
         print(json.dumps({"copy_mib": args.calibrate_copy, "launches": 3, "bytes_read_per_launch": n * 4,
                           "bytes_written_per_launch": n * 4}))
+        return
+    if args.peaks_only:
+        from resolve2d_amd.solver import Context
+        pctx = Context(0)
+        print(json.dumps(dict(pctx.measured_peaks(), unit="GB/s")), flush=True)
+        pctx.close()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: worlds are independent and would be sharded whole across GPUs "
@@ -976,6 +986,9 @@ def main():
                            note="SQ_INSTS_LDS of the committed rocprofv3 run of this window, x 256 B (64 lanes x "
                                 "4 B, the least a wave's LDS instruction moves), over this run's launch time")
     copy = hbm_copy_rate()
+    peaks = ctx.measured_peaks()
+    log(f"measured peaks: HBM read {peaks['hbm_read']:.0f}, copy {peaks['hbm_copy']:.0f}, "
+        f"LDS {peaks['lds_read']:.0f} GB/s")
 
     single = None
     if not args.no_single:
@@ -1116,9 +1129,18 @@ def main():
                      "launch_ms": launch_s * 1e3, "algorithmic_bytes_per_launch": launch_bytes,
                      "measured_copy_rate": copy,
                      "frac_of_measured_copy": achieved / copy["value"] if copy else None,
+                     "measured_peaks": dict(peaks, unit="GB/s", sample=(
+                         "the library's own microbenchmarks (r2d_debug_peaks): a 1 GiB read stream and a 1 GiB "
+                         "copy (read + written bytes) by 16-byte loads, and conflict-free 16-byte LDS reads on "
+                         "every CU")),
+                     "frac_of_measured_read": achieved / peaks["hbm_read"],
                      "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                              "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_launch": lds_bytes,
-                             "counted": lds_counted},
+                             "measured_peak": peaks["lds_read"],
+                             "frac_of_measured": lds_achieved / peaks["lds_read"],
+                             "counted": (dict(lds_counted, frac_lower_bound_of_measured=lds_counted[
+                                 "achieved_lower_bound"] / peaks["lds_read"])
+                                         if lds_counted and "achieved_lower_bound" in lds_counted else lds_counted)},
                      "dominant_phase": dom},
         "cpu_baseline": cpu,
         "setup_s": setup_s,

@@ -304,6 +304,14 @@ int r2d_last_error(char* buf, size_t n);
  * pairs (in[2i] / in[2i+1]; `in` holds 2n floats). */
 int r2d_debug_eval(r2d_ctx* ctx, int op, const float* in, float* out, size_t n);
 
+/* Measurement hook with no counterpart in the reference: the device's
+ * measured peaks, for rooflines against what the hardware delivers.
+ * out[0]: HBM read stream (GB/s, 1 GiB read by 16-byte loads, 4 in flight
+ * per thread); out[1]: HBM copy (GB/s of read + written bytes, 1 GiB);
+ * out[2]: LDS reads (GB/s over all CUs, conflict-free 16-byte reads).
+ * Allocates and frees 2 GiB of device memory and waits for the device. */
+int r2d_debug_peaks(r2d_ctx* ctx, double* out);
+
 /* ------------------------------------------------------- bulk ingestion */
 /* n bodies in one call, in order (ids as n calls of r2d_make_disc /
  * r2d_make_rect would assign them, EntityFactory.appendBody lib.zig:66-71):

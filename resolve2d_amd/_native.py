@@ -140,6 +140,7 @@ SIGNATURES = {
     "r2d_ctx_step_times": [P, ctypes.POINTER(ctypes.c_float), SZ, SZP],
     "r2d_ctx_last_counts": [P, ctypes.POINTER(ctypes.c_uint64)],
     "r2d_debug_eval": [P, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), SZ],
+    "r2d_debug_peaks": [P, ctypes.POINTER(ctypes.c_double)],
     "r2d_last_error": [ctypes.c_char_p, SZ],
     "r2d_ctx_set_solve_variant": [P, I32],
     "r2d_ctx_set_lds_limit": [P, U32],

@@ -2414,6 +2414,118 @@ R2D_EXPORT int r2d_debug_eval(r2d_ctx* c, int op, const float* in, float* out, s
     return R2D_OK;
 }
 
+// ---- measured peaks: HBM read and copy streams, LDS reads
+// Grid-stride stream over n16 16-byte elements, four independent loads in
+// flight per thread (n16 is a multiple of 4 x the grid's threads, so every
+// load is in bounds); the XOR of what was read goes to *sink only when it
+// equals a value it cannot take in practice, which keeps the loads.
+__global__ __launch_bounds__(256) void k_peak_read(const uint4* __restrict__ a, size_t n16, uint32_t* sink)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    uint32_t acc = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += 4 * stride) {
+        const uint4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        acc ^= v0.x ^ v0.y ^ v0.z ^ v0.w ^ v1.x ^ v1.y ^ v1.z ^ v1.w ^ v2.x ^ v2.y ^ v2.z ^ v2.w ^ v3.x ^ v3.y ^
+               v3.z ^ v3.w;
+    }
+    if (acc == 0x9e3779b9u) sink[0] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_peak_copy(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += 4 * stride) {
+        const uint4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0; b[i + stride] = v1; b[i + 2 * stride] = v2; b[i + 3 * stride] = v3;
+    }
+}
+
+// LDS reads: 16 KB per workgroup, each wave reading 64 consecutive 16-byte
+// rows per load (conflict-free), eight loads per round; the rows move with
+// the round so no load is loop-invariant
+constexpr uint32_t PEAK_LDS_ROWS = 1024;
+__global__ __launch_bounds__(256) void k_peak_lds(uint32_t rounds, uint32_t* sink)
+{
+    __shared__ uint4 buf[PEAK_LDS_ROWS];
+    for (uint32_t i = threadIdx.x; i < PEAK_LDS_ROWS; i += blockDim.x)
+        buf[i] = make_uint4(i, i * 3u, i * 5u, i * 7u);
+    __syncthreads();
+    uint32_t acc = 0;
+    for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint4 v = buf[(threadIdx.x + k * 128u + r * 16u) & (PEAK_LDS_ROWS - 1u)];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    if (acc == 0x9e3779b9u) sink[0] = acc;
+}
+
+R2D_EXPORT int r2d_debug_peaks(r2d_ctx* c, double* out)
+{
+    if (!c || !out) return R2D_E_INVALID_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    // 1 GiB per buffer: a multiple of 4 x every grid's threads swept below
+    const unsigned threads = 256;
+    const size_t n16 = ((size_t)1 << 30) / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    uint32_t* sink = nullptr;
+    if (hipMalloc((void**)&a, n16 * 16) != hipSuccess) return R2D_E_OOM;
+    if (hipMalloc((void**)&b, n16 * 16) != hipSuccess) { (void)hipFree(a); return R2D_E_OOM; }
+    if (hipMalloc((void**)&sink, 4) != hipSuccess) { (void)hipFree(a); (void)hipFree(b); return R2D_E_OOM; }
+    // every HIP status after the allocations is kept, so the buffers are
+    // freed on any failure
+    hipError_t err = hipSuccess;
+    auto chk = [&](hipError_t e) { if (err == hipSuccess && e != hipSuccess) err = e; };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    chk(hipEventCreate(&e0));
+    chk(hipEventCreate(&e1));
+    chk(hipMemsetAsync(a, 1, n16 * 16, c->stream));
+    chk(hipMemsetAsync(b, 0, n16 * 16, c->stream));
+    auto timed = [&](auto&& launch, int reps) -> double {
+        if (err != hipSuccess) return 0.0;
+        launch();                                  // warm-up
+        chk(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) launch();
+        chk(hipEventRecord(e1, c->stream));
+        chk(hipEventSynchronize(e1));
+        float ms = 0.f;
+        chk(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / 1e3 / reps;
+    };
+    // the streams at several grid sizes (1,024-16,384 workgroups): the fastest counts
+    double t_read = 0.0, t_copy = 0.0;
+    for (unsigned blocks = 1024; blocks <= 16384; blocks *= 2) {
+        const double tr = timed([&] {
+            hipLaunchKernelGGL(k_peak_read, dim3(blocks), dim3(threads), 0, c->stream, a, n16, sink);
+        }, 5);
+        const double tc = timed([&] {
+            hipLaunchKernelGGL(k_peak_copy, dim3(blocks), dim3(threads), 0, c->stream, a, b, n16);
+        }, 5);
+        if (tr > 0.0 && (t_read == 0.0 || tr < t_read)) t_read = tr;
+        if (tc > 0.0 && (t_copy == 0.0 || tc < t_copy)) t_copy = tc;
+    }
+    const uint32_t rounds = 4096, lds_blocks = 256 * 8;
+    const double t_lds = timed([&] {
+        hipLaunchKernelGGL(k_peak_lds, dim3(lds_blocks), dim3(threads), 0, c->stream, rounds, sink);
+    }, 5);
+    chk(hipGetLastError());
+    chk(hipStreamSynchronize(c->stream));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(sink);
+    if (err != hipSuccess || t_read <= 0.0 || t_copy <= 0.0 || t_lds <= 0.0) {
+        note_error("r2d_debug_peaks", err, __LINE__);
+        return R2D_E_DEVICE;
+    }
+    out[0] = (double)(n16 * 16) / t_read / 1e9;
+    out[1] = (double)(2 * n16 * 16) / t_copy / 1e9;
+    out[2] = (double)lds_blocks * threads * 16.0 * 8.0 * rounds / t_lds / 1e9;
+    return R2D_OK;
+}
+
 R2D_EXPORT int r2d_last_error(char* buf, size_t n)
 {
     if (!buf || n == 0) return R2D_E_INVALID_ARG;

@@ -130,6 +130,13 @@ class Context:
         N.check(self.lib.r2d_ctx_step_times(self.handle, out, n.value, ctypes.byref(n)), "r2d_ctx_step_times")
         return list(out)[:n.value]
 
+    def measured_peaks(self):
+        """The device's measured peaks in GB/s: HBM read stream, HBM copy
+        (read + written bytes), LDS reads over all CUs."""
+        out = (ctypes.c_double * 3)()
+        N.check(self.lib.r2d_debug_peaks(self.handle, out), "r2d_debug_peaks")
+        return {"hbm_read": out[0], "hbm_copy": out[1], "lds_read": out[2]}
+
     def reset_phase_times(self):
         N.check(self.lib.r2d_ctx_reset_phase_times(self.handle), "r2d_ctx_reset_phase_times")
 
