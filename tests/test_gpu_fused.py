@@ -169,3 +169,33 @@ def test_fused_single_steps_equal_one_call():
     for w in a + b:
         w.close()
     ctx.close()
+
+
+def test_size_boundary_of_the_fused_path(gpu_ctx, oracle):
+    """Worlds of exactly 1,024 bodies (the fused kernel's limit, ground
+    included) take the fused kernel; a batch holding one world of 1,025
+    bodies takes the phase kernels for all its worlds.  Both give the
+    oracle's bits over 20 steps."""
+    from resolve2d_amd import scenes
+    at = [scenes.mixed(1023, seed=s, width=30.0, height=40.0, ground_w=40.0) for s in (11, 12)]
+    over = scenes.mixed(1024, seed=13, width=30.0, height=40.0, ground_w=40.0)
+    small = scenes.mixed(60, seed=14, width=10.0, height=10.0, ground_w=14.0)
+    for specs, fused in ((at, True), (at[:1] + [over, small], False)):
+        print(f"size boundary: {[sum(1 for op in s if op[0] in ('disc', 'rect')) for s in specs]} bodies, "
+              f"fused expected {fused}", flush=True)
+        gw = _worlds(gpu_ctx, specs)
+        gpu_ctx.set_profiling(True)
+        gpu_ctx.reset_phase_times()
+        try:
+            gpu_ctx.process_batch_handles([g.handle for g in gw], DT, 4, 4, 20)
+            launches = gpu_ctx.fused_times()[1]
+        finally:
+            gpu_ctx.set_profiling(False)
+        assert (launches > 0) == fused, (fused, launches)
+        for g, ops in zip(gw, specs):
+            assert g.bodies().shape[0] == sum(1 for op in ops if op[0] in ("disc", "rect"))
+            o = _oracle_run(oracle, ops, 20)
+            assert bits_equal(g.bodies(), o.bodies()), first_diff(g.bodies(), o.bodies())
+            assert bits_equal(g.manifolds(), o.manifolds())
+            o.close()
+            g.close()
