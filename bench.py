@@ -56,7 +56,7 @@ LDS_PEAK_GBS = 256 * 256 * 2.4
 HORIZON = 240           # steps of a configuration-3 world and of a served scene
 G = 9.81
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
-# host processes at most: the GPU box's CPU share for one GPU
+# host processes at most: the host CPU share that comes with one GPU of an MI355X node
 CPU_SHARE = 16
 
 # fused kernel shape of each solve variant: <slots, workgroup size, waves per SIMD>
@@ -798,7 +798,7 @@ def main():
                "sample": f"oracle f32 restatement (the reference's algorithm in C), {len(sample)} of the "
                          f"{args.worlds} configuration-3 worlds ({s_bodies} bodies, every {stride}th) from their "
                          f"start over the {HORIZON}-step horizon, worlds in parallel over {used} processes (the "
-                         f"box's CPU share is {CPU_SHARE}; the host shows {ncpu}); median of "
+                         f"host's CPU share per GPU is {CPU_SHARE}; the host shows {ncpu}); median of "
                          f"{args.cpu_repeats} passes"}
         # configuration 5: served scenes sampled from the same size law
         rng = np.random.default_rng(args.seed + 7)
