@@ -95,6 +95,29 @@ def test_config3_batched_sample(gpu_ctx, oracle):
         w.close()
 
 
+def test_config3_every_world_bit_exact(gpu_ctx, oracle):
+    """The whole benchmark batch (1,024 worlds, 286,693 bodies) after its
+    240-step horizon in one call: every world bit for bit against the oracle
+    stepping it alone (the oracle runs in host processes)."""
+    import os
+    from oracle_pool import oracle_bodies
+    specs = [scenes.batched_world(1000 + i) for i in range(1024)]
+    worlds = []
+    for sc in specs:
+        w = GpuWorld(gpu_ctx)
+        apply_scene(sc, w)
+        worlds.append(w)
+    print("config 3, every world: 1,024 worlds x 240 steps on the GPU", flush=True)
+    gpu_ctx.process_batch_handles([w.handle for w in worlds], scenes.DT, 4, 4, 240)
+    procs = max(1, min(15, (os.cpu_count() or 2) - 1))
+    print(f"config 3, every world: the oracle over {procs} host processes", flush=True)
+    ref = oracle_bodies(specs, 240, scenes.DT, 4, 4, procs)
+    bad = [i for i, (w, b) in enumerate(zip(worlds, ref)) if not bits_equal(w.bodies(), b)]
+    assert not bad, (len(bad), bad[:8], first_diff(worlds[bad[0]].bodies(), ref[bad[0]]))
+    for w in worlds:
+        w.close()
+
+
 def test_empty_and_static_only_worlds(gpu_ctx, oracle):
     g = GpuWorld(gpu_ctx)
     assert g.process(scenes.DT, 4, 4) == 0 and g.bodies().shape == (0, 19)

@@ -70,3 +70,15 @@ def test_spatial_hash_function(oracle):
     for (t, x, y) in [(10, 0, 0), (422, 3, -7), (8194, -123456, 98765), (2, -1, -1), (1000003, 2**31, -2**31)]:
         h = (((x * 92837111) % M) ^ ((y * 689287499) % M)) % t
         assert lib.or_hash(t, x, y) == h
+
+
+def test_oracle_pool_equals_in_process():
+    """The spawned-process oracle runner used by the full-batch GPU check
+    gives the in-process oracle's bits."""
+    from helpers import bits_equal
+    from oracle_pool import oracle_bodies, run_worlds
+    from resolve2d_amd import scenes
+    specs = [scenes.batched_world(1000 + i, n_min=50, n_max=120) for i in range(6)]
+    ref = oracle_bodies(specs, 10, scenes.DT, 4, 4, 3)
+    loc = run_worlds((specs, 10, scenes.DT, 4, 4))
+    assert len(ref) == len(specs) and all(bits_equal(a, b) for a, b in zip(ref, loc))
