@@ -28,6 +28,11 @@
 //!     for signature compatibility; the library owns its memory (host mirror
 //!     and device state).
 //!
+//! Beyond the reference's surface: bulk ingestion (`makeBodies`), readback
+//! (`Solver.getBodies` / `getManifolds`), the tuned hash, warm starting,
+//! the coloured schedule, the step without allocation (`processStatic`) and
+//! continuous batching (`Server`).
+//!
 //! No Zig toolchain exists where this library is built, so this file is
 //! written against include/r2d.h and has not been compiled there.
 const std = @import("std");
@@ -108,8 +113,115 @@ extern fn r2d_get_body(w: *World, id: u32, out: *CBodyState) c_int;
 extern fn r2d_set_body(w: *World, id: u32, in: *const CBodyState) c_int;
 extern fn r2d_set_warm_start(w: *World, factor: f32) c_int;
 extern fn r2d_set_schedule(w: *World, mode: c_int) c_int;
+extern fn r2d_set_hash_params(w: *World, mode: c_int, cell_width: f32, table_mult: usize) c_int;
+extern fn r2d_process_static(w: *World, dt: f32, sub_steps: usize, collision_iters: usize) c_int;
+extern fn r2d_make_bodies(w: *World, n: usize, bo: [*]const CBodyOptions, kind: [*]const i32, dim_a: [*]const f32, dim_b: [*]const f32, ids_out: ?[*]u32) c_int;
+extern fn r2d_get_bodies(w: *World, out: [*]CBodyState, cap: usize, n_out: *usize) c_int;
+extern fn r2d_get_manifolds(w: *World, out: [*]CManifoldState, cap: usize, n_out: *usize) c_int;
+extern fn r2d_last_error(buf: [*]u8, n: usize) c_int;
+extern fn r2d_world_footprint(w: *World, bytes: *usize) c_int;
+extern fn r2d_serve_create(ctx: *Ctx, opt: *const ServeOptions, out: *?*Server.Handle) c_int;
+extern fn r2d_serve_destroy(sv: *Server.Handle) c_int;
+extern fn r2d_serve_submit(sv: *Server.Handle, w: *World, steps: u32, ticket: *u64) c_int;
+extern fn r2d_serve_tick(sv: *Server.Handle, info: ?*ServeTickInfo) c_int;
+extern fn r2d_serve_drain(sv: *Server.Handle) c_int;
+extern fn r2d_serve_poll(sv: *Server.Handle, ticket: u64, out: *RequestStatus) c_int;
+extern fn r2d_serve_completed(sv: *Server.Handle, tickets: [*]u64, cap: u32, n: *u32) c_int;
+extern fn r2d_serve_stats(sv: *Server.Handle, out: *ServeSummary) c_int;
 
-pub const Error = error{ InvalidRigidBodyId, NoSuchIdExists, OutOfMemory, NonFinite, Device, Busy, InvalidArgument };
+/// One contact point of a manifold (collision.zig:22-51), as r2d_get_manifolds reports it.
+pub const CManifoldPoint = extern struct {
+    valid: u32,
+    acc_pn: f32,
+    acc_pt: f32,
+    ref_r_x: f32,
+    ref_r_y: f32,
+    inc_r_x: f32,
+    inc_r_y: f32,
+    pos_x: f32,
+    pos_y: f32,
+    depth: f32,
+    original_depth: f32,
+    mass_n: f32,
+    mass_t: f32,
+};
+
+/// One contact manifold (collision.zig:54-71).
+pub const CManifoldState = extern struct {
+    ref_id: u32,
+    inc_id: u32,
+    normal_x: f32,
+    normal_y: f32,
+    tangent_x: f32,
+    tangent_y: f32,
+    friction: f32,
+    points: [2]CManifoldPoint,
+};
+
+/// r2d_serve_options (include/r2d.h).
+pub const ServeOptions = extern struct {
+    hbm_budget_bytes: usize,
+    max_worlds: u32,
+    dt: f32,
+    sub_steps: u32,
+    collision_iters: u32,
+    split_bodies: [2]u32,
+    steps_per_tick: u32,
+};
+
+pub const ServeTickInfo = extern struct {
+    admitted: u32,
+    completed: u32,
+    queued: u32,
+    running: u32,
+    launched_worlds: u32,
+    launched_bodies: u64,
+    resident_bytes: usize,
+    step_gpu_ms: f64,
+    step_wall_ms: f64,
+};
+
+pub const RequestState = enum(i32) { queued = 0, running = 1, done = 2, failed = 3 };
+
+pub const RequestStatus = extern struct {
+    state: i32, // RequestState
+    status: i32,
+    steps_done: u32,
+    steps_requested: u32,
+    footprint_bytes: usize,
+    lane: u32,
+    wait_ms: f64,
+    total_ms: f64,
+    max_step_ms: f64,
+};
+
+pub const ServeSummary = extern struct {
+    submitted: u64,
+    completed: u64,
+    rejected: u64,
+    queued: u64,
+    running: u64,
+    batch_steps: u64,
+    scene_steps: u64,
+    body_steps: u64,
+    gpu_ms: f64,
+    wall_ms: f64,
+    p50_step_ms: f64,
+    p99_step_ms: f64,
+    p50_batch_ms: f64,
+    p99_batch_ms: f64,
+    resident_bytes: usize,
+    peak_resident_bytes: usize,
+    lanes: u32,
+};
+
+/// The text of the calling thread's last failure (which HIP call, which world).
+pub fn lastError(buf: []u8) []const u8 {
+    if (buf.len == 0 or r2d_last_error(buf.ptr, buf.len) != 0) return buf[0..0];
+    return std.mem.sliceTo(buf, 0);
+}
+
+pub const Error = error{ InvalidRigidBodyId, NoSuchIdExists, OutOfMemory, NonFinite, Device, Busy, InvalidArgument, OutOfRange, Capacity };
 
 fn check(rc: c_int) Error!void {
     return switch (rc) {
@@ -118,6 +230,8 @@ fn check(rc: c_int) Error!void {
         2 => error.InvalidRigidBodyId,
         3 => error.OutOfMemory,
         4 => error.NonFinite,
+        7 => error.OutOfRange,
+        8 => error.Capacity,
         9 => error.Busy,
         else => error.Device,
     };
@@ -423,6 +537,119 @@ pub const Solver = struct {
     /// 0: the reference's order (default); 1: graph-coloured batches.
     pub fn setSchedule(self: *Self, mode: c_int) Error!void {
         try check(r2d_set_schedule(self.world, mode));
+    }
+
+    /// Spatial-hash parameters: mode 0 the reference's cell 4.0 and 2N
+    /// buckets (lib.zig:254-255); mode 1 the given cell width and
+    /// table_mult * N buckets (README.md:62-63).
+    pub fn setHashParams(self: *Self, mode: c_int, cell_width: f32, table_mult: usize) Error!void {
+        try check(r2d_set_hash_params(self.world, mode, cell_width, table_mult));
+    }
+
+    /// `process` for callers that cannot allocate device memory: the step's
+    /// scratch is a fixed device array of the library (worlds of at most
+    /// 1,024 bodies, reference order, no warm starting; error.Capacity
+    /// beyond).  Calls on one GPU must not overlap.
+    pub fn processStatic(self: *Self, dt: f32, sub_steps: usize, collision_iters: usize) Error!void {
+        try check(r2d_process_static(self.world, dt, sub_steps, collision_iters));
+    }
+
+    /// All bodies in iteration order into `buf`; the slice of it filled, or
+    /// error.OutOfRange when `buf` is too short (numBodies gives the size).
+    pub fn getBodies(self: *Self, buf: []CBodyState) Error![]CBodyState {
+        var n: usize = 0;
+        try check(r2d_get_bodies(self.world, buf.ptr, buf.len, &n));
+        if (n > buf.len) return error.OutOfRange;
+        return buf[0..n];
+    }
+
+    /// The last step's manifolds in the reference's order (Solver.manifolds);
+    /// error.OutOfRange when `buf` is too short.
+    pub fn getManifolds(self: *Self, buf: []CManifoldState) Error![]CManifoldState {
+        var n: usize = 0;
+        try check(r2d_get_manifolds(self.world, buf.ptr, buf.len, &n));
+        if (n > buf.len) return error.OutOfRange;
+        return buf[0..n];
+    }
+
+    /// Device bytes a server charges this world at admission.
+    pub fn footprint(self: *Self) Error!usize {
+        var b: usize = 0;
+        try check(r2d_world_footprint(self.world, &b));
+        return b;
+    }
+};
+
+// ----------------------------------------------------------- bulk bodies
+/// Many bodies in one call, ids as the same number of makeDiscBody /
+/// makeRectangleBody calls would assign them (EntityFactory.appendBody,
+/// lib.zig:66-71): kind[i] 0 disc (dim_a = radius), 1 rectangle (dim_a x
+/// dim_b).  From 32 bodies on the device builds them.
+pub fn makeBodies(solver: *Solver, bo: []const EntityFactory.BodyOptions, kind: []const i32, dim_a: []const f32, dim_b: []const f32, ids_out: ?[]u32, scratch: Allocator) Error!void {
+    const n = bo.len;
+    if (kind.len != n or dim_a.len != n or dim_b.len != n) return error.InvalidArgument;
+    if (ids_out) |ids| if (ids.len != n) return error.InvalidArgument;
+    const cbo = scratch.alloc(CBodyOptions, n) catch return error.OutOfMemory;
+    defer scratch.free(cbo);
+    for (bo, cbo) |b, *c| c.* = b.toC();
+    try check(r2d_make_bodies(solver.world, n, cbo.ptr, kind.ptr, dim_a.ptr, dim_b.ptr, if (ids_out) |ids| ids.ptr else null));
+}
+
+// ----------------------------------------------------------------- serving
+/// Continuous batching of many worlds on one device (no counterpart in the
+/// reference): requests -- a built Solver and a number of steps -- are
+/// admitted while their footprints fit the HBM budget, stepped in lanes by
+/// world size, and each world's result is bit-identical to stepping it
+/// alone.  The solver of a running request refuses edits (error.Busy).
+pub const Server = struct {
+    pub const Handle = opaque {};
+    handle: *Handle,
+
+    pub fn create(device: *Device, opt: ServeOptions) Error!Server {
+        var h: ?*Handle = null;
+        try check(r2d_serve_create(device.ctx, &opt, &h));
+        return .{ .handle = h.? };
+    }
+
+    pub fn destroy(self: *Server) Error!void {
+        try check(r2d_serve_destroy(self.handle));
+    }
+
+    /// Queues `steps` steps of the solver; returns its ticket.
+    pub fn submit(self: *Server, solver: *Solver, steps: u32) Error!u64 {
+        var t: u64 = 0;
+        try check(r2d_serve_submit(self.handle, solver.world, steps, &t));
+        return t;
+    }
+
+    /// Collects finished lane steps, retires, admits and launches.
+    pub fn tick(self: *Server) Error!ServeTickInfo {
+        var info: ServeTickInfo = undefined;
+        try check(r2d_serve_tick(self.handle, &info));
+        return info;
+    }
+
+    pub fn drain(self: *Server) Error!void {
+        try check(r2d_serve_drain(self.handle));
+    }
+
+    pub fn poll(self: *Server, ticket: u64) Error!RequestStatus {
+        var st: RequestStatus = undefined;
+        try check(r2d_serve_poll(self.handle, ticket, &st));
+        return st;
+    }
+
+    /// Tickets finished since the last call, oldest first, into `buf`.
+    pub fn completed(self: *Server, buf: []u64) Error![]u64 {
+        var n: u32 = 0;
+        try check(r2d_serve_completed(self.handle, buf.ptr, @intCast(buf.len), &n));
+        return buf[0..n];
+    }
+
+    pub fn stats(self: *Server) Error!ServeSummary {
+        var s: ServeSummary = undefined;
+        try check(r2d_serve_stats(self.handle, &s));
+        return s;
     }
 };
 
