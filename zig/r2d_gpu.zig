@@ -120,6 +120,14 @@ extern fn r2d_get_bodies(w: *World, out: [*]CBodyState, cap: usize, n_out: *usiz
 extern fn r2d_get_manifolds(w: *World, out: [*]CManifoldState, cap: usize, n_out: *usize) c_int;
 extern fn r2d_last_error(buf: [*]u8, n: usize) c_int;
 extern fn r2d_world_footprint(w: *World, bytes: *usize) c_int;
+extern fn r2d_ctx_sync(ctx: *Ctx) c_int;
+extern fn r2d_ctx_set_solve_variant(ctx: *Ctx, variant: c_int) c_int;
+extern fn r2d_ctx_set_fused(ctx: *Ctx, on: c_int) c_int;
+extern fn r2d_ctx_set_profiling(ctx: *Ctx, on: c_int) c_int;
+extern fn r2d_ctx_reset_phase_times(ctx: *Ctx) c_int;
+extern fn r2d_ctx_fused_times(ctx: *Ctx, kernel_ms: *f64, launches: *u64, world_steps: *u64) c_int;
+extern fn r2d_ctx_step_times(ctx: *Ctx, out: [*]f32, cap: usize, n: *usize) c_int;
+extern fn r2d_debug_peaks(ctx: *Ctx, out: *[3]f64) c_int;
 extern fn r2d_serve_create(ctx: *Ctx, opt: *const ServeOptions, out: *?*Server.Handle) c_int;
 extern fn r2d_serve_destroy(sv: *Server.Handle) c_int;
 extern fn r2d_serve_submit(sv: *Server.Handle, w: *World, steps: u32, ticket: *u64) c_int;
@@ -269,6 +277,56 @@ pub const Device = struct {
             try check(r2d_process_batch(self.ctx, &buf, n, dt, sub_steps, collision_iters, steps));
             done += n;
         }
+    }
+
+    /// Waits for all device work of the context.
+    pub fn sync(self: *Device) Error!void {
+        try check(r2d_ctx_sync(self.ctx));
+    }
+
+    /// Launch-shape knobs (results are the same bits for every setting):
+    /// the solve variant (-1: the library's default), whole steps per
+    /// workgroup on or off.
+    pub fn setSolveVariant(self: *Device, variant: c_int) Error!void {
+        try check(r2d_ctx_set_solve_variant(self.ctx, variant));
+    }
+
+    pub fn setFused(self: *Device, on: bool) Error!void {
+        try check(r2d_ctx_set_fused(self.ctx, @intFromBool(on)));
+    }
+
+    /// Device timing of the following calls (per step and per fused launch).
+    pub fn setProfiling(self: *Device, on: bool) Error!void {
+        try check(r2d_ctx_set_profiling(self.ctx, @intFromBool(on)));
+    }
+
+    pub fn resetPhaseTimes(self: *Device) Error!void {
+        try check(r2d_ctx_reset_phase_times(self.ctx));
+    }
+
+    pub const FusedTimes = struct { kernel_ms: f64, launches: u64, world_steps: u64 };
+
+    /// Device time of the profiled fused launches since the last reset.
+    pub fn fusedTimes(self: *Device) Error!FusedTimes {
+        var t: FusedTimes = .{ .kernel_ms = 0, .launches = 0, .world_steps = 0 };
+        try check(r2d_ctx_fused_times(self.ctx, &t.kernel_ms, &t.launches, &t.world_steps));
+        return t;
+    }
+
+    /// Device time (ms) of every profiled step since the last reset, into `buf`.
+    pub fn stepTimes(self: *Device, buf: []f32) Error![]f32 {
+        var n: usize = 0;
+        try check(r2d_ctx_step_times(self.ctx, buf.ptr, buf.len, &n));
+        return buf[0..@min(n, buf.len)];
+    }
+
+    pub const Peaks = struct { hbm_read: f64, hbm_copy: f64, lds_read: f64 };
+
+    /// The device's measured HBM read / copy and LDS read rates (GB/s).
+    pub fn measuredPeaks(self: *Device) Error!Peaks {
+        var out: [3]f64 = undefined;
+        try check(r2d_debug_peaks(self.ctx, &out));
+        return .{ .hbm_read = out[0], .hbm_copy = out[1], .lds_read = out[2] };
     }
 };
 
