@@ -3050,6 +3050,47 @@ __device__ __forceinline__ uint32_t pair_passes(const PairCache& P, uint32_t a, 
     return aabb_hit(P, a, b) ? 1u : 0u;
 }
 
+// The raw candidates of one cell entry of body a: the distinct members of
+// its bucket above a (s[0, c), members in descending order, a body's repeats
+// adjacent) that pass pair_passes, as the serial walk
+//   for (j = 0; j < c && s[j] > a; ++j)
+//       if (j == 0 || s[j] != s[j - 1]) { ok = pair_passes(P, a, s[j]); ... }
+// counts them: their number, and the passing positions below 32 as a mask.
+// Four members at a time, so the filters' LDS loads of four bodies are in
+// flight together instead of one body's at a time.
+__device__ __forceinline__ void entry_candidates(const PairCache& P, uint32_t a, const uint32_t* s, uint32_t c,
+                                                 uint32_t& q_out, uint32_t& mask_out)
+{
+    uint32_t q = 0, mask = 0, prev = 0;
+    for (uint32_t j = 0; j < c; j += 4) {
+        uint32_t m[4];
+        bool live[4];
+        bool stop = false;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) m[k] = j + k < c ? s[j + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const bool above = j + k < c && m[k] > a;
+            const uint32_t before = k ? m[k - 1] : prev;
+            live[k] = !stop && above && (j + k == 0 || m[k] != before);
+            stop = stop || !above;
+        }
+        uint32_t ok[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) ok[k] = pair_passes(P, a, m[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const bool pass = live[k] && ok[k] != 0u;
+            q += pass;
+            if (pass && j + k < 32u) mask |= 1u << (j + k);
+        }
+        if (stop) break;
+        prev = m[3];
+    }
+    q_out = q;
+    mask_out = mask;
+}
+
 // LDS of the fused kernel for a world of n bodies and tb hash buckets
 struct FusedLds {
     uint32_t* coff;    // [n+1] cells per body -> entry offsets
@@ -3320,14 +3361,10 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         // passing members among the first 32 as a mask (in `items`, whose
         // bucket lists are no longer read)
         for (uint32_t e = t; e < E; e += T) {
-            uint32_t a = ent_body[e], b = ent_bkt[e];
-            uint32_t o = L.boff[b], c = L.boff[b + 1] - o, q = 0, mask = 0;
-            for (uint32_t j = 0; j < c && sorted[o + j] > a; ++j)
-                if (j == 0 || sorted[o + j] != sorted[o + j - 1]) {
-                    const bool ok = pair_passes(P, a, sorted[o + j]);
-                    q += ok;
-                    if (ok && j < 32) mask |= 1u << j;
-                }
+            const uint32_t a = ent_body[e], b = ent_bkt[e];
+            const uint32_t o = L.boff[b], c = L.boff[b + 1] - o;
+            uint32_t q, mask;
+            entry_candidates(P, a, sorted + o, c, q, mask);
             ecnt[e] = q;
             items[e] = mask;
         }
