@@ -451,6 +451,27 @@ __global__ void k_bucket_fill(StepDev S)
     }
 }
 
+// A cell entry's rank among its bucket's members it[0, c): how many are
+// greater than its body a and how many equal it.  Four members per
+// iteration, so their loads are in flight together.
+__device__ __forceinline__ void bucket_rank_of(const uint32_t* it, uint32_t c, uint32_t a, uint32_t& greater,
+                                               uint32_t& same)
+{
+    uint32_t g = 0, e = 0, j = 0;
+    for (; j + 4 <= c; j += 4) {
+        const uint32_t v0 = it[j], v1 = it[j + 1], v2 = it[j + 2], v3 = it[j + 3];
+        g += (v0 > a) + (v1 > a) + (v2 > a) + (v3 > a);
+        e += (v0 == a) + (v1 == a) + (v2 == a) + (v3 == a);
+    }
+    for (; j < c; ++j) {
+        const uint32_t v = it[j];
+        g += v > a;
+        e += v == a;
+    }
+    greater = g;
+    same = e;
+}
+
 // Within a bucket the reference stores entries in reverse insertion order
 // (pre-decrement scatter, SpatialHash.zig:36-40, 59-68): descending body slot.
 // Each entry's place is the number of bucket members with a higher slot plus
@@ -466,12 +487,8 @@ __global__ void k_bucket_rank(StepDev S)
         uint32_t a = S.ent_body[e], gb = S.ent_bkt[e];
         uint32_t o = S.bkt_off[gb], n = S.bkt_cnt[gb];
         const uint32_t* it = S.bkt_items + o;
-        uint32_t greater = 0, same = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-            uint32_t v = it[j];
-            greater += v > a;
-            same += v == a;
-        }
+        uint32_t greater, same;
+        bucket_rank_of(it, n, a, greater, same);
         uint32_t dup = 0;
         if (same > 1)
             for (uint32_t x = S.cell_off[a]; x < e; ++x) dup += S.ent_bkt[x] == gb;
@@ -3343,12 +3360,8 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         for (uint32_t e = t; e < E; e += T) {
             uint32_t a = ent_body[e], b = ent_bkt[e];
             uint32_t o = L.boff[b], c = L.boff[b + 1] - o;
-            uint32_t greater = 0, same = 0;
-            for (uint32_t j = 0; j < c; ++j) {
-                uint32_t v = items[o + j];
-                greater += v > a;
-                same += v == a;
-            }
+            uint32_t greater, same;
+            bucket_rank_of(items + o, c, a, greater, same);
             uint32_t dup = 0;
             if (same > 1)
                 for (uint32_t x = L.coff[a]; x < e; ++x) dup += ent_bkt[x] == b;
