@@ -451,6 +451,19 @@ __global__ void k_bucket_fill(StepDev S)
     }
 }
 
+// How many of l[0, c) are below v: four entries per iteration, their loads
+// in flight together
+__device__ __forceinline__ uint32_t count_below(const uint32_t* l, uint32_t c, uint32_t v)
+{
+    uint32_t r = 0, i = 0;
+    for (; i + 4 <= c; i += 4) {
+        const uint32_t a0 = l[i], a1 = l[i + 1], a2 = l[i + 2], a3 = l[i + 3];
+        r += (a0 < v) + (a1 < v) + (a2 < v) + (a3 < v);
+    }
+    for (; i < c; ++i) r += l[i] < v;
+    return r;
+}
+
 // A cell entry's rank among its bucket's members it[0, c): how many are
 // greater than its body a and how many equal it.  Four members per
 // iteration, so their loads are in flight together.
@@ -3469,31 +3482,55 @@ void k_fused(StepDev S, float dt, uint32_t sub_steps, uint32_t iters, uint32_t n
         for (uint32_t i = t; i < n; i += T) L.doff[i] = L.dcnt[i];
         __syncthreads();
         group_scan(L.doff, n, s_sm);
-        // per-body item lists (k_list_fill), ranks in item order (k_list_rank)
+        // per-body item lists (k_list_fill), ranks in item order (k_list_rank).
+        // When the lists fit the front part of LDS (its cell, trig and bucket
+        // arrays are dead after the contacts), they are filled there in any
+        // order and each entry then finds its rank by counting the smaller
+        // entries of its body's list (independent LDS reads) and goes to that
+        // place of the global lists; else they are filled in global memory and
+        // sorted per body.
+        uint32_t* const ul = L.coff;
+        const bool lds_lists = L.doff[n] <= 3u * n + 2u * tb + 2u;
+        uint32_t* const fill = lds_lists ? ul : lists;
         for (uint32_t j = t; j < nit; j += T) {
             const int32_t r0 = S.it_r0[X.i0 + j], r1 = S.it_r1[X.i0 + j];
             if (r0 >= 0) {
                 const uint32_t x = (uint32_t)r0 - base;
-                lists[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u;
+                fill[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u;
             }
             if (r1 >= 0) {
                 const uint32_t x = (uint32_t)r1 - base;
-                lists[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u + 1u;
+                fill[L.doff[x] + atomicAdd(&L.dfill[x], 1u)] = j * 2u + 1u;
             }
         }
         __syncthreads();
-        for (uint32_t x = t; x < n; x += T) {
-            uint32_t* l = lists + L.doff[x];
-            const uint32_t c = L.dcnt[x];
-            for (uint32_t i = 1; i < c; ++i) {
-                uint32_t v = l[i];
-                uint32_t j = i;
-                while (j > 0 && l[j - 1] > v) { l[j] = l[j - 1]; --j; }
-                l[j] = v;
+        if (lds_lists) {
+            for (uint32_t j = t; j < nit; j += T) {
+#pragma unroll
+                for (uint32_t side = 0; side < 2; ++side) {
+                    const int32_t r = side ? S.it_r1[X.i0 + j] : S.it_r0[X.i0 + j];
+                    if (r < 0) continue;
+                    const uint32_t x = (uint32_t)r - base, o = L.doff[x];
+                    const uint32_t v = j * 2u + side;
+                    const uint32_t rank = count_below(ul + o, L.dcnt[x], v);
+                    lists[o + rank] = v;
+                    if (side) S.it_k1[X.i0 + j] = rank; else S.it_k0[X.i0 + j] = rank;
+                }
             }
-            for (uint32_t p = 0; p < c; ++p) {
-                uint32_t v = l[p];
-                if (v & 1u) S.it_k1[X.i0 + (v >> 1)] = p; else S.it_k0[X.i0 + (v >> 1)] = p;
+        } else {
+            for (uint32_t x = t; x < n; x += T) {
+                uint32_t* l = lists + L.doff[x];
+                const uint32_t c = L.dcnt[x];
+                for (uint32_t i = 1; i < c; ++i) {
+                    uint32_t v = l[i];
+                    uint32_t j = i;
+                    while (j > 0 && l[j - 1] > v) { l[j] = l[j - 1]; --j; }
+                    l[j] = v;
+                }
+                for (uint32_t p = 0; p < c; ++p) {
+                    uint32_t v = l[p];
+                    if (v & 1u) S.it_k1[X.i0 + (v >> 1)] = p; else S.it_k0[X.i0 + (v >> 1)] = p;
+                }
             }
         }
         __syncthreads();
