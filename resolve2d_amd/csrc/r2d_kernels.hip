@@ -1308,23 +1308,29 @@ __device__ __forceinline__ void rec_build(Rec& r, const StepDev& S, const WorldD
 // updateTGSDepth (collision.zig:73-100) for a record whose points may move:
 // rotate lever arms by the bodies' rotation since the previous sub-step,
 // recompute depth and (preStep) the effective masses
-template <class ST>
-__device__ __forceinline__ void rec_tgs(Rec& r, const StepDev& S, const WorldDev& D, const ST& st, bool = false)
+// DV: WorldDev or PassArrays (the body arrays it reads: ang, px, py)
+template <class ST, class DV>
+__device__ __forceinline__ void rec_tgs(Rec& r, const StepDev& S, const DV& D, const ST& st, bool = false)
 {
     const uint32_t m = r.idx;
     const size_t cm = S.cap_m;
     const float* P = S.m_pt;
-    float a1 = D.ang[r.a], a2 = D.ang[r.b];
-    float prev1 = S.m_prev1[m], prev2 = S.m_prev2[m];
+    // everything the update may read, loaded before any of it is tested (one
+    // round trip instead of three dependent ones; the indices are valid
+    // whether or not a point is present or moved)
+    const float a1 = D.ang[r.a], a2 = D.ang[r.b];
+    const float prev1 = S.m_prev1[m], prev2 = S.m_prev2[m];
+    const float ods[2] = {P[(MP_ODEPTH * 2 + 0) * cm + m], P[(MP_ODEPTH * 2 + 1) * cm + m]};
+    const v2 pa = mk(D.px[r.a], D.py[r.a]), pb = mk(D.px[r.b], D.py[r.b]);
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         if (!(r.flags & (RF_P0 << p))) continue;
-        float od = P[(MP_ODEPTH * 2 + p) * cm + m];
+        const float od = ods[p];
         if (approx(r.p[p][4], od, 1e-4f)) continue;
         v2 rr1 = rotate2(mk(r.p[p][0], r.p[p][1]), a1 - prev1);
         v2 rr2 = rotate2(mk(r.p[p][2], r.p[p][3]), a2 - prev2);
-        v2 q1 = add2(rr1, mk(D.px[r.a], D.py[r.a]));
-        v2 q2 = add2(rr2, mk(D.px[r.b], D.py[r.b]));
+        v2 q1 = add2(rr1, pa);
+        v2 q2 = add2(rr2, pb);
         float dd = dot2(mk(r.nx, r.ny), sub2(q2, q1));
         r.p[p][4] = dd + od;
         r.p[p][0] = rr1.x; r.p[p][1] = rr1.y; r.p[p][2] = rr2.x; r.p[p][3] = rr2.y;
@@ -1446,8 +1452,10 @@ __device__ __forceinline__ void rec_joint_params(Rec& r, const Joint& J, const W
 }
 
 // the sub-step's pose of a joint record's bodies (after the previous
-// sub-step's position integration; torque as set for this sub-step)
-__device__ __forceinline__ void rec_joint_pose(Rec& r, const Joint& J, const WorldDev& D)
+// sub-step's position integration; torque as set for this sub-step);
+// DV: WorldDev or PassArrays
+template <class DV>
+__device__ __forceinline__ void rec_joint_pose(Rec& r, const Joint& J, const DV& D)
 {
     const int32_t a = J.s1, b = J.s2;
     r.p[1][3] = D.px[a]; r.p[1][4] = D.py[a];
@@ -1658,35 +1666,68 @@ __device__ __forceinline__ void body_setup(const WorldDev& D, const ST& st, int 
     st.deg[i] = degs[i];
 }
 
-// forces (lib.zig:200-205), AABB refresh + velocity integration
-// (lib.zig:207-216) for body i; clears its solver counter
-template <bool WARM = false, class ST>
-__device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int i, bool last, float sub_dt)
+// The body arrays that the per-body passes of a sub-step read and write,
+// copied out of the world's descriptor once per pass and typed by address
+// space: global for the body arrays (their loads and stores then wait on the
+// memory counter alone; a generic pointer's wait also covers LDS operations),
+// constant for the force generators, which no kernel writes (scalar loads,
+// one per generator for the whole wave).  Through a reference to the
+// descriptor in global memory the compiler reloads every pointer for every
+// body, since it cannot tell that the pass's own stores leave it unchanged.
+typedef __attribute__((address_space(1))) float* GlbF;
+typedef const __attribute__((address_space(1))) float* GlbCF;
+typedef const __attribute__((address_space(1))) uint32_t* GlbCU;
+typedef const __attribute__((address_space(1))) float4* GlbCF4;
+typedef const __attribute__((address_space(4))) Gen* ConstGen;
+struct BodyArrays {
+    GlbF px, py, ang, fx, fy, tq, ax, ay, ahw, ahh;
+    GlbCF mass, inertia, rad, hw, hh;
+    GlbCU flags;
+    GlbCF4 wsum;
+    ConstGen gens;
+    int32_t ngens;
+    float warm;
+};
+__device__ __forceinline__ BodyArrays body_arrays(const WorldDev& D)
 {
-    uint32_t f = D.flags[i];
-    bool stat = (f & BF_STATIC) != 0;
-    float fx = D.fx[i], fy = D.fy[i];
-    for (int g = 0; g < D.ngens; ++g) {
-        const Gen& G = D.gens[g];
-        if (stat) continue;
-        if (G.type == 0) {
-            fx += G.gx * D.mass[i];
-            fy += G.gy * D.mass[i];
-        } else if (G.slot == i) {
-            float inv_m = 1.0f / D.mass[i];
-            v2 v = scale2(mk(st.mx[i], st.my[i]), inv_m);
-            v2 d = sub2(mk(D.px[i], D.py[i]), mk(G.ax, G.ay));
-            fx += (-G.k) * d.x - G.c * v.x;
-            fy += (-G.k) * d.y - G.c * v.y;
+    return BodyArrays{(GlbF)D.px, (GlbF)D.py, (GlbF)D.ang, (GlbF)D.fx, (GlbF)D.fy, (GlbF)D.tq, (GlbF)D.ax,
+                      (GlbF)D.ay, (GlbF)D.ahw, (GlbF)D.ahh, (GlbCF)D.mass, (GlbCF)D.inertia, (GlbCF)D.rad,
+                      (GlbCF)D.hw, (GlbCF)D.hh, (GlbCU)D.flags, (GlbCF4)D.wsum, (ConstGen)D.gens, D.ngens, D.warm};
+}
+
+// forces (lib.zig:200-205), AABB refresh + velocity integration
+// (lib.zig:207-216) for body i; clears its solver counter.  The body's
+// inputs are loaded before any of them is used (one round trip; every index
+// is valid whatever the flags).  DV: WorldDev or BodyArrays.
+template <bool WARM = false, class ST, class DV>
+__device__ __forceinline__ void body_forces(const DV& D, const ST& st, int i, bool last, float sub_dt)
+{
+    const uint32_t f = D.flags[i];
+    const float fx0 = D.fx[i], fy0 = D.fy[i], mass = D.mass[i], px = D.px[i], py = D.py[i], tq = D.tq[i];
+    const float ang = D.ang[i], bhw = D.hw[i], bhh = D.hh[i], rad = D.rad[i];
+    const bool stat = (f & BF_STATIC) != 0;
+    float fx = fx0, fy = fy0;
+    if (!stat) {
+        for (int g = 0; g < D.ngens; ++g) {
+            const auto& G = D.gens[g];   // fields read one by one (keeps G's address space)
+            if (G.type == 0) {
+                fx += G.gx * mass;
+                fy += G.gy * mass;
+            } else if (G.slot == i) {
+                float inv_m = 1.0f / mass;
+                v2 v = scale2(mk(st.mx[i], st.my[i]), inv_m);
+                v2 d = sub2(mk(px, py), mk(G.ax, G.ay));
+                fx += (-G.k) * d.x - G.c * v.x;
+                fy += (-G.k) * d.y - G.c * v.y;
+            }
         }
     }
     if (last) {
         // Disc.updateAABB / Rectangle.updateAABB; only the last refresh is observable
-        float a = D.ang[i];
         float hw, hh;
         if (f & BF_RECT) {
-            float c = cosf_r(a), sn = sinf_r(a);
-            float x = D.hw[i], y = D.hh[i];
+            float c = cosf_r(ang), sn = sinf_r(ang);
+            float x = bhw, y = bhh;
             v2 lv[4] = {mk(-x, -y), mk(-x, y), mk(x, y), mk(x, -y)};
             hw = 0.f; hh = 0.f;
 #pragma unroll
@@ -1696,21 +1737,21 @@ __device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int
                 if (r.y > hh) hh = r.y;
             }
         } else {
-            hw = D.rad[i]; hh = D.rad[i];
+            hw = rad; hh = rad;
         }
-        D.ax[i] = D.px[i]; D.ay[i] = D.py[i]; D.ahw[i] = hw; D.ahh[i] = hh;
+        D.ax[i] = px; D.ay[i] = py; D.ahw[i] = hw; D.ahh[i] = hh;
     }
     if (!stat) {
         st.mx[i] += fx * sub_dt;
         st.my[i] += fy * sub_dt;
-        st.L[i] += D.tq[i] * sub_dt;
+        st.L[i] += tq * sub_dt;
         if constexpr (WARM) {
             if (D.warm > 0.0f) {
                 // warm starting: the body's share of the matched manifolds' carried impulses
-                const float4 ws = D.wsum[i];
-                st.mx[i] += ws.x;
-                st.my[i] += ws.y;
-                st.L[i] += ws.z;
+                const auto* ws = reinterpret_cast<decltype(&D.wsum[i].x)>(&D.wsum[i]);
+                st.mx[i] += ws[0];
+                st.my[i] += ws[1];
+                st.L[i] += ws[2];
             }
         }
     }
@@ -1718,15 +1759,18 @@ __device__ __forceinline__ void body_forces(const WorldDev& D, const ST& st, int
     st.cnt[i] = 0u;
 }
 
-// position integration (lib.zig:238-249) for body i
-template <class ST>
-__device__ __forceinline__ void body_positions(const WorldDev& D, const ST& st, int i, float sub_dt)
+// position integration (lib.zig:238-249) for body i, its inputs loaded
+// before any is used; DV: WorldDev or BodyArrays
+template <class ST, class DV>
+__device__ __forceinline__ void body_positions(const DV& D, const ST& st, int i, float sub_dt)
 {
-    if (D.flags[i] & BF_STATIC) return;
-    float qd = sub_dt / D.mass[i];
-    D.px[i] += st.mx[i] * qd;
-    D.py[i] += st.my[i] * qd;
-    D.ang[i] += st.L[i] * sub_dt / D.inertia[i];
+    const uint32_t f = D.flags[i];
+    const float mass = D.mass[i], inertia = D.inertia[i], px = D.px[i], py = D.py[i], ang = D.ang[i];
+    if (f & BF_STATIC) return;
+    float qd = sub_dt / mass;
+    D.px[i] = px + st.mx[i] * qd;
+    D.py[i] = py + st.my[i] * qd;
+    D.ang[i] = ang + st.L[i] * sub_dt / inertia;
     D.fx[i] = 0.0f; D.fy[i] = 0.0f;
     D.tq[i] = 0.0f;
 }
@@ -1864,10 +1908,24 @@ __device__ __forceinline__ void item_build(const StepDev& S, const WorldDev& D, 
     rec_store(r, recs + (size_t)pos * 8);
 }
 
+// The body arrays that the per-sub-step item pass reads, copied out of the
+// world's descriptor once per pass: through a reference to the descriptor
+// (global memory) the compiler reloads each pointer for every item, since it
+// cannot tell that the pass's own stores leave the descriptor unchanged.
+struct PassArrays {
+    const float *px, *py, *ang, *inertia, *tq;
+    const Joint* joints;
+};
+__device__ __forceinline__ PassArrays pass_arrays(const WorldDev& D)
+{
+    return PassArrays{D.px, D.py, D.ang, D.inertia, D.tq, D.joints};
+}
+
 // updateTGSDepth (collision.zig:73-100): live only for records flagged at
-// build time; every manifold's previous angles end as the last sub-step's
-template <bool WARM = false, class ST>
-__device__ __forceinline__ void item_tgs(const StepDev& S, const WorldDev& D, const ST& st, uint4* recs, uint32_t pos,
+// build time; every manifold's previous angles end as the last sub-step's.
+// DV: WorldDev or PassArrays.
+template <bool WARM = false, class ST, class DV>
+__device__ __forceinline__ void item_tgs(const StepDev& S, const DV& D, const ST& st, uint4* recs, uint32_t pos,
                                          bool last, float sub_dt)
 {
     uint4* q = recs + (size_t)pos * 8;
@@ -2288,15 +2346,18 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
         item_build<WARM>(S, D, st, recs, pos, ibase, mbeg, sub_dt);
     for (uint32_t s = 0; s < sub_steps; ++s) {
         const bool last = s + 1 == sub_steps;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces<WARM>(D, st, i, last, sub_dt);
+        const BodyArrays bf = body_arrays(D);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces<WARM>(bf, st, i, last, sub_dt);
         __syncthreads();
+        const PassArrays pa = pass_arrays(D);
         for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x)
-            item_tgs<WARM>(S, D, st, recs, pos, last, sub_dt);
+            item_tgs<WARM>(S, pa, st, recs, pos, last, sub_dt);
         __syncthreads();
         if constexpr (LANES) item_chain_lanes<false, false, WARM>(S, st, recs, lanes, nitems, iters, sub_dt, w, s_abort);
         else item_chain<true, false, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(D, st, i, sub_dt);
+        const BodyArrays bp = body_arrays(D);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(bp, st, i, sub_dt);
         __syncthreads();
     }
     for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_writeback(S, recs, pos);
