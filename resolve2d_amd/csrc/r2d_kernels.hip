@@ -2155,7 +2155,7 @@ __device__ __forceinline__ void lane_load(Rec& r, const uint4* q, uint32_t strid
 // are in LDS (lcnt) under the compact chain indices, so a poll is an LDS
 // read; acquire / release at workgroup scope order the state around them.
 template <bool COMPACT = false, bool GLOBAL = false, bool WARM = false, bool HYBRID = false, class ST>
-__device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st, uint4* recs, uint4* lanes,
+__device__ __forceinline__ void item_chain_lanes(int32_t* w_status, const ST& st, uint4* recs, uint4* lanes,
                                                  uint32_t nitems, uint32_t iters, float sub_dt, uint32_t w,
                                                  uint32_t& s_abort, uint32_t* lcnt = nullptr)
 {
@@ -2182,8 +2182,18 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 if (held != NONE && !(flags & RF_JOINT)) rec_put_acc(recs, held, acc);
                 const uint4* q = recs + (size_t)pos * 8;
                 const uint4 h0 = q[0], h1 = q[1], ac = q[REC_ACC];
-#pragma unroll
-                for (uint32_t row = 0; row < LANE_ROWS; ++row) slot[row * p_step] = q[REC_LINK + row];
+                // every row's load is issued before any is written to the
+                // slot (one round trip for the record): without the barrier
+                // the scheduler may put each row's load after the previous
+                // row's LDS write, one round trip per row
+                const uint4 w0 = q[REC_LINK], w1 = q[REC_LINK + 1], w2 = q[REC_LINK + 2], w3 = q[REC_LINK + 3],
+                            w4 = q[REC_LINK + 4];
+                __builtin_amdgcn_sched_barrier(0);
+                slot[0] = w0;
+                slot[p_step] = w1;
+                slot[2 * p_step] = w2;
+                slot[3 * p_step] = w3;
+                slot[4 * p_step] = w4;
                 if constexpr (LITE) slot[LANE_ROWS * p_step] = st.rmass[pos];
                 flags = h0.x; a = h0.y; b = h0.z; rk0 = h1.x; rk1 = h1.y; degs = h1.z;
                 if (HYBRID) cpair = h1.w;
@@ -2289,7 +2299,7 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
                 if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 if (spins > SPIN_LIMIT) {
                     __hip_atomic_store(&s_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    S.w_status[w] = 5;
+                    w_status[w] = 5;
                     break;
                 }
             }
@@ -2299,6 +2309,35 @@ __device__ __forceinline__ void item_chain_lanes(const StepDev& S, const ST& st,
         // issue ahead of it anyway
     }
     if (held != NONE && !(flags & RF_JOINT)) rec_put_acc(recs, held, acc);
+}
+
+// The LDS-state chain of a sub-step as an out-of-line call.  Its register
+// allocation is then its own: inlined, any change of register pressure
+// elsewhere in the step kernel reshaped the loop (at worst a record pickup
+// became one global round trip per row instead of one for all rows).  It
+// takes pointers typed by address space (LDS: the body state laid out as in
+// rec_world, the lane slots and the abort flag; global: the records and the
+// world status) and values only, so the callee keeps LDS and global
+// instructions and no argument block is copied.
+typedef __attribute__((address_space(3))) uint32_t* LdsU32;
+typedef __attribute__((address_space(3))) uint4* LdsU4;
+typedef __attribute__((address_space(1))) int32_t* GlbI32;
+typedef __attribute__((address_space(1))) uint4* GlbU4;
+template <bool WARM>
+__device__ __attribute__((noinline)) void chain_lds_call(GlbI32 w_status, LdsU32 lds, uint32_t n, GlbU4 recs,
+                                                        LdsU4 lanes, uint32_t nitems, uint32_t iters, float sub_dt,
+                                                        uint32_t w, LdsU32 s_abort)
+{
+    SolverState<true> st;
+    st.mx = (float*)(uint32_t*)lds;
+    st.my = st.mx + n;
+    st.L = st.my + n;
+    st.im = st.L + n;
+    st.ii = st.im + n;
+    st.cnt = (uint32_t*)(st.ii + n);
+    st.deg = st.cnt + n;
+    item_chain_lanes<false, false, WARM>((int32_t*)w_status, st, (uint4*)recs, (uint4*)lanes, nitems, iters, sub_dt, w,
+                                         *(uint32_t*)s_abort);
 }
 
 // the records' impulses, effective masses, lever arms and depths -> the
@@ -2353,7 +2392,9 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
         for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x)
             item_tgs<WARM>(S, pa, st, recs, pos, last, sub_dt);
         __syncthreads();
-        if constexpr (LANES) item_chain_lanes<false, false, WARM>(S, st, recs, lanes, nitems, iters, sub_dt, w, s_abort);
+        if constexpr (LANES)
+            chain_lds_call<WARM>((GlbI32)S.w_status, (LdsU32)lds, (uint32_t)n, (GlbU4)recs, (LdsU4)lanes, nitems, iters,
+                                 sub_dt, w, (LdsU32)&s_abort);
         else item_chain<true, false, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         const BodyArrays bp = body_arrays(D);
@@ -2649,7 +2690,7 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
                 st.cnt[c] = 0u;
             }
             __syncthreads();
-            item_chain_lanes<true, false, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+            item_chain_lanes<true, false, WARM>(S.w_status, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
                                                 s_abort);
             __syncthreads();
             for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
@@ -2675,7 +2716,7 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
         }
         __syncthreads();
         if (lanes)
-            item_chain_lanes<true, false, WARM>(S, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
+            item_chain_lanes<true, false, WARM>(S.w_status, st, recs, reinterpret_cast<uint4*>(smem), nitems, iters, sub_dt, w,
                                                 s_abort);
         else item_chain<true, true, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
@@ -2695,9 +2736,9 @@ __global__ __launch_bounds__(MAXT) void k_big_chain(StepDev S, uint32_t lds_nmax
             uint32_t* lcnt = smem + (size_t)MAXT * LANE_WORDS;
             for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) lcnt[c] = 0u;
             __syncthreads();
-            item_chain_lanes<false, true, WARM, true>(S, st, recs, slots, nitems, iters, sub_dt, w, s_abort, lcnt);
+            item_chain_lanes<false, true, WARM, true>(S.w_status, st, recs, slots, nitems, iters, sub_dt, w, s_abort, lcnt);
         } else {
-            item_chain_lanes<false, true, WARM>(S, st, recs, slots, nitems, iters, sub_dt, w, s_abort);
+            item_chain_lanes<false, true, WARM>(S.w_status, st, recs, slots, nitems, iters, sub_dt, w, s_abort);
         }
     }
 }
