@@ -1695,16 +1695,29 @@ __device__ __forceinline__ BodyArrays body_arrays(const WorldDev& D)
                       (GlbCF)D.hw, (GlbCF)D.hh, (GlbCU)D.flags, (GlbCF4)D.wsum, (ConstGen)D.gens, D.ngens, D.warm};
 }
 
-// forces (lib.zig:200-205), AABB refresh + velocity integration
-// (lib.zig:207-216) for body i; clears its solver counter.  The body's
-// inputs are loaded before any of them is used (one round trip; every index
-// is valid whatever the flags).  DV: WorldDev or BodyArrays.
-template <bool WARM = false, class ST, class DV>
-__device__ __forceinline__ void body_forces(const DV& D, const ST& st, int i, bool last, float sub_dt)
+// what body_forces reads of body i, loaded before any of it is used (one
+// round trip; every index is valid whatever the flags)
+struct ForceIn {
+    uint32_t f;
+    float fx, fy, mass, px, py, tq, ang, hw, hh, rad;
+};
+template <class DV>
+__device__ __forceinline__ ForceIn force_in(const DV& D, int i)
 {
-    const uint32_t f = D.flags[i];
-    const float fx0 = D.fx[i], fy0 = D.fy[i], mass = D.mass[i], px = D.px[i], py = D.py[i], tq = D.tq[i];
-    const float ang = D.ang[i], bhw = D.hw[i], bhh = D.hh[i], rad = D.rad[i];
+    return ForceIn{D.flags[i], D.fx[i], D.fy[i], D.mass[i], D.px[i], D.py[i], D.tq[i], D.ang[i], D.hw[i], D.hh[i],
+                   D.rad[i]};
+}
+
+// forces (lib.zig:200-205), AABB refresh + velocity integration
+// (lib.zig:207-216) for body i from its loaded inputs; clears its solver
+// counter.  DV: WorldDev or BodyArrays.
+template <bool WARM = false, class ST, class DV>
+__device__ __forceinline__ void body_forces_in(const DV& D, const ST& st, int i, const ForceIn& in, bool last,
+                                               float sub_dt)
+{
+    const uint32_t f = in.f;
+    const float fx0 = in.fx, fy0 = in.fy, mass = in.mass, px = in.px, py = in.py, tq = in.tq;
+    const float ang = in.ang, bhw = in.hw, bhh = in.hh, rad = in.rad;
     const bool stat = (f & BF_STATIC) != 0;
     float fx = fx0, fy = fy0;
     if (!stat) {
@@ -1759,13 +1772,30 @@ __device__ __forceinline__ void body_forces(const DV& D, const ST& st, int i, bo
     st.cnt[i] = 0u;
 }
 
-// position integration (lib.zig:238-249) for body i, its inputs loaded
-// before any is used; DV: WorldDev or BodyArrays
-template <class ST, class DV>
-__device__ __forceinline__ void body_positions(const DV& D, const ST& st, int i, float sub_dt)
+template <bool WARM = false, class ST, class DV>
+__device__ __forceinline__ void body_forces(const DV& D, const ST& st, int i, bool last, float sub_dt)
 {
-    const uint32_t f = D.flags[i];
-    const float mass = D.mass[i], inertia = D.inertia[i], px = D.px[i], py = D.py[i], ang = D.ang[i];
+    body_forces_in<WARM>(D, st, i, force_in(D, i), last, sub_dt);
+}
+
+// what body_positions reads of body i, loaded before any of it is used
+struct PosIn {
+    uint32_t f;
+    float mass, inertia, px, py, ang;
+};
+template <class DV>
+__device__ __forceinline__ PosIn pos_in(const DV& D, int i)
+{
+    return PosIn{D.flags[i], D.mass[i], D.inertia[i], D.px[i], D.py[i], D.ang[i]};
+}
+
+// position integration (lib.zig:238-249) for body i from its loaded inputs;
+// DV: WorldDev or BodyArrays
+template <class ST, class DV>
+__device__ __forceinline__ void body_positions_in(const DV& D, const ST& st, int i, const PosIn& in, float sub_dt)
+{
+    const uint32_t f = in.f;
+    const float mass = in.mass, inertia = in.inertia, px = in.px, py = in.py, ang = in.ang;
     if (f & BF_STATIC) return;
     float qd = sub_dt / mass;
     D.px[i] = px + st.mx[i] * qd;
@@ -1773,6 +1803,31 @@ __device__ __forceinline__ void body_positions(const DV& D, const ST& st, int i,
     D.ang[i] = ang + st.L[i] * sub_dt / inertia;
     D.fx[i] = 0.0f; D.fy[i] = 0.0f;
     D.tq[i] = 0.0f;
+}
+
+template <class ST, class DV>
+__device__ __forceinline__ void body_positions(const DV& D, const ST& st, int i, float sub_dt)
+{
+    body_positions_in(D, st, i, pos_in(D, i), sub_dt);
+}
+
+// A per-body pass over [0, n) in rounds of K bodies per thread: each round
+// loads the inputs of all its bodies before it applies the update to any, so
+// their loads are in flight together (bodies are independent in these
+// passes, so the order changes no result).
+template <int K, class In, class Load, class Apply>
+__device__ __forceinline__ void body_pass(int n, Load load, Apply apply)
+{
+    const int T = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n; i0 += K * T) {
+        In in[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (i0 + k * T < n) in[k] = load(i0 + k * T);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (i0 + k * T < n) apply(i0 + k * T, in[k]);
+    }
 }
 
 // ------------------------------------------------------------ warm starting
@@ -1924,12 +1979,12 @@ __device__ __forceinline__ PassArrays pass_arrays(const WorldDev& D)
 // updateTGSDepth (collision.zig:73-100): live only for records flagged at
 // build time; every manifold's previous angles end as the last sub-step's.
 // DV: WorldDev or PassArrays.
+// h: the record's header row (recs[pos * 8])
 template <bool WARM = false, class ST, class DV>
-__device__ __forceinline__ void item_tgs(const StepDev& S, const DV& D, const ST& st, uint4* recs, uint32_t pos,
-                                         bool last, float sub_dt)
+__device__ __forceinline__ void item_tgs_h(const StepDev& S, const DV& D, const ST& st, uint4* recs, uint32_t pos,
+                                           uint4 h, bool last, float sub_dt)
 {
     uint4* q = recs + (size_t)pos * 8;
-    uint4 h = q[0];
     if (h.x & RF_JOINT) {
         Rec r;
         rec_load(r, q);
@@ -1962,6 +2017,13 @@ __device__ __forceinline__ void item_tgs(const StepDev& S, const DV& D, const ST
         S.m_prev1[h.w] = D.ang[h.y];
         S.m_prev2[h.w] = D.ang[h.z];
     }
+}
+
+template <bool WARM = false, class ST, class DV>
+__device__ __forceinline__ void item_tgs(const StepDev& S, const DV& D, const ST& st, uint4* recs, uint32_t pos,
+                                         bool last, float sub_dt)
+{
+    item_tgs_h<WARM>(S, D, st, recs, pos, recs[(size_t)pos * 8], last, sub_dt);
 }
 
 // K iterations of one sub-step's (constraints, manifolds) for the world of
@@ -2386,11 +2448,21 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
     for (uint32_t s = 0; s < sub_steps; ++s) {
         const bool last = s + 1 == sub_steps;
         const BodyArrays bf = body_arrays(D);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_forces<WARM>(bf, st, i, last, sub_dt);
+        body_pass<3, ForceIn>(
+            n, [&](int i) { return force_in(bf, i); },
+            [&](int i, const ForceIn& in) { body_forces_in<WARM>(bf, st, i, in, last, sub_dt); });
         __syncthreads();
-        const PassArrays pa = pass_arrays(D);
-        for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x)
-            item_tgs<WARM>(S, pa, st, recs, pos, last, sub_dt);
+        {
+            // the next record's header is loaded while this one is updated
+            const PassArrays pa = pass_arrays(D);
+            uint32_t pos = threadIdx.x;
+            uint4 hn = pos < nitems ? recs[(size_t)pos * 8] : make_uint4(0u, 0u, 0u, 0u);
+            for (; pos < nitems; pos += blockDim.x) {
+                const uint4 h = hn;
+                if (pos + blockDim.x < nitems) hn = recs[(size_t)(pos + blockDim.x) * 8];
+                item_tgs_h<WARM>(S, pa, st, recs, pos, h, last, sub_dt);
+            }
+        }
         __syncthreads();
         if constexpr (LANES)
             chain_lds_call<WARM>((GlbI32)S.w_status, (LdsU32)lds, (uint32_t)n, (GlbU4)recs, (LdsU4)lanes, nitems, iters,
@@ -2398,7 +2470,9 @@ __device__ __forceinline__ void rec_world(const StepDev& S, const WorldDev& D, u
         else item_chain<true, false, WARM>(S, D, st, recs, nitems, iters, sub_dt, w, s_abort);
         __syncthreads();
         const BodyArrays bp = body_arrays(D);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) body_positions(bp, st, i, sub_dt);
+        body_pass<3, PosIn>(
+            n, [&](int i) { return pos_in(bp, i); },
+            [&](int i, const PosIn& in) { body_positions_in(bp, st, i, in, sub_dt); });
         __syncthreads();
     }
     for (uint32_t pos = threadIdx.x; pos < nitems; pos += blockDim.x) item_writeback(S, recs, pos);
