@@ -1677,13 +1677,12 @@ __device__ __forceinline__ void body_setup(const WorldDev& D, const ST& st, int 
 typedef __attribute__((address_space(1))) float* GlbF;
 typedef const __attribute__((address_space(1))) float* GlbCF;
 typedef const __attribute__((address_space(1))) uint32_t* GlbCU;
-typedef const __attribute__((address_space(1))) float4* GlbCF4;
 typedef const __attribute__((address_space(4))) Gen* ConstGen;
 struct BodyArrays {
     GlbF px, py, ang, fx, fy, tq, ax, ay, ahw, ahh;
     GlbCF mass, inertia, rad, hw, hh;
     GlbCU flags;
-    GlbCF4 wsum;
+    GlbCF wsum;                      // the world's float4 warm sums, as floats
     ConstGen gens;
     int32_t ngens;
     float warm;
@@ -1692,8 +1691,17 @@ __device__ __forceinline__ BodyArrays body_arrays(const WorldDev& D)
 {
     return BodyArrays{(GlbF)D.px, (GlbF)D.py, (GlbF)D.ang, (GlbF)D.fx, (GlbF)D.fy, (GlbF)D.tq, (GlbF)D.ax,
                       (GlbF)D.ay, (GlbF)D.ahw, (GlbF)D.ahh, (GlbCF)D.mass, (GlbCF)D.inertia, (GlbCF)D.rad,
-                      (GlbCF)D.hw, (GlbCF)D.hh, (GlbCU)D.flags, (GlbCF4)D.wsum, (ConstGen)D.gens, D.ngens, D.warm};
+                      (GlbCF)D.hw, (GlbCF)D.hh, (GlbCU)D.flags, (GlbCF)(const float*)D.wsum, (ConstGen)D.gens,
+                      D.ngens, D.warm};
 }
+
+// body i's warm impulse per sub-step (momentum x, y, angular: the first
+// three floats of its float4)
+__device__ __forceinline__ const float* warm_sum(const WorldDev& D, int i)
+{
+    return reinterpret_cast<const float*>(&D.wsum[i]);
+}
+__device__ __forceinline__ GlbCF warm_sum(const BodyArrays& D, int i) { return D.wsum + 4 * i; }
 
 // what body_forces reads of body i, loaded before any of it is used (one
 // round trip; every index is valid whatever the flags)
@@ -1761,7 +1769,7 @@ __device__ __forceinline__ void body_forces_in(const DV& D, const ST& st, int i,
         if constexpr (WARM) {
             if (D.warm > 0.0f) {
                 // warm starting: the body's share of the matched manifolds' carried impulses
-                const auto* ws = reinterpret_cast<decltype(&D.wsum[i].x)>(&D.wsum[i]);
+                const auto ws = warm_sum(D, i);
                 st.mx[i] += ws[0];
                 st.my[i] += ws[1];
                 st.L[i] += ws[2];
